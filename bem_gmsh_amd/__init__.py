@@ -1,0 +1,27 @@
+"""MI355X-native dense 2-D Laplace BEM path of BEM_gmsh.
+
+    dad = format2d(quadrado(ndiv=20), Laplace(1.0))
+    attach_analytical(dad, ana_laplace_linear())
+    assemble(dad, 20)          # H, G on the GPU (libbemhip.so)
+    solve(dad)                 # FP64 LU on the GPU -> dad.T, dad.q
+    rel_error(dad)
+"""
+
+from .problem import BEMData, Element, Laplace
+from .mesh import (Mesh2D, SplitMix64, ellipse_mesh, format2d, formatdata, n_eff, quadrado,
+                   read_msh, star_mesh, star_params, write_msh)
+from .analytical import (AnalyticalSolution, ana_exp_cos, ana_laplace_linear,
+                         ana_quadratic_mixed, analytical, analytical_flux, apply_analytical_bc,
+                         apply_bc_from_physical, attach_analytical, rel_error, rel_error_flux)
+from .api import (ArgumentError, HipContext, apply_bc, apply_operator, assemble, get_HG,
+                  lu_factor, lu_solve, near_pairs, solve, split_sol)
+
+__all__ = [
+    "BEMData", "Element", "Laplace", "Mesh2D", "SplitMix64", "ellipse_mesh", "format2d",
+    "formatdata", "n_eff", "quadrado", "read_msh", "star_mesh", "star_params", "write_msh",
+    "AnalyticalSolution", "ana_exp_cos", "ana_laplace_linear", "ana_quadratic_mixed",
+    "analytical", "analytical_flux", "apply_analytical_bc", "apply_bc_from_physical",
+    "attach_analytical", "rel_error", "rel_error_flux", "ArgumentError", "HipContext",
+    "apply_bc", "apply_operator", "assemble", "get_HG", "lu_factor", "lu_solve", "near_pairs",
+    "solve", "split_sol",
+]

@@ -1,0 +1,231 @@
+"""``assemble`` / ``apply_bc`` / ``solve`` on the HIP library.
+
+Python counterparts of the reference's student-facing calls:
+
+* ``assemble(dad, npg=20, method="hip", precision="fp64")`` —
+  ``assemble!(dad; method, npg)`` (src/Core/Assembly_full.jl:107-118). The
+  method symbol is the plugin seam; ``"hip"`` is this library
+  (``"dense"`` is an alias). Stores ``dad.H``, ``dad.G``, ``qsi``, ``w``,
+  ``near_factor`` in the cache like H_G_full_direct (:61-85).
+* ``apply_bc(dad)`` — ``applyBC`` (src/Core/Boundary_conditions.jl:20-75).
+* ``solve(dad, mode="fp64")`` — ``solve`` (src/Core/Solver.jl:24-64) with
+  ``split_sol!`` (:88-98); sets ``dad.T`` (nt) and ``dad.q`` (n).
+
+Matrices are returned to the host lazily: the device copies stay resident in
+the context (``dad.hip``); ``dad.H`` / ``dad.G`` are fetched on first access
+unless ``fetch=False``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, fptr, i64ptr, iptr, lib
+from .problem import BEMData
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+MODES = {"fp64": 0, "mixed": 1}
+
+
+class HipContext:
+    """Owns one bemhip_ctx (device buffers, factors) bound to a GPU."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        cnt = C.c_int(0)
+        if L.bemhip_device_count(C.byref(cnt)) != 0 or cnt.value == 0:
+            raise ArgumentError("assemble(method='hip') needs a visible GPU; none was found")
+        h = C.c_void_p()
+        check(L.bemhip_create(C.byref(h), int(device)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().bemhip_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, status: int):
+        check(status, self.h)
+
+    def counters(self) -> dict:
+        buf = C.create_string_buffer(1 << 16)
+        self.check(lib().bemhip_counters(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+
+class ArgumentError(ValueError):
+    """Invalid option (the reference throws ArgumentError)."""
+
+
+def _context(dad: BEMData, device: int = 0) -> HipContext:
+    if dad.hip is None or dad.hip.device != device:
+        dad.hip = HipContext(device)
+    return dad.hip
+
+
+def upload_geometry(dad: BEMData, ctx: HipContext) -> None:
+    nn = dad.p + 1
+    coll = np.ascontiguousarray(dad.collocation, dtype=np.float64)
+    nrm = np.ascontiguousarray(dad.Normal, dtype=np.float64)
+    J = np.ascontiguousarray(dad.node_J(), dtype=np.float64)
+    ew = np.ascontiguousarray(dad.elem_weight, dtype=np.float64)
+    geo = np.ascontiguousarray(dad.elem_geo.reshape(-1, 2), dtype=np.float64)
+    elen = np.ascontiguousarray(dad.elem_L, dtype=np.float64)
+    ctx.check(lib().bemhip_set_geometry(ctx.h, dad.p, dad.ne, dad.n, dad.ni, dptr(coll),
+                                        dptr(nrm), dptr(J), dptr(ew), dptr(geo), dptr(elen),
+                                        float(dad.properties.k)))
+
+
+def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = "fp64",
+             near_factor: Optional[float] = 1.5, device: int = 0, fetch: bool = True):
+    """Assemble H (nt x nt) and G (nt x n) on the GPU; returns (H, G)."""
+    if method not in ("hip", "dense", "gpu"):
+        raise ArgumentError(f"assemble method must be :hip (aliases :dense, :gpu); got {method}")
+    if precision not in PRECISIONS:
+        raise ArgumentError(f"precision must be 'fp64' or 'mixed'; got {precision!r}")
+    if near_factor is None or near_factor == "auto":
+        near_factor = 1.5
+    if not isinstance(near_factor, (int, float)):
+        raise ArgumentError(f"near_factor must be a Real, :auto, or nothing; got {near_factor!r}")
+    nf = float(near_factor)
+    if nf == float("inf"):
+        nf = 1e300
+    ctx = _context(dad, device)
+    upload_geometry(dad, ctx)
+    ctx.check(lib().bemhip_assemble(ctx.h, int(npg), nf, PRECISIONS[precision]))
+    qsi, w = _lib.gauss_legendre(int(npg))
+    for key in ("H", "G", "A", "B", "b", "T", "q"):
+        dad.cache.pop(key, None)
+    dad.set_cache(qsi=qsi, w=w, near_factor=float(near_factor),
+                  assembly_info={"precision": precision, "npg": int(npg)})
+    if fetch:
+        H, G = get_HG(dad)
+        return H, G
+    return None
+
+
+def get_HG(dad: BEMData):
+    """Copy H, G from the device into the cache (column-major nt x nt, nt x n)."""
+    ctx = dad.hip
+    prec = dad.assembly_info["precision"]
+    nt, n = dad.nt, dad.n
+    if prec == "fp64":
+        H = np.empty((nt, nt), dtype=np.float64, order="F")
+        G = np.empty((nt, n), dtype=np.float64, order="F")
+        ctx.check(lib().bemhip_get_HG(ctx.h, dptr(H), dptr(G)))
+    else:
+        H = np.empty((nt, nt), dtype=np.float32, order="F")
+        G = np.empty((nt, n), dtype=np.float32, order="F")
+        ctx.check(lib().bemhip_get_HG_f32(ctx.h, fptr(H), fptr(G)))
+    dad.set_cache(H=H, G=G)
+    return H, G
+
+
+def near_pairs(dad: BEMData):
+    """(rows, elems, h, g) of the near / on-element pairs of the last assembly."""
+    ctx = dad.hip
+    cnt = C.c_int64(0)
+    ctx.check(lib().bemhip_near_pairs(ctx.h, C.byref(cnt)))
+    m = cnt.value
+    nn = dad.p + 1
+    rows = np.empty(m, dtype=np.int32)
+    elems = np.empty(m, dtype=np.int32)
+    h = np.empty((m, nn), dtype=np.float64)
+    g = np.empty((m, nn), dtype=np.float64)
+    if m:
+        ctx.check(lib().bemhip_get_near_pairs(ctx.h, iptr(rows), iptr(elems), dptr(h), dptr(g)))
+    return rows, elems, h, g
+
+
+def apply_bc(dad: BEMData, fetch: bool = False) -> None:
+    """Mixed-BC column swap on the device; b (and optionally A) to the cache."""
+    ctx = dad.hip
+    if ctx is None:
+        raise RuntimeError("cache.H is not set — assemble/solve first?")
+    BC = np.ascontiguousarray(dad.BC, dtype=np.int64)
+    BV = np.ascontiguousarray(dad.BV, dtype=np.float64)
+    b = np.empty(dad.nt, dtype=np.float64)
+    if fetch:
+        A = np.empty((dad.nt, dad.nt), dtype=np.float64, order="F")
+        ctx.check(lib().bemhip_apply_bc(ctx.h, i64ptr(BC), dptr(BV), dptr(A), dptr(b)))
+        dad.set_cache(A=A)
+    else:
+        ctx.check(lib().bemhip_apply_bc(ctx.h, i64ptr(BC), dptr(BV), None, dptr(b)))
+    dad.set_cache(b=b)
+
+
+def split_sol(dad: BEMData, x: np.ndarray):
+    """split_sol! (src/Core/Solver.jl:88-98): x -> (T over nt, q over n)."""
+    T = x[: dad.nt].copy()
+    q = np.zeros(dad.n)
+    dirichlet = dad.BC == 0
+    q[dirichlet] = T[: dad.n][dirichlet]
+    T[: dad.n][dirichlet] = dad.BV[dirichlet]
+    q[~dirichlet] = dad.BV[~dirichlet]
+    return T, q
+
+
+def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: int = 30):
+    """Apply BCs and solve on the GPU. ``mode`` 'fp64' (FP64 LU) or 'mixed'
+    (FP32 LU + FP64 refinement); default follows the assembly precision."""
+    if mode is None:
+        mode = "fp64" if dad.assembly_info["precision"] == "fp64" else "mixed"
+    if mode not in MODES:
+        raise ArgumentError(f"mode must be 'fp64' or 'mixed'; got {mode!r}")
+    apply_bc(dad)
+    ctx = dad.hip
+    x = np.empty(dad.nt, dtype=np.float64)
+    it = C.c_int(0)
+    rel = C.c_double(0.0)
+    ctx.check(lib().bemhip_solve(ctx.h, MODES[mode], float(tol), int(maxit), dptr(x),
+                                 C.byref(it), C.byref(rel)))
+    T, q = split_sol(dad, x)
+    dad.set_cache(T=T, q=q, solve_info={"mode": mode, "iters": it.value, "relres": rel.value,
+                                        "x": x})
+    return dad.T
+
+
+def apply_operator(dad: BEMData, x: np.ndarray) -> np.ndarray:
+    """y = A x in FP64 for the current BC system (verification helper)."""
+    ctx = dad.hip
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    ctx.check(lib().bemhip_apply_operator(ctx.h, dptr(x), dptr(y)))
+    return y
+
+
+def lu_factor(A: np.ndarray, ctx: Optional[HipContext] = None):
+    """Dense LU with partial pivoting on the GPU of a host matrix (float64/float32).
+    Returns (LU, ipiv) in the LAPACK convention (0-based ipiv)."""
+    ctx = ctx or HipContext(0)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ArgumentError("lu_factor: square matrix required")
+    dtype = 0 if A.dtype == np.float64 else 1
+    LU = np.array(A, dtype=np.float64 if dtype == 0 else np.float32, order="F", copy=True)
+    ipiv = np.empty(A.shape[0], dtype=np.int32)
+    ctx.check(lib().bemhip_lu_factor(ctx.h, A.shape[0], dtype, LU.ctypes.data_as(C.c_void_p),
+                                     iptr(ipiv)))
+    return LU, ipiv
+
+
+def lu_solve(LU: np.ndarray, ipiv: np.ndarray, b: np.ndarray,
+             ctx: Optional[HipContext] = None) -> np.ndarray:
+    ctx = ctx or HipContext(0)
+    dtype = 0 if LU.dtype == np.float64 else 1
+    LUc = np.asfortranarray(LU)
+    x = np.array(b, dtype=np.float64, copy=True)
+    ctx.check(lib().bemhip_lu_solve(ctx.h, LU.shape[0], dtype, LUc.ctypes.data_as(C.c_void_p),
+                                    iptr(np.ascontiguousarray(ipiv, dtype=np.int32)), dptr(x), 0))
+    return x

@@ -1,0 +1,1110 @@
+// Influence-matrix assembly for the dense 2-D Laplace BEM path (gfx950).
+//
+// Kernels (one launch each, all on the context stream):
+//   far_kernel      rows x 32-element column blocks: near/far classification of
+//                   every (row, element) pair (src/Core/Assembly_full.jl:736-754),
+//                   far nodal lumping (:801-810) written column-major with lanes
+//                   over rows (coalesced), a near bit per pair, FP64 row sums.
+//   near_count / near_scan / near_fill   near bits -> row-sorted CSR of pairs.
+//   near_kernel     one wavefront per near pair: tan-p3c rule for off-element
+//                   pairs (:649-695, src/Core/Integration.jl:68-93, :1117-1365),
+//                   interpolated Guiggiani for on-element pairs (:397-424,
+//                   src/Core/Integration.jl:1597-1845); lanes over quadrature
+//                   points, wave reductions.
+//   diag_kernel     rigid-body diagonal H_ii = -sum_{j!=i} H_ij (:384-390).
+//   op64_kernel / op64_finalize   FP64 operator passes that recompute the far
+//                   entries on the fly (row sums, right-hand side, A*x).
+//
+// This file is compiled with -ffp-contract=off: the FP64 parity path keeps
+// the un-fused operation order of the reference's FP64 arithmetic.
+
+#include "common.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace bemhip {
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kEps = 2.220446049250313e-16;
+
+// ---------------------------------------------------------------------------
+// Classification of one (row, element) pair. rx/ry: node - source; d2 per node.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dist_chord(double px, double py, double ax, double ay,
+                                             double bx, double by) {
+  double abx = bx - ax, aby = by - ay;
+  double t = ((px - ax) * abx + (py - ay) * aby) / ((abx * abx + aby * aby) + kEps);
+  t = t > 1.0 ? 1.0 : (t < 0.0 ? 0.0 : t);
+  double qx = ax + t * abx, qy = ay + t * aby;
+  double dx = px - qx, dy = py - qy;
+  return sqrt(dx * dx + dy * dy);
+}
+
+template <int NN>
+__device__ __forceinline__ bool is_near(double px, double py, const double* cx,
+                                        const double* cy, const double* d2, double L,
+                                        double nf) {
+  if (nf > 1e20) return true;  // includes Inf
+  double lim = nf * L;
+  double lim2 = lim * lim;
+  double dn2 = INFINITY;
+  double lox = cx[0], loy = cy[0], hix = cx[0], hiy = cy[0];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    lox = fmin(lox, cx[k]); loy = fmin(loy, cy[k]);
+    hix = fmax(hix, cx[k]); hiy = fmax(hiy, cy[k]);
+    if (d2[k] < dn2) dn2 = d2[k];
+  }
+  if (dn2 < lim2) return true;
+  double s = 0.0;
+  if (px < lox) { double d = lox - px; s += d * d; }
+  else if (px > hix) { double d = px - hix; s += d * d; }
+  if (py < loy) { double d = loy - py; s += d * d; }
+  else if (py > hiy) { double d = py - hiy; s += d * d; }
+  if (sqrt(s) >= lim) return false;
+  double d = INFINITY;
+#pragma unroll
+  for (int k = 0; k + 1 < NN; ++k)
+    d = fmin(d, dist_chord(px, py, cx[k], cy[k], cx[k + 1], cy[k + 1]));
+  return d < lim;
+}
+
+struct Consts {
+  double cU;  // 0.5/(2 pi k)
+  double cT;  // 1/(2 pi)
+  double nf;  // near factor
+};
+
+// ---------------------------------------------------------------------------
+// Far-field tile kernel.
+// Block = 256 rows x 32 elements. Element data staged in LDS; each lane owns
+// one row i, so every store H[i + j*nt] of a wave is 64 consecutive entries.
+// ---------------------------------------------------------------------------
+template <typename TS, int NN>
+__global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* __restrict__ H,
+                                                          TS* __restrict__ G,
+                                                          uint32_t* __restrict__ nearbits,
+                                                          double* __restrict__ hpart) {
+  __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
+      s_jw[kEB * NN], s_L[kEB];
+  const int cb = blockIdx.y;
+  const int e0 = cb * kEB;
+  const int ecount = min(kEB, g.ne - e0);
+  for (int t = threadIdx.x; t < ecount * NN; t += blockDim.x) {
+    const int j = e0 * NN + t;
+    s_cx[t] = g.x[j]; s_cy[t] = g.y[j];
+    s_nx[t] = g.nx[j]; s_ny[t] = g.ny[j];
+    s_jw[t] = g.jw[j];
+  }
+  for (int t = threadIdx.x; t < ecount; t += blockDim.x) s_L[t] = g.elen[e0 + t];
+  __syncthreads();
+  const int i = blockIdx.x * kFarThreads + threadIdx.x;
+  if (i >= g.nt) return;
+  const double px = g.x[i], py = g.y[i];
+  const size_t ld = (size_t)g.nt;
+  double rowsum = 0.0;
+  uint32_t bits = 0u;
+  for (int el = 0; el < ecount; ++el) {
+    const double* cx = s_cx + el * NN;
+    const double* cy = s_cy + el * NN;
+    double rx[NN], ry[NN], d2[NN];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      rx[k] = cx[k] - px;
+      ry[k] = cy[k] - py;
+      d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
+    }
+    const int col0 = (e0 + el) * NN;
+    const bool onel = (i >= col0) && (i < col0 + NN);
+    const bool nr = onel || is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf);
+    TS* Hc = H + (size_t)col0 * ld + i;
+    TS* Gc = G + (size_t)col0 * ld + i;
+    if (nr) {
+      bits |= 1u << el;
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        Hc[k * ld] = TS(0);
+        Gc[k * ld] = TS(0);
+      }
+    } else if constexpr (sizeof(TS) == 8) {
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        const double U = -log(d2[k]) * cs.cU;
+        const double T = (rx[k] * s_nx[el * NN + k] + ry[k] * s_ny[el * NN + k]) *
+                         (cs.cT / d2[k]);
+        const double w = s_jw[el * NN + k];
+        const double h = T * w;
+        Hc[k * ld] = h;
+        Gc[k * ld] = U * w;
+        rowsum += h;
+      }
+    } else {
+      // Mixed precision: the regular far quadrature in FP32.
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        const float fx = (float)rx[k], fy = (float)ry[k];
+        const float r2 = fx * fx + fy * fy;
+        const float U = -logf(r2) * (float)cs.cU;
+        const float T = (fx * (float)s_nx[el * NN + k] + fy * (float)s_ny[el * NN + k]) *
+                        ((float)cs.cT / r2);
+        const float w = (float)s_jw[el * NN + k];
+        Hc[k * ld] = T * w;
+        Gc[k * ld] = U * w;
+      }
+    }
+  }
+  nearbits[(size_t)cb * ld + i] = bits;
+  if (hpart) hpart[(size_t)cb * ld + i] = rowsum;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 operator pass: for every far pair recompute H_ij, G_ij exactly as
+// far_kernel<double> and accumulate acc1 += H wH1_j + G wG1_j (and acc2 with
+// wH2, wG2 when nvec == 2). Partial sums per 32-element column block.
+// ---------------------------------------------------------------------------
+template <int NN>
+__global__ __launch_bounds__(kFarThreads) void op64_kernel(
+    Geo g, Consts cs, int nvec, const double* __restrict__ wH1, const double* __restrict__ wG1,
+    const double* __restrict__ wH2, const double* __restrict__ wG2, double* __restrict__ part1,
+    double* __restrict__ part2) {
+  __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
+      s_jw[kEB * NN], s_L[kEB];
+  __shared__ double s_w[4][kEB * NN];
+  const int cb = blockIdx.y;
+  const int e0 = cb * kEB;
+  const int ecount = min(kEB, g.ne - e0);
+  for (int t = threadIdx.x; t < ecount * NN; t += blockDim.x) {
+    const int j = e0 * NN + t;
+    s_cx[t] = g.x[j]; s_cy[t] = g.y[j];
+    s_nx[t] = g.nx[j]; s_ny[t] = g.ny[j];
+    s_jw[t] = g.jw[j];
+    s_w[0][t] = wH1[j];
+    s_w[1][t] = wG1[j];
+    if (nvec == 2) { s_w[2][t] = wH2[j]; s_w[3][t] = wG2[j]; }
+  }
+  for (int t = threadIdx.x; t < ecount; t += blockDim.x) s_L[t] = g.elen[e0 + t];
+  __syncthreads();
+  const int i = blockIdx.x * kFarThreads + threadIdx.x;
+  if (i >= g.nt) return;
+  const double px = g.x[i], py = g.y[i];
+  double acc1 = 0.0, acc2 = 0.0;
+  for (int el = 0; el < ecount; ++el) {
+    const double* cx = s_cx + el * NN;
+    const double* cy = s_cy + el * NN;
+    double rx[NN], ry[NN], d2[NN];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      rx[k] = cx[k] - px;
+      ry[k] = cy[k] - py;
+      d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
+    }
+    const int col0 = (e0 + el) * NN;
+    const bool onel = (i >= col0) && (i < col0 + NN);
+    if (onel || is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf)) continue;
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      const int t = el * NN + k;
+      const double U = -log(d2[k]) * cs.cU;
+      const double T = (rx[k] * s_nx[t] + ry[k] * s_ny[t]) * (cs.cT / d2[k]);
+      const double w = s_jw[t];
+      const double h = T * w, gg = U * w;
+      acc1 += h * s_w[0][t] + gg * s_w[1][t];
+      if (nvec == 2) acc2 += h * s_w[2][t] + gg * s_w[3][t];
+    }
+  }
+  const size_t ld = (size_t)g.nt;
+  part1[(size_t)cb * ld + i] = acc1;
+  if (nvec == 2) part2[(size_t)cb * ld + i] = acc2;
+}
+
+// Finalize an operator pass for row i:
+//   mode 0 (row sums):  diag_i = -(far + near H entries except column i)
+//                       written to diag[] (and to H's diagonal when given);
+//   mode 1 (apply):     y_i = far + near (self column: G_ii wG_i + diag_i wH_i)
+//                       + diag_i wH_i for internal rows.
+template <typename TS>
+__global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode,
+                              const double* __restrict__ part, const int* __restrict__ row_ptr,
+                              const int* __restrict__ pair_elem,
+                              const double* __restrict__ pair_h,
+                              const double* __restrict__ pair_g, const double* __restrict__ wH,
+                              const double* __restrict__ wG, double* __restrict__ diag,
+                              double* __restrict__ y, TS* __restrict__ Hdiag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  double s = 0.0;
+  for (int cb = 0; cb < ncb; ++cb) s += part[(size_t)cb * nt + i];
+  const int p0 = row_ptr[i], p1 = row_ptr[i + 1];
+  if (mode == 0) {
+    for (int pr = p0; pr < p1; ++pr) {
+      const int c0 = pair_elem[pr] * nn;
+      for (int k = 0; k < nn; ++k)
+        if (c0 + k != i) s += pair_h[(size_t)pr * nn + k];
+    }
+    const double d = -s;
+    diag[i] = d;
+    if (Hdiag) Hdiag[(size_t)i * nt + i] = TS(d);
+  } else {
+    for (int pr = p0; pr < p1; ++pr) {
+      const int c0 = pair_elem[pr] * nn;
+      for (int k = 0; k < nn; ++k) {
+        const int col = c0 + k;
+        const double h = (col == i) ? diag[i] : pair_h[(size_t)pr * nn + k];
+        s += h * wH[col] + pair_g[(size_t)pr * nn + k] * wG[col];
+      }
+    }
+    if (i >= n) s += diag[i] * wH[i];
+    y[i] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Near-pair CSR from the bit matrix [word][row].
+// ---------------------------------------------------------------------------
+__global__ void near_count(int nt, int nwords, const uint32_t* __restrict__ bits,
+                           int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  int c = 0;
+  for (int w = 0; w < nwords; ++w) c += __popc(bits[(size_t)w * nt + i]);
+  cnt[i] = c;
+}
+
+// Exclusive scan of cnt[0..nt) into ptr[0..nt] with one workgroup of 1024.
+__global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict__ cnt,
+                                                  int* __restrict__ ptr) {
+  __shared__ int s[1024];
+  const int tid = threadIdx.x;
+  const int per = (nt + 1023) / 1024;
+  const int lo = min(nt, tid * per), hi = min(nt, lo + per);
+  int local = 0;
+  for (int r = lo; r < hi; ++r) local += cnt[r];
+  s[tid] = local;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = (tid >= off) ? s[tid - off] : 0;
+    __syncthreads();
+    s[tid] += v;
+    __syncthreads();
+  }
+  int run = s[tid] - local;
+  for (int r = lo; r < hi; ++r) {
+    ptr[r] = run;
+    run += cnt[r];
+  }
+  if (tid == 1023) ptr[nt] = s[1023];
+}
+
+__global__ void near_fill(int nt, int nwords, const uint32_t* __restrict__ bits,
+                          const int* __restrict__ ptr, int* __restrict__ prow,
+                          int* __restrict__ pelem) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  int off = ptr[i];
+  for (int w = 0; w < nwords; ++w) {
+    uint32_t b = bits[(size_t)w * nt + i];
+    while (b) {
+      const int t = __ffs(b) - 1;
+      b &= b - 1u;
+      prow[off] = i;
+      pelem[off] = w * 32 + t;
+      ++off;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Near-singular and singular element integrals: device math.
+// ---------------------------------------------------------------------------
+struct Cx {
+  double re, im;
+};
+__device__ __forceinline__ Cx cadd(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Cx csub(Cx a, Cx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ Cx cmul(Cx a, Cx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ Cx cscale(Cx a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ Cx cdiv(Cx a, Cx b) {  // Smith's algorithm
+  if (fabs(b.re) >= fabs(b.im)) {
+    double r = b.im / b.re, d = b.re + b.im * r;
+    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  double r = b.re / b.im, d = b.re * r + b.im;
+  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+__device__ __forceinline__ double cabs(Cx a) { return hypot(a.re, a.im); }
+__device__ __forceinline__ Cx csqrt_(Cx z) {  // principal branch
+  if (z.re == 0.0 && z.im == 0.0) return {0.0, z.im};
+  double r = hypot(z.re, z.im);
+  if (z.re >= 0.0) {
+    double t = sqrt(0.5 * (r + z.re));
+    return {t, z.im / (2.0 * t)};
+  }
+  double t = sqrt(0.5 * (r - z.re));
+  return {fabs(z.im) / (2.0 * t), copysign(t, z.im)};
+}
+
+__device__ __forceinline__ double clampd(double x, double lo, double hi) {
+  return x > hi ? hi : (x < lo ? lo : x);
+}
+
+// Barycentric interpolation row (src/Core/Interpolation.jl:148-178), NN <= 4.
+template <int NN>
+__device__ __forceinline__ void interp_row(const double* x0, const double* bw, double xx,
+                                           double* M) {
+  double msum = 0.0;
+  int exact = -1;
+#pragma unroll
+  for (int i = 0; i < NN; ++i) {
+    if (xx == x0[i]) exact = i;
+    M[i] = bw[i] / (xx - x0[i]);
+    msum += M[i];
+  }
+  if (msum == 0.0) {
+#pragma unroll
+    for (int i = 0; i < NN; ++i) M[i] = 0.0;
+  } else if (exact >= 0) {
+#pragma unroll
+    for (int i = 0; i < NN; ++i) M[i] = (i == exact) ? 1.0 : 0.0;
+  } else {
+#pragma unroll
+    for (int i = 0; i < NN; ++i) M[i] = M[i] / msum;
+  }
+}
+
+// shapefun: N and dN = N * D (src/Core/Interpolation.jl:215-218).
+template <int NN>
+__device__ __forceinline__ void shapefun(const double* x0, const double* bw, const double* D,
+                                         double xx, double* N, double* dN) {
+  interp_row<NN>(x0, bw, xx, N);
+#pragma unroll
+  for (int j = 0; j < NN; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NN; ++i) s += N[i] * D[i * NN + j];
+    dN[j] = s;
+  }
+}
+
+// Complex pole zeta + i eta of z(xi) through the collocation nodes
+// (src/Core/Integration.jl:1117-1160) for p = 1, 2.
+template <int NN>
+__device__ void complex_pole(const double* lx, const double* cx, const double* cy, double px,
+                             double py, double& zeta, double& eta) {
+  const Cx y = {px, py};
+  if (NN == 2) {
+    const double d = lx[1] - lx[0] + 1e-30;
+    const Cx dz = {cx[1] - cx[0], cy[1] - cy[0]};
+    const Cx B = {dz.re / d, dz.im / d};
+    const Cx C = csub(Cx{cx[0], cy[0]}, cscale(B, lx[0]));
+    if (cabs(B) < 1e-30) { zeta = 0.0; eta = 1.0; return; }
+    const Cx r = cdiv(csub(y, C), B);
+    zeta = r.re;
+    eta = fabs(r.im);
+    return;
+  }
+  const Cx Z0 = {cx[0], cy[0]}, Z1 = {cx[1], cy[1]}, Z2 = {cx[2], cy[2]};
+  const double a11 = lx[0] * lx[0], a12 = lx[0], a21 = lx[1] * lx[1], a22 = lx[1],
+               a31 = lx[2] * lx[2], a32 = lx[2];
+  const double det = a11 * (a22 - a32) - a12 * (a21 - a31) + (a21 * a32 - a22 * a31);
+  // Cramer's rule for [xi^2 xi 1] c = Z (_fit_quad_complex, :1153-1160)
+  const Cx A = cscale(cadd(csub(cscale(Z0, a22 - a32), cscale(csub(Z1, Z2), a12)),
+                           csub(cscale(Z1, a32), cscale(Z2, a22))), 1.0 / det);
+  const Cx B = cscale(cadd(csub(cscale(csub(Z1, Z2), a11), cscale(Z0, a21 - a31)),
+                           csub(cscale(Z2, a21), cscale(Z1, a31))), 1.0 / det);
+  const Cx C = cscale(cadd(csub(cscale(csub(cscale(Z2, a22), cscale(Z1, a32)), a11),
+                                cscale(csub(cscale(Z2, a21), cscale(Z1, a31)), a12)),
+                           cscale(Z0, a21 * a32 - a22 * a31)), 1.0 / det);
+  if (cabs(A) < 1e-14) {
+    if (cabs(B) < 1e-30) { zeta = 0.0; eta = 1.0; return; }
+    const Cx r = cdiv(csub(y, C), B);
+    zeta = r.re;
+    eta = fabs(r.im);
+    return;
+  }
+  const Cx disc = csqrt_(csub(cmul(B, B), cmul(cscale(A, 4.0), csub(C, y))));
+  const Cx twoA = cscale(A, 2.0);
+  const Cx mB = {-B.re, -B.im};
+  const Cx r1 = cdiv(cadd(mB, disc), twoA);
+  const Cx r2 = cdiv(csub(mB, disc), twoA);
+  const Cx r = (fabs(r1.im) <= fabs(r2.im)) ? r1 : r2;
+  zeta = r.re;
+  eta = fabs(r.im);
+}
+
+// p3c inverse pole (src/Core/Integration.jl:1220-1266).
+__device__ void p3c_inv_pole(double zeta, double eta, double& zt_out, double& et_out) {
+  if (fabs(eta) < 1e-14) {
+    double x = zeta;
+    for (int it = 0; it < 25; ++it) {
+      const double g = (((x * x * x) - (3.0 * zeta) * (x * x)) + 3.0 * x) - zeta;
+      const double gp = ((3.0 * (x * x)) - (6.0 * zeta) * x) + 3.0;
+      if (fabs(gp) < 1e-30) break;
+      x -= g / gp;
+      if (fabs(g) < 1e-14) break;
+    }
+    zt_out = x;
+    et_out = 0.0;
+    return;
+  }
+  const double sgn = eta > 0.0 ? 1.0 : (eta < 0.0 ? -1.0 : 0.0);
+  double et = cbrt(fabs(eta)) * sgn;
+  for (int it = 0; it < 20; ++it) {
+    const double g = ((2.0 * (et * et * et)) - (3.0 * eta) * (et * et)) - eta;
+    const double gp = (6.0 * (et * et)) - (6.0 * eta) * et;
+    if (fabs(gp) < 1e-30) break;
+    et -= g / gp;
+  }
+  et += cbrt(eta) * sqrt(zeta * zeta + eta * eta);
+  double zt = zeta;
+  for (int it = 0; it < 30; ++it) {
+    const double den = (1.0 + 3.0 * (zt * zt)) + 3.0 * (et * et);
+    const double nz = ((3.0 + zt * zt) + 3.0 * (et * et)) * zt;
+    const double ne = 2.0 * (et * et * et);
+    const double f1 = nz / den - zeta;
+    const double f2 = ne / den - eta;
+    if (fabs(f1) + fabs(f2) < 1e-14) break;
+    const double denz = 6.0 * zt, dene = 6.0 * et;
+    const double nz_z = (3.0 + 3.0 * (zt * zt)) + 3.0 * (et * et);
+    const double nz_e = (6.0 * et) * zt;
+    const double ne_e = 6.0 * (et * et);
+    const double den2 = den * den;
+    const double j11 = (nz_z * den - nz * denz) / den2;
+    const double j12 = (nz_e * den - nz * dene) / den2;
+    const double j21 = (0.0 * den - ne * denz) / den2;
+    const double j22 = (ne_e * den - ne * dene) / den2;
+    const double detJ = j11 * j22 - j12 * j21;
+    if (fabs(detJ) < 1e-30) break;
+    zt -= (j22 * f1 - j12 * f2) / detJ;
+    et -= (-j21 * f1 + j11 * f2) / detJ;
+  }
+  zt_out = zt;
+  et_out = et;
+}
+
+struct P3C {
+  double zt, et, A, B;
+};
+__device__ __forceinline__ P3C p3c_make(double zeta, double eta) {
+  P3C c;
+  p3c_inv_pole(zeta, eta, c.zt, c.et);
+  const double den = (1.0 + 3.0 * (c.zt * c.zt)) + 3.0 * (c.et * c.et);
+  c.A = 3.0 / den;
+  c.B = ((3.0 + c.zt * c.zt) * c.zt) / den;
+  return c;
+}
+__device__ __forceinline__ void p3c_node(const P3C& c, double s, double w, double& x,
+                                         double& wo) {
+  const double ds = s - c.zt;
+  x = c.A * ((ds * ds * ds) / 3.0 + (c.et * c.et) * s) + c.B;
+  wo = w * (c.A * (ds * ds + c.et * c.et));
+}
+
+__device__ __forceinline__ bool eq41_collinear(double zeta, double eta, bool has_den,
+                                               double den) {
+  if (fabs(zeta) <= 1.0) return false;
+  if (fabs(eta) < 1e-8) return true;
+  if (has_den && fabs(den) < 1e-10) return true;
+  return false;
+}
+
+__device__ __forceinline__ void mobius_node(double zeta, double u, double w, double& x,
+                                            double& wo) {
+  if (fabs(zeta) <= 1.0) { x = u; wo = w; return; }
+  const double A = zeta * zeta - 1.0, B = -zeta;
+  const double d = u - B;
+  x = zeta - A / d;
+  wo = w * (A / (d * d));
+}
+
+__device__ __forceinline__ void tan_back(double zeta, double eta, double At, double Bt,
+                                         double ut, double wt, double& x, double& wo) {
+  const double halfpi = kPi / 2.0 - 1e-8;
+  const double th = clampd((ut - Bt) / At, -halfpi, halfpi);
+  const double tn = tan(th);
+  x = zeta + eta * tn;
+  wo = wt * ((eta / At) * (1.0 + tn * tn));
+}
+
+// _tangenttrans for one node (src/Core/Integration.jl:1177-1200).
+__device__ void tangent_node(double zeta, double eta, double u, double w, double& x,
+                             double& wo) {
+  if (eq41_collinear(zeta, eta, false, 0.0)) { mobius_node(zeta, u, w, x, wo); return; }
+  eta = fmax(eta, 1e-14);
+  const double tp = atan((1.0 - zeta) / eta);
+  const double tm = atan((-1.0 - zeta) / eta);
+  const double den = tp - tm;
+  if (eq41_collinear(zeta, eta, true, den)) { mobius_node(zeta, u, w, x, wo); return; }
+  const double A = (2.0 * eta) / den;
+  const double B = -(tp + tm) / den;
+  const double halfpi = kPi / 2.0 - 1e-8;
+  const double phi = clampd((eta * (u - B)) / A, -halfpi, halfpi);
+  const double tn = tan(phi);
+  x = zeta + eta * tn;
+  wo = w * (((eta * eta) / A) * (1.0 + tn * tn));
+}
+
+// Plan of the tan-p3c rule (src/Core/Integration.jl:1291-1365), uniform per pair.
+struct TanPlan {
+  int kind;       // 0 Mobius, 1 p3c only, 2 compose, 3 split
+  double zeta, eta, At, Bt;
+  P3C c1, c2;     // compose: c1; split: c1 left, c2 right
+  double mid1, h1, mid2, h2;
+  bool tan1, tan2;  // split half degenerated to the tangent map
+};
+
+__device__ void tanp3c_plan(double zeta, double eta, TanPlan& P) {
+  P.zeta = zeta;
+  P.eta = eta;
+  P.tan1 = P.tan2 = false;
+  if (eq41_collinear(zeta, eta, false, 0.0)) { P.kind = 0; return; }
+  eta = fmax(eta, 1e-14);
+  P.eta = eta;
+  const double ap = atan((1.0 - zeta) / eta);
+  const double am = atan((1.0 + zeta) / eta);
+  const double den = ap + am;
+  if (eq41_collinear(zeta, eta, true, den)) { P.kind = 0; return; }
+  if (fabs(den) < 1e-30) { P.kind = 1; P.c1 = p3c_make(zeta, eta); return; }
+  const double At = 2.0 / den;
+  const double Bt = ((-At) * (ap - am)) / 2.0;
+  const double uL = Bt - (At * kPi) / 2.0;
+  const double uR = Bt + (At * kPi) / 2.0;
+  P.At = At;
+  P.Bt = Bt;
+  if (fabs(zeta) < 1.0 && -1.0 < Bt && Bt < 1.0) {
+    P.kind = 3;
+    P.mid1 = (Bt + -1.0) / 2.0; P.h1 = (Bt - -1.0) / 2.0;
+    P.mid2 = (1.0 + Bt) / 2.0;  P.h2 = (1.0 - Bt) / 2.0;
+    P.tan1 = P.h1 <= 1e-14;
+    P.tan2 = P.h2 <= 1e-14;
+    if (!P.tan1) P.c1 = p3c_make((uL - P.mid1) / P.h1, 1e-14);
+    if (!P.tan2) P.c2 = p3c_make((uR - P.mid2) / P.h2, 1e-14);
+    return;
+  }
+  P.kind = 2;
+  P.c1 = p3c_make(zeta >= 1.0 ? uL : uR, 1e-14);
+}
+
+__device__ __forceinline__ int tanp3c_count(const TanPlan& P, int npg) {
+  return P.kind == 3 ? 2 * npg : npg;
+}
+
+// Node q of the tan-p3c rule built from the Gauss rule (u, w).
+__device__ void tanp3c_node(const TanPlan& P, int q, int npg, const double* u, const double* w,
+                            double& x, double& wo) {
+  switch (P.kind) {
+    case 0: mobius_node(P.zeta, u[q], w[q], x, wo); return;
+    case 1: p3c_node(P.c1, u[q], w[q], x, wo); return;
+    case 2: {
+      double ut, wt;
+      p3c_node(P.c1, u[q], w[q], ut, wt);
+      tan_back(P.zeta, P.eta, P.At, P.Bt, ut, wt, x, wo);
+      return;
+    }
+    default: {
+      const bool left = q < npg;
+      const int qq = left ? q : q - npg;
+      if (left ? P.tan1 : P.tan2) { tangent_node(P.zeta, P.eta, u[qq], w[qq], x, wo); return; }
+      const P3C& c = left ? P.c1 : P.c2;
+      const double mid = left ? P.mid1 : P.mid2, h = left ? P.h1 : P.h2;
+      double s, ws;
+      p3c_node(c, u[qq], w[qq], s, ws);
+      tan_back(P.zeta, P.eta, P.At, P.Bt, mid + h * s, ws * h, x, wo);
+      return;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// near_kernel: one wavefront per near pair.
+// ---------------------------------------------------------------------------
+template <typename TS, int NN>
+__global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __restrict__ qc,
+                                                   int64_t npairs, const int* __restrict__ prow,
+                                                   const int* __restrict__ pelem,
+                                                   double* __restrict__ ph,
+                                                   double* __restrict__ pg, TS* __restrict__ H,
+                                                   TS* __restrict__ G) {
+  const int64_t pair = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (pair >= npairs) return;
+  const int i = prow[pair];
+  const int e = pelem[pair];
+  const double px = g.x[i], py = g.y[i];
+  double cx[NN], cy[NN], gxx[NN], gyy[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    cx[k] = g.x[e * NN + k]; cy[k] = g.y[e * NN + k];
+    gxx[k] = g.gx[e * NN + k]; gyy[k] = g.gy[e * NN + k];
+  }
+  const double nrefx = g.nx[e * NN], nrefy = g.ny[e * NN];
+  const double* lx = qc->lp_x;
+  const double* lbw = qc->lp_bw;
+  const double* lD = qc->lp_D;
+  const double* ex = qc->eq_x;
+  const double* ebw = qc->eq_bw;
+  const double* eD = qc->eq_D;
+  const double cU = 0.5 * (1.0 / (2.0 * kPi)) / qc->k;
+  const double cT = 1.0 / (2.0 * kPi);
+  double hs[NN], gs[NN];
+  const bool singular = (i >= e * NN) && (i < (e + 1) * NN);
+
+  if (!singular) {
+    double zeta, eta;
+    complex_pole<NN>(lx, cx, cy, px, py, zeta, eta);
+    TanPlan P;
+    tanp3c_plan(zeta, eta, P);
+    const int npg = qc->npg;
+    const int nq = tanp3c_count(P, npg);
+    // normal orientation from the first rule node (src/Core/Assembly_full.jl:664-667)
+    double flip = 1.0;
+    {
+      double x0, w0;
+      tanp3c_node(P, 0, npg, qc->gl_x, qc->gl_w, x0, w0);
+      double Ng[NN], dNg[NN];
+      shapefun<NN>(ex, ebw, eD, x0, Ng, dNg);
+      double dxx = 0.0, dxy = 0.0;
+#pragma unroll
+      for (int k = 0; k < NN; ++k) { dxx += dNg[k] * gxx[k]; dxy += dNg[k] * gyy[k]; }
+      const double J = sqrt(dxx * dxx + dxy * dxy);
+      const double nx = dxy / J, ny = -(dxx / J);
+      if (nx * nrefx + ny * nrefy < 0.0) flip = -1.0;
+    }
+    double ha[NN], ga[NN];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) { ha[k] = 0.0; ga[k] = 0.0; }
+    for (int q0 = 0; q0 < nq; q0 += kWave) {
+      const int q = q0 + lane;
+      if (q < nq) {
+        double xq, wq;
+        tanp3c_node(P, q, npg, qc->gl_x, qc->gl_w, xq, wq);
+        double N[NN], dN[NN], Ng[NN], dNg[NN];
+        shapefun<NN>(lx, lbw, lD, xq, N, dN);
+        shapefun<NN>(ex, ebw, eD, xq, Ng, dNg);
+        double pgx = 0.0, pgy = 0.0, dxx = 0.0, dxy = 0.0;
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          pgx += Ng[k] * gxx[k]; pgy += Ng[k] * gyy[k];
+          dxx += dNg[k] * gxx[k]; dxy += dNg[k] * gyy[k];
+        }
+        const double rx = pgx - px, ry = pgy - py;
+        const double J = sqrt(dxx * dxx + dxy * dxy);
+        double nx = dxy / J, ny = -(dxx / J);
+        if (flip < 0.0) { nx = -nx; ny = -ny; }
+        const double R2 = rx * rx + ry * ry;
+        const double U = -log(R2) * cU;
+        const double T = (rx * nx + ry * ny) * (cT / R2);
+        const double wi = J * wq;
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          const double Nj = N[k] * wi;
+          ha[k] += T * Nj;
+          ga[k] += U * Nj;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NN; ++k) { hs[k] = wave_sum(ha[k]); gs[k] = wave_sum(ga[k]); }
+  } else {
+    // closest point on the collocation interpolant (src/Core/Integration.jl:323-349)
+    double a;
+    {
+      const double dxs = cx[NN - 1] - cx[0], dys = cy[NN - 1] - cy[0];
+      const double nd = sqrt(dxs * dxs + dys * dys);
+      double xi = (lx[NN - 1] - lx[0]) * (dxs * (px - cx[0]) + dys * (py - cy[0])) /
+                      (nd * nd + kEps) + lx[0];
+      xi = clampd(xi, -1.0, 1.0);
+      for (int it = 0; it < 25; ++it) {
+        double N[NN], dN[NN], d2N[NN];
+        shapefun<NN>(lx, lbw, lD, xi, N, dN);
+#pragma unroll
+        for (int j = 0; j < NN; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < NN; ++q) s += dN[q] * lD[q * NN + j];
+          d2N[j] = s;
+        }
+        double X = 0, Y = 0, DX = 0, DY = 0, D2X = 0, D2Y = 0;
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          X += N[k] * cx[k]; Y += N[k] * cy[k];
+          DX += dN[k] * cx[k]; DY += dN[k] * cy[k];
+          D2X += d2N[k] * cx[k]; D2Y += d2N[k] * cy[k];
+        }
+        const double rx = X - px, ry = Y - py;
+        const double res = rx * DX + ry * DY;
+        const double Jn = (DX * DX + DY * DY) + (rx * D2X + ry * D2Y);
+        if (fabs(Jn) < 1e-30) break;
+        const double d = res / Jn;
+        xi -= d;
+        if (fabs(d) < 1e-14) break;
+      }
+      if (xi < -1.0 || xi > 1.0) {
+        double N[NN], dN[NN];
+        double X0 = 0, Y0 = 0, X1 = 0, Y1 = 0;
+        shapefun<NN>(lx, lbw, lD, -1.0, N, dN);
+#pragma unroll
+        for (int k = 0; k < NN; ++k) { X0 += N[k] * cx[k]; Y0 += N[k] * cy[k]; }
+        shapefun<NN>(lx, lbw, lD, 1.0, N, dN);
+#pragma unroll
+        for (int k = 0; k < NN; ++k) { X1 += N[k] * cx[k]; Y1 += N[k] * cy[k]; }
+        const double d0 = sqrt((X0 - px) * (X0 - px) + (Y0 - py) * (Y0 - py));
+        const double d1 = sqrt((X1 - px) * (X1 - px) + (Y1 - py) * (Y1 - py));
+        xi = (d0 <= d1) ? -1.0 : 1.0;
+      }
+      a = xi;
+    }
+    a = clampd(a, __longlong_as_double(0xBFEFFFFFFFFFFFFFll),
+               __longlong_as_double(0x3FEFFFFFFFFFFFFFll));
+    // interpolated-Guiggiani rule (src/Core/Integration.jl:1654-1670)
+    const bool lane_on = lane < kNinterp;
+    const double xm = lane_on ? qc->g20_x[lane] : 0.0;
+    const bool keep = lane_on && !(fabs(xm - a) <= 1e-12);
+    const uint64_t keep_mask = __ballot(keep);
+    const int m = __popcll(keep_mask);
+    double bw = 0.0;
+    if (keep) {
+      if (m == kNinterp) {
+        bw = qc->g20_bw[lane];
+      } else {
+        double prod = 1.0;
+        for (int j = 0; j < kNinterp; ++j) {
+          const double xj = qc->g20_x[j];
+          if (j == lane || fabs(xj - a) <= 1e-12) continue;
+          prod *= xm - xj;
+        }
+        bw = 1.0 / prod;
+      }
+    }
+    const double dm = a - xm;
+    const double bnum = keep ? bw / dm : 0.0;
+    const double msum = wave_sum(bnum);
+    const double Ni = keep ? bnum * (1.0 / msum) : 0.0;
+    const double delta = xm - a;
+    // kernel samples F = (U, T) * N_k * J  (src/Core/Assembly_full.jl:629-643)
+    double Fg[NN], Fh[NN];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) { Fg[k] = 0.0; Fh[k] = 0.0; }
+    if (keep) {
+      double N[NN], dN[NN], Ng[NN], dNg[NN];
+      shapefun<NN>(lx, lbw, lD, xm, N, dN);
+      shapefun<NN>(ex, ebw, eD, xm, Ng, dNg);
+      double pgx = 0.0, pgy = 0.0, dxx = 0.0, dxy = 0.0;
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        pgx += Ng[k] * gxx[k]; pgy += Ng[k] * gyy[k];
+        dxx += dNg[k] * gxx[k]; dxy += dNg[k] * gyy[k];
+      }
+      const double J = sqrt(dxx * dxx + dxy * dxy);
+      const double rx = pgx - px, ry = pgy - py;
+      if (!(J < 1e-30) && !(sqrt(rx * rx + ry * ry) < 1e-30)) {
+        double nx = dxy / J, ny = -(dxx / J);
+        if (nx * nrefx + ny * nrefy < 0.0) { nx = -nx; ny = -ny; }
+        const double R2 = rx * rx + ry * ry;
+        const double U = -log(R2) * cU;
+        const double T = (rx * nx + ry * ny) * (cT / R2);
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          const double NjJ = N[k] * J;
+          Fg[k] = U * NjJ;
+          Fh[k] = T * NjJ;
+        }
+      }
+    }
+    // log-coefficient fit for G (order 0) and CPV coefficient for H (order -1)
+    const bool used = keep && !(fabs(delta) < 1e-14);
+    const double lg = used ? log(fabs(delta)) : 0.0;
+    const double sL = wave_sum(lg);
+    const double sL2 = wave_sum(lg * lg);
+    const int nused = __popcll(__ballot(used));
+    const double detA = sL2 * nused - sL * sL;
+    const double wq = keep ? qc->g20_w[lane] : 0.0;
+    const double hfp1 = log(fabs((1.0 - a) / (1.0 + a)));
+    const double logp = ((1.0 - a) * log(fabs(1.0 - a)) + (1.0 + a) * log(fabs(1.0 + a))) - 2.0;
+    const double lgd = keep ? log(fabs(delta)) : 0.0;
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      const double Fm1g = wave_sum(keep ? Ni * (Fg[k] * delta) : 0.0);
+      const double F1h = wave_sum(keep ? Ni * (Fh[k] * delta) : 0.0);
+      const double wi = keep ? Fg[k] - Fm1g / delta : 0.0;
+      const double sW = wave_sum(used ? wi : 0.0);
+      const double sWL = wave_sum(used ? wi * lg : 0.0);
+      double F0 = 0.0;
+      if (nused >= 2 && !(fabs(detA) < 1e-30)) F0 = (nused * sWL - sL * sW) / detA;
+      const double regg = keep ? wq * ((Fg[k] - Fm1g / delta) - F0 * lgd) : 0.0;
+      const double regh = keep ? wq * (Fh[k] - F1h / delta) : 0.0;
+      gs[k] = wave_sum(regg) + (Fm1g * hfp1 + F0 * logp);
+      hs[k] = wave_sum(regh) + F1h * hfp1;
+    }
+  }
+  if (lane < NN) {
+    double hv = hs[0], gv = gs[0];
+#pragma unroll
+    for (int k = 1; k < NN; ++k)
+      if (lane == k) { hv = hs[k]; gv = gs[k]; }
+    ph[(size_t)pair * NN + lane] = hv;
+    pg[(size_t)pair * NN + lane] = gv;
+    const size_t col = (size_t)e * NN + lane;
+    H[i + col * (size_t)g.nt] = TS(hv);
+    G[i + col * (size_t)g.nt] = TS(gv);
+  }
+}
+
+// Column swap A = H with Dirichlet columns -G (src/Core/Boundary_conditions.jl:54-64).
+template <typename TS>
+__global__ void bc_swap_kernel(int nt, int n, const int* __restrict__ bc,
+                               const TS* __restrict__ H, const TS* __restrict__ G,
+                               TS* __restrict__ A) {
+  const int j = blockIdx.y;
+  const size_t ld = (size_t)nt;
+  const bool dir = (j < n) && bc[j] == 0;
+  const TS* src = dir ? G + (size_t)j * ld : H + (size_t)j * ld;
+  TS* dst = A + (size_t)j * ld;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
+    dst[i] = dir ? -src[i] : src[i];
+}
+
+template <typename TS, int NN>
+int launch_far(bemhip_ctx* c, const Geo& g, const Consts& cs, TS* H, TS* G, double* hpart) {
+  dim3 grid((g.nt + kFarThreads - 1) / kFarThreads, (g.ne + kEB - 1) / kEB);
+  hipLaunchKernelGGL((far_kernel<TS, NN>), grid, dim3(kFarThreads), 0, c->stream, g, cs, H,
+                     G, c->nearbits.ptr, hpart);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  return BEMHIP_OK;
+}
+
+template <typename TS, int NN>
+int launch_near(bemhip_ctx* c, const Geo& g, TS* H, TS* G) {
+  if (c->npairs == 0) return BEMHIP_OK;
+  const int waves_per_block = 256 / kWave;
+  const int64_t blocks = (c->npairs + waves_per_block - 1) / waves_per_block;
+  hipLaunchKernelGGL((near_kernel<TS, NN>), dim3((unsigned)blocks), dim3(256), 0, c->stream, g,
+                     c->qc.ptr, c->npairs, c->pair_row.ptr, c->pair_elem.ptr, c->pair_h.ptr,
+                     c->pair_g.ptr, H, G);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  return BEMHIP_OK;
+}
+
+}  // namespace
+
+Geo make_geo(bemhip_ctx* c) {
+  Geo g;
+  g.p = c->p; g.nn = c->nn; g.ne = c->ne; g.n = c->n; g.ni = c->ni; g.nt = c->nt;
+  g.x = c->x.ptr; g.y = c->y.ptr; g.nx = c->nx.ptr; g.ny = c->ny.ptr; g.jw = c->jw.ptr;
+  g.elen = c->elen.ptr; g.gx = c->gx.ptr; g.gy = c->gy.ptr;
+  return g;
+}
+
+static Consts make_consts(const bemhip_ctx* c) {
+  Consts cs;
+  const double inv2pi = 1.0 / (2.0 * kPi);
+  cs.cU = (0.5 * inv2pi) / c->k;
+  cs.cT = inv2pi;
+  cs.nf = c->near_factor;
+  return cs;
+}
+
+// Full assembly: far tile kernel, near-pair CSR, near/singular kernel, diagonal.
+int assemble_impl(bemhip_ctx* c) {
+  const Geo g = make_geo(c);
+  const Consts cs = make_consts(c);
+  const size_t nt = (size_t)c->nt;
+  const int ncb = (c->ne + kEB - 1) / kEB;
+  const bool fp64 = (c->precision == BEMHIP_FP64);
+  BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
+  BEMHIP_HIP_TRY(c, c->hpart.ensure((size_t)ncb * nt));
+  BEMHIP_HIP_TRY(c, c->diag64.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->row_cnt.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->row_ptr.ensure(nt + 1));
+  if (fp64) {
+    c->H32.release(); c->G32.release();
+    BEMHIP_HIP_TRY(c, c->H64.ensure(nt * nt));
+    BEMHIP_HIP_TRY(c, c->G64.ensure(nt * (size_t)c->n));
+    if (c->ni > 0)
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + (size_t)c->n * nt, 0,
+                                       sizeof(double) * nt * c->ni, c->stream));
+  } else {
+    c->H64.release(); c->G64.release();
+    BEMHIP_HIP_TRY(c, c->H32.ensure(nt * nt));
+    BEMHIP_HIP_TRY(c, c->G32.ensure(nt * (size_t)c->n));
+    if (c->ni > 0)
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + (size_t)c->n * nt, 0,
+                                       sizeof(float) * nt * c->ni, c->stream));
+  }
+  int st;
+  {
+    ScopedTimer t(c, "assemble_far");
+    if (fp64)
+      st = (c->nn == 2) ? launch_far<double, 2>(c, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
+                        : launch_far<double, 3>(c, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+    else
+      st = (c->nn == 2) ? launch_far<float, 2>(c, g, cs, c->H32.ptr, c->G32.ptr, nullptr)
+                        : launch_far<float, 3>(c, g, cs, c->H32.ptr, c->G32.ptr, nullptr);
+    if (st) return st;
+  }
+  {
+    ScopedTimer t(c, "assemble_pairs");
+    const int tb = 256;
+    const int nb = (int)((nt + tb - 1) / tb);
+    hipLaunchKernelGGL(near_count, dim3(nb), dim3(tb), 0, c->stream, (int)nt, ncb,
+                       c->nearbits.ptr, c->row_cnt.ptr);
+    hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
+                       c->row_ptr.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    int total = 0;
+    BEMHIP_HIP_TRY(c, hipMemcpyAsync(&total, c->row_ptr.ptr + nt, sizeof(int),
+                                     hipMemcpyDeviceToHost, c->stream));
+    BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->npairs = total;
+    BEMHIP_HIP_TRY(c, c->pair_row.ensure((size_t)std::max(total, 1)));
+    BEMHIP_HIP_TRY(c, c->pair_elem.ensure((size_t)std::max(total, 1)));
+    BEMHIP_HIP_TRY(c, c->pair_h.ensure((size_t)std::max(total, 1) * c->nn));
+    BEMHIP_HIP_TRY(c, c->pair_g.ensure((size_t)std::max(total, 1) * c->nn));
+    hipLaunchKernelGGL(near_fill, dim3(nb), dim3(tb), 0, c->stream, (int)nt, ncb,
+                       c->nearbits.ptr, c->row_ptr.ptr, c->pair_row.ptr, c->pair_elem.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
+  {
+    ScopedTimer t(c, "assemble_near");
+    if (fp64)
+      st = (c->nn == 2) ? launch_near<double, 2>(c, g, c->H64.ptr, c->G64.ptr)
+                        : launch_near<double, 3>(c, g, c->H64.ptr, c->G64.ptr);
+    else
+      st = (c->nn == 2) ? launch_near<float, 2>(c, g, c->H32.ptr, c->G32.ptr)
+                        : launch_near<float, 3>(c, g, c->H32.ptr, c->G32.ptr);
+    if (st) return st;
+  }
+  {
+    ScopedTimer t(c, "assemble_diag");
+    const int tb = 256;
+    const int nb = (int)((nt + tb - 1) / tb);
+    if (fp64) {
+      hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt,
+                         c->n, c->nn, ncb, 0, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                         c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr,
+                         nullptr, c->H64.ptr);
+    } else {
+      // FP64 row sums of the far field recomputed on the fly.
+      BEMHIP_HIP_TRY(c, c->wH.ensure(nt));
+      BEMHIP_HIP_TRY(c, c->wG.ensure(nt));
+      std::vector<double> ones(nt, 1.0), zeros(nt, 0.0);
+      BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->wH.ptr, ones.data(), sizeof(double) * nt,
+                                       hipMemcpyHostToDevice, c->stream));
+      BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->wG.ptr, zeros.data(), sizeof(double) * nt,
+                                       hipMemcpyHostToDevice, c->stream));
+      dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)ncb);
+      if (c->nn == 2)
+        hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1,
+                           c->wH.ptr, c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
+      else
+        hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1,
+                           c->wH.ptr, c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
+      hipLaunchKernelGGL((op64_finalize<float>), dim3(nb), dim3(tb), 0, c->stream, (int)nt,
+                         c->n, c->nn, ncb, 0, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                         c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr,
+                         nullptr, c->H32.ptr);
+    }
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
+  return BEMHIP_OK;
+}
+
+__global__ void apply_weights_kernel(int nt, int n, const int* __restrict__ bc,
+                                     const double* __restrict__ x, double* __restrict__ wH,
+                                     double* __restrict__ wG) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nt) return;
+  const bool dir = (j < n) && bc[j] == 0;
+  wH[j] = dir ? 0.0 : x[j];
+  wG[j] = dir ? -x[j] : 0.0;
+}
+
+__global__ void rhs_weights_kernel(int nt, int n, const int* __restrict__ bc,
+                                   const double* __restrict__ bv, double* __restrict__ wH,
+                                   double* __restrict__ wG) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nt) return;
+  if (j >= n) { wH[j] = 0.0; wG[j] = 0.0; return; }
+  const bool dir = bc[j] == 0;
+  wH[j] = dir ? -bv[j] : 0.0;
+  wG[j] = dir ? 0.0 : bv[j];
+}
+
+// One FP64 operator pass y = sum_j H_ij wH_j + G_ij wG_j over all columns, with
+// the far entries recomputed, near entries from the pair list and the
+// rigid-body diagonal in column i (mode 1 of op64_finalize).
+static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, double* d_y) {
+  const Geo g = make_geo(c);
+  const Consts cs = make_consts(c);
+  const size_t nt = (size_t)c->nt;
+  const int ncb = (c->ne + kEB - 1) / kEB;
+  BEMHIP_HIP_TRY(c, c->part1.ensure((size_t)ncb * nt));
+  dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)ncb);
+  if (c->nn == 2)
+    hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1, wH, wG,
+                       nullptr, nullptr, c->part1.ptr, nullptr);
+  else
+    hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1, wH, wG,
+                       nullptr, nullptr, c->part1.ptr, nullptr);
+  const int tb = 256;
+  const int nb = (int)((nt + tb - 1) / tb);
+  hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
+                     c->nn, ncb, 1, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                     c->pair_h.ptr, c->pair_g.ptr, wH, wG, c->diag64.ptr, d_y,
+                     (double*)nullptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  return BEMHIP_OK;
+}
+
+// y = A64 x for the mixed-BC system (Dirichlet columns of A are -G).
+int operator_apply(bemhip_ctx* c, const double* d_x, double* d_y) {
+  const size_t nt = (size_t)c->nt;
+  BEMHIP_HIP_TRY(c, c->wH2.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->wG2.ensure(nt));
+  const int tb = 256, nb = (int)((nt + tb - 1) / tb);
+  hipLaunchKernelGGL(apply_weights_kernel, dim3(nb), dim3(tb), 0, c->stream, c->nt, c->n,
+                     c->d_bc.ptr, d_x, c->wH2.ptr, c->wG2.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  return operator_pass(c, c->wH2.ptr, c->wG2.ptr, d_y);
+}
+
+// applyBC on the device: b from an FP64 operator pass (identical in both
+// precisions), A from the column swap of the stored H, G.
+int apply_bc_impl(bemhip_ctx* c) {
+  const size_t nt = (size_t)c->nt;
+  BEMHIP_HIP_TRY(c, c->wH2.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->wG2.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->b64.ensure(nt));
+  const int tb = 256, nb = (int)((nt + tb - 1) / tb);
+  {
+    ScopedTimer t(c, "apply_bc_rhs");
+    hipLaunchKernelGGL(rhs_weights_kernel, dim3(nb), dim3(tb), 0, c->stream, c->nt, c->n,
+                       c->d_bc.ptr, c->d_bv.ptr, c->wH2.ptr, c->wG2.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    int st = operator_pass(c, c->wH2.ptr, c->wG2.ptr, c->b64.ptr);
+    if (st) return st;
+  }
+  {
+    ScopedTimer t(c, "apply_bc_swap");
+    dim3 grid((unsigned)std::min<size_t>((nt + 255) / 256, 64), (unsigned)nt);
+    if (c->precision == BEMHIP_FP64) {
+      c->A32.release();
+      BEMHIP_HIP_TRY(c, c->A64.ensure(nt * nt));
+      hipLaunchKernelGGL((bc_swap_kernel<double>), grid, dim3(256), 0, c->stream, c->nt, c->n,
+                         c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A64.ptr);
+    } else {
+      c->A64.release();
+      BEMHIP_HIP_TRY(c, c->A32.ensure(nt * nt));
+      hipLaunchKernelGGL((bc_swap_kernel<float>), grid, dim3(256), 0, c->stream, c->nt, c->n,
+                         c->d_bc.ptr, c->H32.ptr, c->G32.ptr, c->A32.ptr);
+    }
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
+  return BEMHIP_OK;
+}
+
+}  // namespace bemhip

@@ -1,0 +1,188 @@
+// Shared definitions of libbemhip.so: context, device buffers, constants.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/bemhip.h"
+
+namespace bemhip {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kMaxNpg = 64;        // largest near-field Gauss rule
+constexpr int kNinterp = 20;       // Guiggiani interpolation nodes (reference default)
+constexpr int kEB = 32;            // elements per far-field column block (one bit word)
+constexpr int kFarThreads = 256;   // rows per far-field workgroup
+
+// Quadrature constants shared by the near/singular kernels (device buffer).
+struct QuadConst {
+  int npg;
+  int p;
+  double gl_x[kMaxNpg];     // near-field Gauss-Legendre rule
+  double gl_w[kMaxNpg];
+  double g20_x[kNinterp];   // interpolated-Guiggiani rule (ninterp = 20)
+  double g20_w[kNinterp];
+  double g20_bw[kNinterp];  // barycentric weights 1/prod(x_i - x_j)
+  double lp_x[4];           // Legendre(p) collocation nodes
+  double lp_bw[4];          // Legendre(p) barycentric weights
+  double lp_D[16];          // Legendre(p) differentiation matrix, row-major
+  double eq_x[4];           // Equispaced(p) nodes
+  double eq_bw[4];          // Equispaced(p) barycentric weights
+  double eq_D[16];          // Equispaced(p) differentiation matrix, row-major
+  double k;                 // conductivity
+  double near_factor;
+};
+
+// Device geometry (structure of arrays).
+struct Geo {
+  int p, nn, ne, n, ni, nt;
+  const double* x;     // nt collocation x
+  const double* y;     // nt collocation y
+  const double* nx;    // n normals
+  const double* ny;
+  const double* jw;    // n: J_k * w_k (far lumping weight)
+  const double* elen;  // ne element lengths
+  const double* gx;    // ne*nn geometric nodes (equispaced order)
+  const double* gy;
+};
+
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t count = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= count && ptr) return hipSuccess;
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&ptr, n * sizeof(T));
+    if (e != hipSuccess) { ptr = nullptr; return e; }
+    count = n;
+    return hipSuccess;
+  }
+};
+
+}  // namespace bemhip
+
+struct bemhip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int num_cu = 0;
+
+  // geometry
+  bool has_geo = false;
+  int p = 0, nn = 0, ne = 0, n = 0, ni = 0, nt = 0;
+  double k = 1.0;
+  std::vector<double> h_elem_w;
+  bemhip::DevBuf<double> x, y, nx, ny, jw, elen, gx, gy;
+
+  // assembly
+  bool assembled = false;
+  int precision = 0;
+  int npg = 0;
+  double near_factor = 1.5;
+  bemhip::DevBuf<bemhip::QuadConst> qc;
+  bemhip::DevBuf<double> H64, G64;
+  bemhip::DevBuf<float> H32, G32;
+  bemhip::DevBuf<uint32_t> nearbits;   // [ceil(ne/32)][nt]
+  bemhip::DevBuf<int> row_cnt, row_ptr, pair_row, pair_elem;
+  bemhip::DevBuf<double> pair_h, pair_g;  // [npairs][nn]
+  int64_t npairs = 0;
+  bemhip::DevBuf<double> hpart;   // [ncb][nt] far row sums of H
+  bemhip::DevBuf<double> diag64;  // [nt] rigid-body diagonal
+  bemhip::DevBuf<char> scan_tmp;
+
+  // boundary conditions / system
+  bool has_bc = false;
+  std::vector<int64_t> h_bc;
+  std::vector<double> h_bv;
+  bemhip::DevBuf<int> d_bc;      // n BC codes (0 Dirichlet, 1 Neumann)
+  bemhip::DevBuf<double> d_bv;   // n BC values
+  bemhip::DevBuf<double> wH, wG, wH2, wG2;  // per-column weights for FP64 operator passes
+  bemhip::DevBuf<double> b64;
+  bemhip::DevBuf<double> part1, part2;     // [ncb][nt] operator partial sums
+  bemhip::DevBuf<double> A64;
+  bemhip::DevBuf<float> A32;
+  bemhip::DevBuf<int> ipiv;
+  bemhip::DevBuf<int> perm;
+  bool factored64 = false, factored32 = false;
+
+  // scratch for LU and solves
+  bemhip::DevBuf<char> lu_work;
+  bemhip::DevBuf<double> vec64a, vec64b, vec64c;
+  bemhip::DevBuf<float> vec32a;
+  bemhip::DevBuf<int> info;
+
+  // timings (ms) of the last calls, keyed by phase
+  std::map<std::string, double> ms;
+  std::map<std::string, double> stats;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace bemhip {
+
+// Error helpers ---------------------------------------------------------------
+void set_global_error(const std::string& s);
+int fail(bemhip_ctx* c, int code, const std::string& msg);
+
+#define BEMHIP_HIP_TRY(ctx, call)                                                     \
+  do {                                                                                \
+    hipError_t _e = (call);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return ::bemhip::fail((ctx), _e == hipErrorOutOfMemory ? BEMHIP_ERR_NOMEM       \
+                                                             : BEMHIP_ERR_HIP,        \
+                            std::string(#call) + ": " + hipGetErrorString(_e));       \
+  } while (0)
+
+// Wave-level reductions (64 lanes) ---------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Host-side launchers (defined in assemble.hip / lu.hip) ----------------------
+int assemble_impl(bemhip_ctx* c);
+int apply_bc_impl(bemhip_ctx* c);
+int operator_apply(bemhip_ctx* c, const double* d_x, double* d_y);
+int lu_factor_device(bemhip_ctx* c, int n, int dtype, void* dA, int lda, int* dipiv,
+                     int* info_host);
+int lu_solve_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int lda,
+                    const int* dperm, double* d_b);
+int build_perm(bemhip_ctx* c, int n, const int* dipiv, int* dperm);
+
+// Small host helpers
+void gauss_legendre_host(int n, double* x, double* w);
+
+struct ScopedTimer {
+  bemhip_ctx* c;
+  const char* key;
+  ScopedTimer(bemhip_ctx* ctx, const char* k) : c(ctx), key(k) {
+    hipEventRecord(c->ev0, c->stream);
+  }
+  ~ScopedTimer() {
+    hipEventRecord(c->ev1, c->stream);
+    hipEventSynchronize(c->ev1);
+    float t = 0.f;
+    hipEventElapsedTime(&t, c->ev0, c->ev1);
+    c->ms[key] = t;
+  }
+};
+
+}  // namespace bemhip

@@ -1,0 +1,260 @@
+// Linear solve of the mixed-BC system A x = b (bemhip_solve) and the
+// standalone LU entry points. Replaces bem_linsolve
+// (src/Core/LinearSolveUtils.jl:44-49) as called by solve (src/Core/Solver.jl:24-64).
+//
+//   BEMHIP_SOLVE_FP64   FP64 LU of the FP64-assembled A, one forward/backward solve.
+//   BEMHIP_SOLVE_MIXED  FP32 LU (MFMA trailing updates) of A32, then FP64
+//                       iterative refinement: r = b - A64 x with A64 applied in
+//                       FP64 (recomputed far field + FP64 near/singular entries +
+//                       FP64 diagonal), d = (LU32)^-1 r, x += d, until
+//                       ||d||_inf <= tol ||x||_inf.
+
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace bemhip {
+
+namespace {
+
+__global__ void sub_kernel(int n, const double* __restrict__ b, const double* __restrict__ y,
+                           double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - y[i];
+}
+
+__global__ void axpy_kernel(int n, const double* __restrict__ d, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+__global__ void round_f32_kernel(size_t count, const double* __restrict__ s, float* __restrict__ d) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+       t += (size_t)gridDim.x * blockDim.x)
+    d[t] = (float)s[t];
+}
+
+// max |v| over n entries into out[slot] (one workgroup).
+__global__ __launch_bounds__(1024) void absmax_kernel(int n, const double* __restrict__ v,
+                                                      double* __restrict__ out, int slot) {
+  __shared__ double s[16];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(v[i]));
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = s[0];
+    for (int w = 1; w < 16; ++w) r = fmax(r, s[w]);
+    out[slot] = r;
+  }
+}
+
+int absmax2(bemhip_ctx* c, int n, const double* a, const double* b, double* ha, double* hb) {
+  BEMHIP_HIP_TRY(c, c->vec64b.ensure(std::max<size_t>(c->vec64b.count, 4)));
+  double* slots = c->vec64b.ptr;  // first entries reused as scalars
+  hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(1024), 0, c->stream, n, a, slots, 0);
+  if (b) hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(1024), 0, c->stream, n, b, slots, 1);
+  double h[2] = {0, 0};
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(h, slots, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *ha = h[0];
+  if (hb) *hb = h[1];
+  return BEMHIP_OK;
+}
+
+}  // namespace
+}  // namespace bemhip
+
+using namespace bemhip;
+
+extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, double* x, int* iters,
+                            double* relres) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "solve: NULL context");
+  if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "solve: apply_bc first");
+  if (mode != BEMHIP_SOLVE_FP64 && mode != BEMHIP_SOLVE_MIXED)
+    return fail(c, BEMHIP_ERR_ARG, "solve: mode must be BEMHIP_SOLVE_FP64 or BEMHIP_SOLVE_MIXED");
+  if (mode == BEMHIP_SOLVE_FP64 && c->precision != BEMHIP_FP64)
+    return fail(c, BEMHIP_ERR_STATE, "solve: FP64 LU needs an FP64 assembly (precision 0)");
+  if (mode == BEMHIP_SOLVE_MIXED && (!(tol > 0.0) || maxit < 0))
+    return fail(c, BEMHIP_ERR_ARG, "solve: tol must be > 0 and maxit >= 0");
+  hipSetDevice(c->device);
+  const int nt = c->nt;
+  const size_t nn2 = (size_t)nt * nt;
+  BEMHIP_HIP_TRY(c, c->ipiv.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->perm.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->vec64a.ensure(nt));  // x
+  BEMHIP_HIP_TRY(c, c->vec64b.ensure(std::max(nt, 4)));
+  BEMHIP_HIP_TRY(c, c->part2.ensure(2 * (size_t)nt));  // r, y
+  double* dx = c->vec64a.ptr;
+  double* dr = c->part2.ptr;
+  double* dy = c->part2.ptr + nt;
+  const int tb = 256, nb = (nt + tb - 1) / tb;
+  hipEvent_t t0, t1;
+  hipEventCreate(&t0);
+  hipEventCreate(&t1);
+  hipEventRecord(t0, c->stream);
+  int st = 0, info = 0, it = 0;
+  double rel = NAN;
+  if (mode == BEMHIP_SOLVE_FP64) {
+    if (!c->factored64) {
+      hipEvent_t f0, f1;
+      hipEventCreate(&f0);
+      hipEventCreate(&f1);
+      hipEventRecord(f0, c->stream);
+      st = lu_factor_device(c, nt, 0, c->A64.ptr, nt, c->ipiv.ptr, &info);
+      hipEventRecord(f1, c->stream);
+      hipEventSynchronize(f1);
+      float ms = 0;
+      hipEventElapsedTime(&ms, f0, f1);
+      c->ms["lu64"] = ms;
+      hipEventDestroy(f0);
+      hipEventDestroy(f1);
+      if (st) return st;
+      if (info) return fail(c, BEMHIP_ERR_SINGULAR, "solve: zero pivot in the FP64 LU");
+      if ((st = build_perm(c, nt, c->ipiv.ptr, c->perm.ptr))) return st;
+      c->factored64 = true;
+    }
+    BEMHIP_HIP_TRY(c, hipMemcpyAsync(dx, c->b64.ptr, sizeof(double) * nt, hipMemcpyDeviceToDevice,
+                                     c->stream));
+    if ((st = lu_solve_device(c, nt, 0, c->A64.ptr, nt, c->perm.ptr, dx))) return st;
+  } else {
+    if (!c->factored32) {
+      if (c->precision == BEMHIP_FP64) {
+        // FP32 copy of the FP64-assembled system
+        BEMHIP_HIP_TRY(c, c->A32.ensure(nn2));
+        hipLaunchKernelGGL(round_f32_kernel, dim3(4096), dim3(256), 0, c->stream, nn2,
+                           c->A64.ptr, c->A32.ptr);
+      }
+      hipEvent_t f0, f1;
+      hipEventCreate(&f0);
+      hipEventCreate(&f1);
+      hipEventRecord(f0, c->stream);
+      st = lu_factor_device(c, nt, 1, c->A32.ptr, nt, c->ipiv.ptr, &info);
+      hipEventRecord(f1, c->stream);
+      hipEventSynchronize(f1);
+      float ms = 0;
+      hipEventElapsedTime(&ms, f0, f1);
+      c->ms["lu32"] = ms;
+      hipEventDestroy(f0);
+      hipEventDestroy(f1);
+      if (st) return st;
+      if (info) return fail(c, BEMHIP_ERR_SINGULAR, "solve: zero pivot in the FP32 LU");
+      if ((st = build_perm(c, nt, c->ipiv.ptr, c->perm.ptr))) return st;
+      c->factored32 = true;
+    }
+    hipEvent_t r0, r1;
+    hipEventCreate(&r0);
+    hipEventCreate(&r1);
+    hipEventRecord(r0, c->stream);
+    BEMHIP_HIP_TRY(c, hipMemcpyAsync(dx, c->b64.ptr, sizeof(double) * nt, hipMemcpyDeviceToDevice,
+                                     c->stream));
+    if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm.ptr, dx))) return st;
+    bool converged = false;
+    for (it = 0; it < maxit; ++it) {
+      if ((st = operator_apply(c, dx, dy))) return st;
+      hipLaunchKernelGGL(sub_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, c->b64.ptr, dy, dr);
+      if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm.ptr, dr))) return st;
+      double dn = 0, xn = 0;
+      if ((st = absmax2(c, nt, dr, dx, &dn, &xn))) return st;
+      hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dr, dx);
+      if (!std::isfinite(dn)) break;
+      if (dn <= tol * xn) { converged = true; ++it; break; }
+    }
+    hipEventRecord(r1, c->stream);
+    hipEventSynchronize(r1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, r0, r1);
+    c->ms["refine"] = ms;
+    hipEventDestroy(r0);
+    hipEventDestroy(r1);
+    c->stats["refine_iters"] = it;
+    if (!converged && maxit > 0) {
+      hipEventDestroy(t0);
+      hipEventDestroy(t1);
+      return fail(c, BEMHIP_ERR_NOCONV, "solve: iterative refinement did not converge");
+    }
+  }
+  hipEventRecord(t1, c->stream);
+  hipEventSynchronize(t1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, t0, t1);
+  c->ms[mode == BEMHIP_SOLVE_FP64 ? "solve64" : "solve_mixed"] = ms;
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  // final FP64 relative residual (reported, outside the timed solve)
+  if ((st = operator_apply(c, dx, dy))) return st;
+  hipLaunchKernelGGL(sub_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, c->b64.ptr, dy, dr);
+  double rn = 0, bn = 0;
+  if ((st = absmax2(c, nt, dr, c->b64.ptr, &rn, &bn))) return st;
+  rel = bn > 0 ? rn / bn : rn;
+  c->stats["relres"] = rel;
+  if (x) BEMHIP_HIP_TRY(c, hipMemcpy(x, dx, sizeof(double) * nt, hipMemcpyDeviceToHost));
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  return BEMHIP_OK;
+}
+
+extern "C" int bemhip_lu_factor(bemhip_ctx* c, int n, int dtype, void* A, int* ipiv) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "lu_factor: NULL context");
+  if (n < 1 || !A || !ipiv || (dtype != 0 && dtype != 1))
+    return fail(c, BEMHIP_ERR_ARG, "lu_factor: need n >= 1, dtype 0/1 and non-NULL buffers");
+  hipSetDevice(c->device);
+  const size_t es = dtype == 0 ? 8 : 4;
+  const size_t bytes = es * (size_t)n * n;
+  DevBuf<char> dA;
+  DevBuf<int> dp;
+  BEMHIP_HIP_TRY(c, dA.ensure(bytes));
+  BEMHIP_HIP_TRY(c, dp.ensure(n));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(dA.ptr, A, bytes, hipMemcpyHostToDevice, c->stream));
+  int info = 0;
+  hipEvent_t f0, f1;
+  hipEventCreate(&f0);
+  hipEventCreate(&f1);
+  hipEventRecord(f0, c->stream);
+  int st = lu_factor_device(c, n, dtype, dA.ptr, n, dp.ptr, &info);
+  hipEventRecord(f1, c->stream);
+  hipEventSynchronize(f1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, f0, f1);
+  c->ms[dtype == 0 ? "lu64" : "lu32"] = ms;
+  hipEventDestroy(f0);
+  hipEventDestroy(f1);
+  if (st) return st;
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(A, dA.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(ipiv, dp.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (info) return fail(c, BEMHIP_ERR_SINGULAR, "lu_factor: zero pivot");
+  return BEMHIP_OK;
+}
+
+extern "C" int bemhip_lu_solve(bemhip_ctx* c, int n, int dtype, const void* LU, const int* ipiv,
+                               double* b, int trans) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "lu_solve: NULL context");
+  if (n < 1 || !LU || !ipiv || !b || (dtype != 0 && dtype != 1))
+    return fail(c, BEMHIP_ERR_ARG, "lu_solve: need n >= 1, dtype 0/1 and non-NULL buffers");
+  if (trans != 0) return fail(c, BEMHIP_ERR_ARG, "lu_solve: only trans = 0 is supported");
+  for (int i = 0; i < n; ++i)
+    if (ipiv[i] < i || ipiv[i] >= n) return fail(c, BEMHIP_ERR_ARG, "lu_solve: invalid ipiv");
+  hipSetDevice(c->device);
+  const size_t es = dtype == 0 ? 8 : 4;
+  const size_t bytes = es * (size_t)n * n;
+  DevBuf<char> dA;
+  DevBuf<int> dp, dperm;
+  DevBuf<double> db;
+  BEMHIP_HIP_TRY(c, dA.ensure(bytes));
+  BEMHIP_HIP_TRY(c, dp.ensure(n));
+  BEMHIP_HIP_TRY(c, dperm.ensure(n));
+  BEMHIP_HIP_TRY(c, db.ensure(n));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(dA.ptr, LU, bytes, hipMemcpyHostToDevice, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(dp.ptr, ipiv, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(db.ptr, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  int st = build_perm(c, n, dp.ptr, dperm.ptr);
+  if (st) return st;
+  if ((st = lu_solve_device(c, n, dtype, dA.ptr, n, dperm.ptr, db.ptr))) return st;
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(b, db.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return BEMHIP_OK;
+}

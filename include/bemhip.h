@@ -1,0 +1,161 @@
+/*
+ * bemhip.h — C ABI of libbemhip.so, the MI355X (gfx950) implementation of the
+ * dense 2-D Laplace boundary-element path of BEM_gmsh:
+ *
+ *     format2d  ->  assemble!(dad, npg)  ->  solve(dad)
+ *
+ * Plain C types only (pointers + sizes); every call returns a status code
+ * (BEMHIP_OK on success) and records a message readable with
+ * bemhip_last_error(). No exception crosses this boundary.
+ *
+ * Memory: the caller owns host buffers; the context owns device buffers
+ * (geometry, H, G, A, LU factors, pivots), which stay resident in HBM between
+ * calls. All matrices are column-major with leading dimension nt, exactly the
+ * Julia layout of dad.H (nt x nt) and dad.G (nt x n).
+ *
+ * Threading: a context is bound to one device and one HIP stream; calls on a
+ * context are serialized and not re-entrant. Separate contexts are independent.
+ *
+ * Reference interfaces (paths relative to the BEM_gmsh repository root) are
+ * cited on each declaration.
+ */
+#ifndef BEMHIP_H
+#define BEMHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bemhip_ctx bemhip_ctx;
+
+/* Status codes. */
+#define BEMHIP_OK 0
+#define BEMHIP_ERR_ARG 1        /* invalid argument (reference: ArgumentError)    */
+#define BEMHIP_ERR_HIP 2        /* HIP runtime failure                            */
+#define BEMHIP_ERR_STATE 3      /* call out of order (reference: missing cache)   */
+#define BEMHIP_ERR_SINGULAR 4   /* zero pivot in the LU                           */
+#define BEMHIP_ERR_NOCONV 5     /* iterative refinement did not converge          */
+#define BEMHIP_ERR_NOMEM 6      /* device allocation failed                       */
+#define BEMHIP_ERR_NODEV 7      /* no usable GPU                                  */
+
+/* Assembly precision. */
+#define BEMHIP_FP64 0           /* H, G stored and computed in FP64               */
+#define BEMHIP_MIXED 1          /* far field FP32, near/singular/diagonal FP64    */
+
+/* Solve modes. */
+#define BEMHIP_SOLVE_FP64 0     /* FP64 LU (MFMA trailing updates)                */
+#define BEMHIP_SOLVE_MIXED 1    /* FP32 LU (MFMA) + FP64 iterative refinement     */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int bemhip_version(void);
+
+/* Short name of a status code ("ok", "bad argument", ...). */
+const char* bemhip_strerror(int status);
+
+/* Last error message of a context, or of the library when ctx is NULL. */
+const char* bemhip_last_error(const bemhip_ctx* ctx);
+
+/* Gauss-Legendre nodes/weights on [-1,1], ascending (host only).
+ * Replaces FastGaussQuadrature.gausslegendre as called from
+ * src/Core/Assembly_full.jl:719-723 and src/Core/Input.jl:86-89. */
+int bemhip_gauss_legendre(int n, double* x, double* w);
+
+/* Number of visible GPUs (0 and BEMHIP_ERR_NODEV when none). */
+int bemhip_device_count(int* count);
+
+/* Create / destroy a context on device_id. Replaces the backend selection of
+ * _ka_backend (src/Laplace/Assembly_GPU.jl:152-167). */
+int bemhip_create(bemhip_ctx** out, int device_id);
+void bemhip_destroy(bemhip_ctx* ctx);
+
+/* Upload the collocation geometry built by format2d (src/Core/Input.jl:136-197,
+ * packed like _pack_boundary2d, src/Laplace/Assembly_GPU.jl:169-233).
+ *   p        element order (1 or 2); element e owns nodes e*(p+1) .. e*(p+1)+p
+ *   ne, n    elements and boundary nodes (n == ne*(p+1)); ni internal nodes
+ *   coll_xy  (nt x 2) collocation points, boundary first     (dad.collocation)
+ *   nrm_xy   (n x 2) outward unit normals                    (dad.Normal)
+ *   J        (n) |dx/dxi| at each boundary node              (Element.Jacobian)
+ *   elem_w   (p+1) Gauss-Legendre weights of the element rule (dad.elem_weight)
+ *   geo_xy   (ne x (p+1) x 2) geometric nodes, xi = -1..1 equispaced (Element.geo)
+ *   elen     (ne) element lengths                             (Element.Length)
+ *   k        conductivity (Laplace.k)
+ * Invalidates any assembled matrices and factors. */
+int bemhip_set_geometry(bemhip_ctx* ctx, int p, int ne, int n, int ni,
+                        const double* coll_xy, const double* nrm_xy, const double* J,
+                        const double* elem_w, const double* geo_xy, const double* elen,
+                        double k);
+
+/* Assemble H (nt x nt) and G (nt x n) on the device. Replaces
+ * H_G_full_direct (src/Core/Assembly_full.jl:61-85) behind
+ * assemble!(dad; method, npg) (src/Core/Assembly_full.jl:107-118): far
+ * nodal lumping (:801-810), near classification (:736-754), tan-p3c near
+ * quadrature (:649-695, src/Core/Integration.jl:68-93, :1117-1365),
+ * interpolated-Guiggiani on-element integrals (:397-424,
+ * src/Core/Integration.jl:1597-1845) and the rigid-body diagonal (:384-390).
+ *   npg          Gauss points of the near rule (>= 1, <= 64)
+ *   near_factor  far-lumping cutoff factor (reference default 1.5; > 1e20 = all near)
+ *   precision    BEMHIP_FP64 or BEMHIP_MIXED */
+int bemhip_assemble(bemhip_ctx* ctx, int npg, double near_factor, int precision);
+
+/* Copy the assembled H, G to host buffers (either may be NULL), converting to
+ * double (FP32 storage in MIXED mode), or to float with the _f32 variant. */
+int bemhip_get_HG(bemhip_ctx* ctx, double* H, double* G);
+int bemhip_get_HG_f32(bemhip_ctx* ctx, float* H, float* G);
+
+/* Near/on-element pair list of the last assembly (row-sorted CSR): count, and
+ * per pair its row, element and the (p+1) FP64 values added to H and G.
+ * Replaces the host mask of _near_skip_mask (src/Laplace/Assembly_GPU.jl:236-253)
+ * and the near correction of _correct_near_cpu! (:322-342). */
+int bemhip_near_pairs(bemhip_ctx* ctx, int64_t* count);
+int bemhip_get_near_pairs(bemhip_ctx* ctx, int* row, int* elem, double* h, double* g);
+
+/* Mixed-BC column swap: A = H with Dirichlet (BC==0) columns replaced by -G,
+ * b = -sum_{BC=0} H[:,j] BV_j + sum_{BC=1} G[:,j] BV_j.
+ * Replaces applyBC (src/Core/Boundary_conditions.jl:20-75). BC, BV have n
+ * entries. A (nt x nt, FP64) and b (nt) are optional host copies; the system
+ * stays on the device for bemhip_solve. */
+int bemhip_apply_bc(bemhip_ctx* ctx, const int64_t* BC, const double* BV, double* A,
+                    double* b);
+
+/* Solve A x = b on the device and return x (nt). Replaces bem_linsolve
+ * (src/Core/LinearSolveUtils.jl:44-49) as called by solve
+ * (src/Core/Solver.jl:24-64); split_sol! (:88-98) is left to the caller.
+ *   mode     BEMHIP_SOLVE_FP64 (needs FP64 assembly) or BEMHIP_SOLVE_MIXED
+ *   tol      refinement stop: ||dx||_inf <= tol * ||x||_inf (MIXED only)
+ *   maxit    refinement iteration cap (MIXED only)
+ *   iters    (out, may be NULL) refinement iterations taken
+ *   relres   (out, may be NULL) final ||b - A x||_inf / ||b||_inf in FP64
+ * The factors stay resident (re-used by later solves on the same system). */
+int bemhip_solve(bemhip_ctx* ctx, int mode, double tol, int maxit, double* x, int* iters,
+                 double* relres);
+
+/* y = A x in FP64 for the current system (stored A, or recomputed on the fly
+ * from the FP64 kernels in MIXED mode). x, y have nt entries. Used for the
+ * refinement residual; exposed for verification. */
+int bemhip_apply_operator(bemhip_ctx* ctx, const double* x, double* y);
+
+/* Standalone dense LU with partial pivoting of a host matrix (column-major
+ * n x n), the LU seam of bem_linsolve (src/Core/LinearSolveUtils.jl:44-49).
+ *   dtype 0: A is double*, 1: A is float*. A is overwritten by L\U (unit L)
+ *   and ipiv (n, 0-based: row i was swapped with row ipiv[i]).
+ * bemhip_lu_solve solves with those factors in place of b (nrhs = 1); trans
+ * must be 0. */
+int bemhip_lu_factor(bemhip_ctx* ctx, int n, int dtype, void* A, int* ipiv);
+int bemhip_lu_solve(bemhip_ctx* ctx, int n, int dtype, const void* LU, const int* ipiv,
+                    double* b, int trans);
+
+/* Timings (ms, HIP events on the context stream) and sizes of the last calls
+ * as a JSON object. */
+int bemhip_counters(bemhip_ctx* ctx, char* json, size_t cap);
+
+/* Wait for all work queued on the context stream. */
+int bemhip_synchronize(bemhip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEMHIP_H */
