@@ -412,13 +412,14 @@ __device__ void complex_pole(const double* lx, const double* cx, const double* c
                a31 = lx[2] * lx[2], a32 = lx[2];
   const double det = a11 * (a22 - a32) - a12 * (a21 - a31) + (a21 * a32 - a22 * a31);
   // Cramer's rule for [xi^2 xi 1] c = Z (_fit_quad_complex, :1153-1160)
-  const Cx A = cscale(cadd(csub(cscale(Z0, a22 - a32), cscale(csub(Z1, Z2), a12)),
-                           csub(cscale(Z1, a32), cscale(Z2, a22))), 1.0 / det);
-  const Cx B = cscale(cadd(csub(cscale(csub(Z1, Z2), a11), cscale(Z0, a21 - a31)),
-                           csub(cscale(Z2, a21), cscale(Z1, a31))), 1.0 / det);
-  const Cx C = cscale(cadd(csub(cscale(csub(cscale(Z2, a22), cscale(Z1, a32)), a11),
-                                cscale(csub(cscale(Z2, a21), cscale(Z1, a31)), a12)),
-                           cscale(Z0, a21 * a32 - a22 * a31)), 1.0 / det);
+  auto cdivr = [](Cx z, double d) { return Cx{z.re / d, z.im / d}; };
+  const Cx A = cdivr(cadd(csub(cscale(Z0, a22 - a32), cscale(csub(Z1, Z2), a12)),
+                          csub(cscale(Z1, a32), cscale(Z2, a22))), det);
+  const Cx B = cdivr(cadd(csub(cscale(csub(Z1, Z2), a11), cscale(Z0, a21 - a31)),
+                          csub(cscale(Z2, a21), cscale(Z1, a31))), det);
+  const Cx C = cdivr(cadd(csub(cscale(csub(cscale(Z2, a22), cscale(Z1, a32)), a11),
+                               cscale(csub(cscale(Z2, a21), cscale(Z1, a31)), a12)),
+                          cscale(Z0, a21 * a32 - a22 * a31)), det);
   if (cabs(A) < 1e-14) {
     if (cabs(B) < 1e-30) { zeta = 0.0; eta = 1.0; return; }
     const Cx r = cdiv(csub(y, C), B);

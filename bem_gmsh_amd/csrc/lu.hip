@@ -418,7 +418,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const T*
     for (int tj = 0; tj < 4; ++tj) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int j = bn + wn + tj * 16 + (lane >> 4) * 4 + r;
+        // Output row m of the 16x16 MFMA block held in register r of this lane:
+        // f32: m = 4*(lane/16) + r; f64: m = 4*r + lane/16 (rows interleaved).
+        const int m = (sizeof(T) == 4) ? (lane >> 4) * 4 + r : r * 4 + (lane >> 4);
+        const int j = bn + wn + tj * 16 + m;
         if (i < M && j < N) C[(size_t)i + (size_t)j * ldc] -= acc[ti][tj][r];
       }
     }

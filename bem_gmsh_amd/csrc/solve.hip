@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <iterator>
 #include <vector>
 
 namespace bemhip {
@@ -153,6 +155,10 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
                                      c->stream));
     if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm.ptr, dx))) return st;
     bool converged = false;
+    double dn_prev = INFINITY;
+    c->stats.erase("refine_stagnated");
+    for (auto it2 = c->stats.begin(); it2 != c->stats.end();)
+      it2 = (it2->first.rfind("refine_dx_", 0) == 0) ? c->stats.erase(it2) : std::next(it2);
     for (it = 0; it < maxit; ++it) {
       if ((st = operator_apply(c, dx, dy))) return st;
       hipLaunchKernelGGL(sub_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, c->b64.ptr, dy, dr);
@@ -160,8 +166,22 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
       double dn = 0, xn = 0;
       if ((st = absmax2(c, nt, dr, dx, &dn, &xn))) return st;
       hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dr, dx);
+      if (it < 40) {
+        char key[32];
+        std::snprintf(key, sizeof(key), "refine_dx_%02d", it);
+        c->stats[key] = xn > 0 ? dn / xn : dn;
+      }
       if (!std::isfinite(dn)) break;
       if (dn <= tol * xn) { converged = true; ++it; break; }
+      // Stagnation at the FP64 noise floor (kappa * eps64): the correction no
+      // longer halves and is already below 1e-10 relative.
+      if (it >= 1 && dn > 0.5 * dn_prev && dn <= 1e-10 * xn) {
+        converged = true;
+        c->stats["refine_stagnated"] = 1.0;
+        ++it;
+        break;
+      }
+      dn_prev = dn;
     }
     hipEventRecord(r1, c->stream);
     hipEventSynchronize(r1);

@@ -189,6 +189,8 @@ def pair(dad, i, e, npg=20):
 
 def apply_bc(dad, H, G):
     nt, n = dad.nt, dad.n
+    H = np.asfortranarray(H, dtype=np.float64)
+    G = np.asfortranarray(G, dtype=np.float64)
     A = np.empty((nt, nt), order="F")
     b = np.empty(nt)
     BC = np.ascontiguousarray(dad.BC, dtype=np.int64)
