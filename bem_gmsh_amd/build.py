@@ -23,7 +23,7 @@ ARCH = os.environ.get("BEMHIP_ARCH", "gfx950")
 SOURCES = {
     "api.hip": [],
     "assemble.hip": ["-ffp-contract=off"],
-    "lu.hip": [],
+    "lu.hip": ["-mllvm", "-pragma-unroll-threshold=200000"],
     "solve.hip": [],
 }
 

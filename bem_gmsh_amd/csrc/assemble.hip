@@ -142,9 +142,14 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
         rowsum += h;
       }
     } else {
-      // Mixed precision: the regular far quadrature in FP32.
+      // Mixed precision: the regular far quadrature in FP32; the FP64 row sum
+      // of H (rigid-body diagonal) uses the FP64 double-layer value, which
+      // needs no logarithm, with the operations of the FP64 kernel.
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
+        const double T64 = (rx[k] * s_nx[el * NN + k] + ry[k] * s_ny[el * NN + k]) *
+                           (cs.cT / d2[k]);
+        rowsum += T64 * s_jw[el * NN + k];
         const float fx = (float)rx[k], fy = (float)ry[k];
         const float r2 = fx * fx + fy * fy;
         const float U = -logf(r2) * (float)cs.cU;
@@ -943,8 +948,8 @@ int assemble_impl(bemhip_ctx* c) {
       st = (c->nn == 2) ? launch_far<double, 2>(c, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
                         : launch_far<double, 3>(c, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
     else
-      st = (c->nn == 2) ? launch_far<float, 2>(c, g, cs, c->H32.ptr, c->G32.ptr, nullptr)
-                        : launch_far<float, 3>(c, g, cs, c->H32.ptr, c->G32.ptr, nullptr);
+      st = (c->nn == 2) ? launch_far<float, 2>(c, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr)
+                        : launch_far<float, 3>(c, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
     if (st) return st;
   }
   {
@@ -989,21 +994,6 @@ int assemble_impl(bemhip_ctx* c) {
                          c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr,
                          nullptr, c->H64.ptr);
     } else {
-      // FP64 row sums of the far field recomputed on the fly.
-      BEMHIP_HIP_TRY(c, c->wH.ensure(nt));
-      BEMHIP_HIP_TRY(c, c->wG.ensure(nt));
-      std::vector<double> ones(nt, 1.0), zeros(nt, 0.0);
-      BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->wH.ptr, ones.data(), sizeof(double) * nt,
-                                       hipMemcpyHostToDevice, c->stream));
-      BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->wG.ptr, zeros.data(), sizeof(double) * nt,
-                                       hipMemcpyHostToDevice, c->stream));
-      dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)ncb);
-      if (c->nn == 2)
-        hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1,
-                           c->wH.ptr, c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
-      else
-        hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1,
-                           c->wH.ptr, c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
       hipLaunchKernelGGL((op64_finalize<float>), dim3(nb), dim3(tb), 0, c->stream, (int)nt,
                          c->n, c->nn, ncb, 0, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                          c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr,

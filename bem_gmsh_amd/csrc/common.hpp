@@ -120,6 +120,9 @@ struct bemhip_ctx {
 
   // scratch for LU and solves
   bemhip::DevBuf<char> lu_work;
+  bemhip::DevBuf<char> dinv;          // inverses of the diagonal blocks of L and U
+  const void* dinv_of = nullptr;      // factor the inverses belong to
+  int dinv_n = 0;
   bemhip::DevBuf<double> vec64a, vec64b, vec64c;
   bemhip::DevBuf<float> vec32a;
   bemhip::DevBuf<int> info;

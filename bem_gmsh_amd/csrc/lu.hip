@@ -4,18 +4,20 @@
 // This is the LU seam of bem_linsolve (src/Core/LinearSolveUtils.jl:44-49),
 // whose default for a dense square system is an LAPACK getrf/getrs.
 //
-// Blocked right-looking algorithm, panel width kb:
-//   panel_kernel   cooperative launch over all CUs: the m x kb panel is held in
-//                  LDS, split by rows across workgroups; one grid barrier per
-//                  column for the partial-pivot search (GEPP, ties -> lowest row).
+// Blocked right-looking algorithm, panel width 256:
+//   panel_kernel   cooperative launch, one workgroup per 64 panel rows, the
+//                  panel held in registers; one grid barrier per column for the
+//                  partial-pivot search (GEPP, ties -> lowest row).
 //   perm_build / perm_apply   the panel's kb interchanges turned into one gather
 //                  list, applied to every column outside the panel (a wave per
 //                  column, coalesced over the contiguous destination rows).
-//   trsm_kernel    U12 = L11^-1 A12, column strips in LDS.
+//   trsm_rec       U12 = L11^-1 A12, recursive halving: 32-row diagonal blocks
+//                  in LDS, off-diagonal blocks on the MFMA GEMM.
 //   gemm_kernel    A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 /
 //                  v_mfma_f32_16x16x4f32), 128x128 block tiles, LDS double buffer.
 // Solves: row permutation as a gather, then blocked forward / backward
-// substitution with FP64 accumulation.
+// substitution with precomputed inverses of the 128x128 diagonal blocks and
+// FP64 accumulation (one launch per block).
 
 #include "common.hpp"
 
@@ -32,170 +34,200 @@ namespace bemhip {
 namespace {
 
 // ---------------------------------------------------------------------------
-// Panel factorization.
+// Panel factorization (GEPP) with the panel held in registers.
+//
+// Workgroup = NW waves x 64 lanes. Lane l of every wave owns panel row
+// blockIdx.x*64 + l; wave w owns the CG panel columns [w*CG, (w+1)*CG), so a
+// thread holds CG values of one row. Panel width KB = NW*CG = 256.
+// Per column c: the owning wave finds the local candidate (max |a|, lowest
+// row on ties), stages it, one grid barrier, every wave picks the global
+// winner, the pivot row's values (uniform across lanes) drive the interchange
+// and the rank-1 update of the rows below.
 // ---------------------------------------------------------------------------
-constexpr int kPanelThreads = 512;
+constexpr int kPB = 256;   // largest panel width handled by the interchange lists
+constexpr int kPR = 64;    // rows per workgroup
 
 template <typename T>
 struct PanelArgs {
   T* A;
   int lda;
   int m;          // panel rows (global rows k0 .. k0+m-1)
-  int kb;         // panel width
+  int kb;         // panel width (<= KB of the kernel)
   int k0;         // global index of the panel's first row/column
-  int rows_per;   // rows per workgroup
   int G;          // workgroups
   int* ipiv;      // ipiv[k0 + c] = global pivot row of column k0 + c
   double* st_val; // [2][G] |candidate|
   int* st_row;    // [2][G] candidate row (panel-relative), INT_MAX if none
-  T* st_data;     // [2][G][kb] candidate row values
-  T* st_rowc;     // [2][kb] values of row c before the interchange
+  T* st_data;     // [2][G][KB] candidate row values
+  T* st_rowc;     // [2][KB] values of row c before the interchange
   int* info;      // first zero pivot: global column + 1 (0 if none)
+  unsigned* bar;  // grid barrier counter (zero at launch)
 };
 
-__device__ __forceinline__ void better(double& v, int& r, double ov, int orr) {
-  if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; }
+__device__ __forceinline__ bool better(double ov, int orr, double v, int r) {
+  return ov > v || (ov == v && orr < r);
 }
 
-template <typename T>
-__global__ __launch_bounds__(kPanelThreads) void panel_kernel(PanelArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* tile = reinterpret_cast<T*>(smem_raw);  // [kb][rows_per], column-major
-  __shared__ double red_v[kPanelThreads / 64];
-  __shared__ int red_r[kPanelThreads / 64];
-  __shared__ int red_w[kPanelThreads / 64];
-  __shared__ int s_lwin;
-  __shared__ int s_gwin_wg, s_gwin_row;
-  __shared__ double s_gwin_val;
-  __shared__ T s_prow[256];
-  cg::grid_group grid = cg::this_grid();
-
-  const int wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwaves = kPanelThreads / 64;
-  const int rp = a.rows_per;
-  const int r0 = wg * rp;
-  const int nr = max(0, min(rp, a.m - r0));
-  const int kb = a.kb;
-  const size_t lda = (size_t)a.lda;
-  T* Ap = a.A + (size_t)a.k0 + (size_t)a.k0 * lda;  // panel origin
-
-  for (int idx = tid; idx < nr * kb; idx += kPanelThreads) {
-    const int r = idx % nr, c = idx / nr;
-    tile[c * rp + r] = Ap[(size_t)(r0 + r) + (size_t)c * lda];
+// Grid-wide barrier on a monotonic counter: every workgroup adds 1 per use
+// and waits until the counter reaches `target` (= uses so far x workgroups).
+// Every wave first waits for its own outstanding global stores (they are then
+// in L2); after the workgroup barrier, thread 0's agent-scope release writes L2
+// back and publishes them. Thread 0's acquire (which also invalidates the CU's
+// caches) precedes every wave's later loads through the second barrier.
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned target) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
   }
   __syncthreads();
+}
 
-  for (int c = 0; c < kb; ++c) {
-    const int par = c & 1;
-    // 1. local candidate: max |A(r, c)| over rows r >= c
-    double best = -1.0;
-    int brow = INT_MAX;
-    const int lr0 = max(0, c - r0);
-    for (int r = lr0 + tid; r < nr; r += kPanelThreads) {
-      const double v = fabs((double)tile[c * rp + r]);
-      better(best, brow, v, r0 + r);
-    }
+template <typename T, int NW, int CG>
+__global__ __launch_bounds__(NW * 64, 3) void panel_kernel(PanelArgs<T> a) {
+  constexpr int KB = NW * CG;  // panel width of this instantiation
+  __shared__ T s_l[kPR];
+  __shared__ T s_stage[2][KB];
+  __shared__ T s_pr[KB];   // pivot row (each wave fills and reads its own columns)
+  __shared__ T s_rc[KB];   // row c before the interchange
+  __shared__ int s_cand;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int blk = blockIdx.x;
+  const int rbase = blk * kPR;
+  const int r = rbase + lane;  // panel-relative row
+  const bool valid = r < a.m;
+  const size_t lda = (size_t)a.lda;
+  T* Ap = a.A + (size_t)a.k0 + (size_t)a.k0 * lda;
+  const int G = a.G;
+  T v[CG];
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const double ov = __shfl_xor(best, off, 64);
-      const int orr = __shfl_xor(brow, off, 64);
-      better(best, brow, ov, orr);
-    }
-    if (lane == 0) { red_v[wave] = best; red_r[wave] = brow; }
-    __syncthreads();
-    if (tid == 0) {
-      double b = red_v[0];
-      int br = red_r[0];
-      for (int w = 1; w < nwaves; ++w) better(b, br, red_v[w], red_r[w]);
-      a.st_val[par * a.G + wg] = b;
-      a.st_row[par * a.G + wg] = br;
-      s_lwin = (br == INT_MAX) ? -1 : br - r0;
-    }
-    __syncthreads();
-    // 2. stage the candidate row and (owner of row c) the current row c
-    const int lw = s_lwin;
-    if (lw >= 0)
-      for (int cc = tid; cc < kb; cc += kPanelThreads)
-        a.st_data[((size_t)par * a.G + wg) * kb + cc] = tile[cc * rp + lw];
-    if (c >= r0 && c < r0 + nr)
-      for (int cc = tid; cc < kb; cc += kPanelThreads)
-        a.st_rowc[(size_t)par * kb + cc] = tile[cc * rp + (c - r0)];
-    grid.sync();
-    // 3. global winner (every workgroup computes the same answer)
-    {
-      double b = -1.0;
-      int br = INT_MAX, bw = -1;
-      for (int w = tid; w < a.G; w += kPanelThreads) {
-        const double v = a.st_val[par * a.G + w];
-        const int r = a.st_row[par * a.G + w];
-        if (r != INT_MAX && (v > b || (v == b && r < br))) { b = v; br = r; bw = w; }
+  for (int j = 0; j < CG; ++j) {
+    const int col = w * CG + j;
+    v[j] = (valid && col < a.kb) ? Ap[(size_t)r + (size_t)col * lda] : T(0);
+  }
+  unsigned target = 0;
+  for (int g = 0; g < NW; ++g) {
+    if (g * CG >= a.kb) break;
+#pragma unroll
+    for (int jc = 0; jc < CG; ++jc) {
+      const int c = g * CG + jc;
+      if (c < a.kb) {
+      const int par = c & 1;
+      // 1. local candidate in wave g
+      if (w == g) {
+        double bv = (valid && r >= c) ? fabs((double)v[jc]) : -1.0;
+        int br = (valid && r >= c) ? r : INT_MAX;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          const double ov = __shfl_xor(bv, off, 64);
+          const int orr = __shfl_xor(br, off, 64);
+          if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+        }
+        if (lane == 0) {
+          a.st_val[par * G + blk] = bv;
+          a.st_row[par * G + blk] = br;
+          s_cand = (br == INT_MAX) ? -1 : br - rbase;
+        }
+      }
+      __syncthreads();
+      const int cl = s_cand;
+      const bool own_c = (c >= rbase) && (c < rbase + kPR);
+      if (cl >= 0 && lane == cl) {
+#pragma unroll
+        for (int j = 0; j < CG; ++j) s_stage[0][w * CG + j] = v[j];
+      }
+      if (own_c && lane == c - rbase) {
+#pragma unroll
+        for (int j = 0; j < CG; ++j) s_stage[1][w * CG + j] = v[j];
+      }
+      __syncthreads();
+      if (cl >= 0)
+        for (int t = tid; t < KB; t += NW * 64)
+          a.st_data[((size_t)par * G + blk) * KB + t] = s_stage[0][t];
+      if (own_c)
+        for (int t = tid; t < KB; t += NW * 64) a.st_rowc[(size_t)par * KB + t] = s_stage[1][t];
+      // 2. grid barrier
+      target += (unsigned)G;
+      grid_barrier(a.bar, target);
+      // 3. global winner (each wave independently)
+      double bv = -1.0;
+      int br = INT_MAX, bb = 0;
+      for (int q = lane; q < G; q += 64) {
+        const double vq = a.st_val[par * G + q];
+        const int rq = a.st_row[par * G + q];
+        if (rq != INT_MAX && better(vq, rq, bv, br)) { bv = vq; br = rq; bb = q; }
       }
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(b, off, 64);
+        const double ov = __shfl_xor(bv, off, 64);
         const int orr = __shfl_xor(br, off, 64);
-        const int ow = __shfl_xor(bw, off, 64);
-        if (ov > b || (ov == b && orr < br)) { b = ov; br = orr; bw = ow; }
+        const int ob = __shfl_xor(bb, off, 64);
+        if (better(ov, orr, bv, br)) { bv = ov; br = orr; bb = ob; }
+      }
+      const int p = __builtin_amdgcn_readfirstlane(br);
+      const int pb = __builtin_amdgcn_readfirstlane(bb);
+      if (!(bv > 0.0)) {  // zero column: no interchange, no scaling (LAPACK)
+        if (blk == 0 && tid == 0) {
+          a.ipiv[a.k0 + c] = a.k0 + c;
+          atomicCAS(a.info, 0, a.k0 + c + 1);
+        }
+      } else {
+      // 4. pivot row of this wave's columns into LDS (lane j loads column j;
+      //    LDS operations of one wave complete in order), interchange
+      T* pr = s_pr + w * CG;
+      if (lane < CG) pr[lane] = a.st_data[((size_t)par * G + pb) * KB + w * CG + lane];
+      if (p != c) {
+        T* rc = s_rc + w * CG;
+        if (lane < CG) rc[lane] = a.st_rowc[(size_t)par * KB + w * CG + lane];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && r == p) {
+#pragma unroll
+          for (int j = 0; j < CG; ++j) v[j] = rc[j];
+        }
+        if (valid && r == c) {
+#pragma unroll
+          for (int j = 0; j < CG; ++j) v[j] = pr[j];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (blk == 0 && tid == 0) a.ipiv[a.k0 + c] = a.k0 + p;
+      // 5. multipliers (wave g) and rank-1 update of rows below c
+      if (w == g) {
+        const bool below = valid && r > c;
+        const T l = below ? v[jc] / pr[jc] : T(0);
+        if (below) v[jc] = l;
+        s_l[lane] = l;
       }
       __syncthreads();
-      if (lane == 0) { red_v[wave] = b; red_r[wave] = br; red_w[wave] = bw; }
-      __syncthreads();
-      if (tid == 0) {
-        double bb = red_v[0];
-        int rr = red_r[0], ww = red_w[0];
-        for (int w = 1; w < nwaves; ++w) {
-          if (red_v[w] > bb || (red_v[w] == bb && red_r[w] < rr)) {
-            bb = red_v[w]; rr = red_r[w]; ww = red_w[w];
+      const T l = s_l[lane];
+      // (pivot-row reads in chunks of 8 columns: bounded register live ranges)
+      if (w > g) {
+#pragma unroll
+        for (int j0 = 0; j0 < CG; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8; ++j) v[j] -= l * pr[j];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else if (w == g) {
+#pragma unroll
+        for (int j0 = 0; j0 < CG; j0 += 8) {
+          if (j0 + 8 > jc + 1) {
+#pragma unroll
+            for (int j = (j0 > jc + 1 ? j0 : jc + 1); j < j0 + 8; ++j) v[j] -= l * pr[j];
+            __builtin_amdgcn_sched_barrier(0);
           }
         }
-        s_gwin_val = bb;
-        s_gwin_row = rr;
-        s_gwin_wg = ww;
       }
-      __syncthreads();
+      }  // nonzero pivot
+      }  // c < kb
     }
-    const int p = s_gwin_row;     // panel-relative pivot row
-    const int pw = s_gwin_wg;
-    const bool zero = !(s_gwin_val > 0.0);
-    for (int cc = tid; cc < kb; cc += kPanelThreads)
-      s_prow[cc] = a.st_data[((size_t)par * a.G + pw) * kb + cc];
-    __syncthreads();
-    if (zero) {
-      if (wg == 0 && tid == 0) {
-        a.ipiv[a.k0 + c] = a.k0 + c;
-        atomicCAS(a.info, 0, a.k0 + c + 1);
-      }
-      continue;  // LAPACK: no interchange, no scaling for a zero column
-    }
-    // 4. interchange rows c and p inside the panel
-    if (p != c) {
-      if (p >= r0 && p < r0 + nr)
-        for (int cc = tid; cc < kb; cc += kPanelThreads)
-          tile[cc * rp + (p - r0)] = a.st_rowc[(size_t)par * kb + cc];
-      if (c >= r0 && c < r0 + nr)
-        for (int cc = tid; cc < kb; cc += kPanelThreads) tile[cc * rp + (c - r0)] = s_prow[cc];
-    }
-    if (wg == 0 && tid == 0) a.ipiv[a.k0 + c] = a.k0 + p;
-    __syncthreads();
-    // 5. multipliers and rank-1 update of rows below c
-    const T piv = s_prow[c];
-    const int lr1 = max(0, c + 1 - r0);
-    const int na = nr - lr1;
-    if (na > 0) {
-      for (int r = lr1 + tid; r < nr; r += kPanelThreads) tile[c * rp + r] = tile[c * rp + r] / piv;
-      __syncthreads();
-      const int ncol = kb - c - 1;
-      for (int idx = tid; idx < na * ncol; idx += kPanelThreads) {
-        const int r = lr1 + idx % na;
-        const int cc = c + 1 + idx / na;
-        tile[cc * rp + r] -= tile[c * rp + r] * s_prow[cc];
-      }
-    }
-    __syncthreads();
   }
-  for (int idx = tid; idx < nr * kb; idx += kPanelThreads) {
-    const int r = idx % nr, c = idx / nr;
-    Ap[(size_t)(r0 + r) + (size_t)c * lda] = tile[c * rp + r];
+#pragma unroll
+  for (int j = 0; j < CG; ++j) {
+    const int col = w * CG + j;
+    if (valid && col < a.kb) Ap[(size_t)r + (size_t)col * lda] = v[j];
   }
 }
 
@@ -287,34 +319,39 @@ __global__ __launch_bounds__(256) void perm_apply(T* __restrict__ A, int lda, in
 }
 
 // ---------------------------------------------------------------------------
-// U12 = L11^-1 A12 (L11 unit lower, kb x kb). One workgroup per NC columns.
+// Base case of the recursive TRSM: X = L^-1 B for a unit-lower w x w block
+// (w <= 32) staged in LDS; one workgroup per NC columns of B.
 // ---------------------------------------------------------------------------
+constexpr int kTrsmBase = 32;
+
 template <typename T, int NC>
-__global__ __launch_bounds__(256) void trsm_kernel(int w, int ncols, const T* __restrict__ L,
-                                                   int lda, T* __restrict__ B, int ldb) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* tile = reinterpret_cast<T*>(smem_raw);  // [NC][w]
+__global__ __launch_bounds__(256) void trsm_base_kernel(int w, int ncols, const T* __restrict__ L,
+                                                        int lda, T* __restrict__ B, int ldb) {
+  __shared__ T Ls[kTrsmBase][kTrsmBase + 1];
+  __shared__ T tile[NC][kTrsmBase + 1];
   const int cbase = blockIdx.x * NC;
   const int nc = min(NC, ncols - cbase);
   const int tid = threadIdx.x;
+  for (int idx = tid; idx < w * w; idx += 256) {
+    const int r = idx % w, k = idx / w;
+    Ls[r][k] = L[(size_t)r + (size_t)k * lda];
+  }
   for (int idx = tid; idx < w * nc; idx += 256) {
     const int r = idx % w, c = idx / w;
-    tile[c * w + r] = B[(size_t)r + (size_t)(cbase + c) * ldb];
+    tile[c][r] = B[(size_t)r + (size_t)(cbase + c) * ldb];
   }
   __syncthreads();
-  constexpr int RG = 256 / NC;
-  const int c = tid % NC, rg = tid / NC;
-  for (int s = 0; s < w - 1; ++s) {
-    if (c < nc) {
-      const T xs = tile[c * w + s];
-      const T* Ls = L + (size_t)s * lda;
-      for (int r = s + 1 + rg; r < w; r += RG) tile[c * w + r] -= Ls[r] * xs;
+  // one thread per column: sequential forward substitution in LDS
+  for (int c = tid; c < nc; c += 256) {
+    for (int s = 0; s < w - 1; ++s) {
+      const T xs = tile[c][s];
+      for (int r = s + 1; r < w; ++r) tile[c][r] -= Ls[r][s] * xs;
     }
-    __syncthreads();
   }
+  __syncthreads();
   for (int idx = tid; idx < w * nc; idx += 256) {
-    const int r = idx % w, cc = idx / w;
-    B[(size_t)r + (size_t)(cbase + cc) * ldb] = tile[cc * w + r];
+    const int r = idx % w, c = idx / w;
+    B[(size_t)r + (size_t)(cbase + c) * ldb] = tile[c][r];
   }
 }
 
@@ -343,11 +380,11 @@ struct Mfma<double> {
 constexpr int kBM = 128, kBN = 128, kBK = 16;
 
 template <typename T>
-__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const T* __restrict__ A,
+__global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, const T* __restrict__ A,
                                                    int lda, const T* __restrict__ B, int ldb,
                                                    T* __restrict__ C, int ldc) {
   using acc_t = typename Mfma<T>::acc_t;
-  constexpr int PAD = (sizeof(T) == 4) ? 4 : 2;
+  constexpr int PAD = (sizeof(T) == 4) ? 16 : 2;
   __shared__ T As[2][kBK][kBM + PAD];
   __shared__ T Bs[2][kBK][kBN + PAD];
   const int bm = blockIdx.x * kBM, bn = blockIdx.y * kBN;
@@ -429,7 +466,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const T*
 }
 
 // ---------------------------------------------------------------------------
-// Triangular solves with FP64 accumulation (factors in T).
+// Triangular solves with FP64 accumulation (factors in T), blocked by kSolveNB
+// with the inverses of the diagonal blocks precomputed once per factorization.
 // ---------------------------------------------------------------------------
 constexpr int kSolveNB = 128;
 
@@ -439,90 +477,107 @@ __global__ void gather_kernel(int n, const int* __restrict__ perm, const double*
   if (i < n) y[i] = b[perm[i]];
 }
 
-// Solve the diagonal block [s0, s0+nb) of L (unit, lower) or U (upper).
+// Inverse of diagonal block b of L (unit lower) or U (upper): one workgroup of
+// kSolveNB threads per block; thread c builds column c in LDS. The factor's
+// entries are read at lane-uniform addresses.
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsv_diag(int nb, int s0, const T* __restrict__ F,
-                                                 int lda, double* __restrict__ y) {
-  __shared__ double ys[kSolveNB];
-  const int tid = threadIdx.x;
-  for (int r = tid; r < nb; r += 256) ys[r] = y[s0 + r];
-  __syncthreads();
+__global__ __launch_bounds__(kSolveNB) void tri_inv_kernel(int n, const T* __restrict__ F, int lda,
+                                                          T* __restrict__ Dinv) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* X = reinterpret_cast<T*>(smem_raw);  // [kSolveNB][kSolveNB], X[i*NB + c]
+  const int b = blockIdx.x;
+  const int s0 = b * kSolveNB;
+  const int nb = min(kSolveNB, n - s0);
+  const int c = threadIdx.x;
   const T* Fb = F + (size_t)s0 + (size_t)s0 * lda;
   if (LOWER) {
-    for (int s = 0; s < nb - 1; ++s) {
-      const double xs = ys[s];
-      for (int r = s + 1 + tid; r < nb; r += 256) ys[r] -= (double)Fb[r + (size_t)s * lda] * xs;
-      __syncthreads();
+    for (int i = 0; i < nb; ++i) {
+      double acc = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) acc -= (double)Fb[i + (size_t)k * lda] * (double)X[k * kSolveNB + c];
+      X[i * kSolveNB + c] = (T)acc;
     }
   } else {
-    for (int s = nb - 1; s >= 0; --s) {
-      if (tid == 0) ys[s] = ys[s] / (double)Fb[s + (size_t)s * lda];
-      __syncthreads();
-      const double xs = ys[s];
-      for (int r = tid; r < s; r += 256) ys[r] -= (double)Fb[r + (size_t)s * lda] * xs;
-      __syncthreads();
+    for (int i = nb - 1; i >= 0; --i) {
+      double acc = (i == c) ? 1.0 : 0.0;
+      for (int k = i + 1; k < nb; ++k) acc -= (double)Fb[i + (size_t)k * lda] * (double)X[k * kSolveNB + c];
+      X[i * kSolveNB + c] = (T)(acc / (double)Fb[i + (size_t)i * lda]);
     }
   }
-  for (int r = tid; r < nb; r += 256) y[s0 + r] = ys[r];
+  T* D = Dinv + (size_t)b * kSolveNB * kSolveNB;
+  for (int i = 0; i < nb; ++i) D[i + (size_t)c * kSolveNB] = (c < nb) ? X[i * kSolveNB + c] : T(0);
 }
 
-// y[r] -= sum_{s in [s0, s0+nb)} F[r, s] y[s] for rows r in [r0, r1).
-template <typename T>
-__global__ __launch_bounds__(256) void trsv_update(int r0, int r1, int s0, int nb,
-                                                   const T* __restrict__ F, int lda,
-                                                   double* __restrict__ y) {
+// One block step of a triangular solve: z = Dinv_b y_b (written to x_b), then
+// y_r -= sum_k F[r, s0+k] z_k for rows after (LOWER) or before (upper) the block.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void trsv_block_kernel(int n, int s0, int nb,
+                                                         const T* __restrict__ F, int lda,
+                                                         const T* __restrict__ D,
+                                                         double* __restrict__ y,
+                                                         double* __restrict__ x) {
   __shared__ double ys[kSolveNB];
-  for (int t = threadIdx.x; t < nb; t += 256) ys[t] = y[s0 + t];
+  __shared__ double z[kSolveNB];
+  const int tid = threadIdx.x;
+  if (tid < nb) ys[tid] = y[s0 + tid];
   __syncthreads();
-  const int r = r0 + blockIdx.x * 256 + threadIdx.x;
-  if (r >= r1) return;
+  if (tid < nb) {
+    double acc = 0.0;
+    for (int k = 0; k < nb; ++k) acc += (double)D[tid + (size_t)k * kSolveNB] * ys[k];
+    z[tid] = acc;
+    if (blockIdx.x == 0) x[s0 + tid] = acc;
+  }
+  __syncthreads();
+  const int r = (LOWER ? s0 + nb : 0) + blockIdx.x * 256 + tid;
+  const int rend = LOWER ? n : s0;
+  if (r >= rend) return;
   double acc = 0.0;
   const T* Fr = F + (size_t)r + (size_t)s0 * lda;
-  for (int s = 0; s < nb; ++s) acc += (double)Fr[(size_t)s * lda] * ys[s];
+  for (int k = 0; k < nb; ++k) acc += (double)Fr[(size_t)k * lda] * z[k];
   y[r] -= acc;
 }
 
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
-struct LuWork {
-  double* st_val;
-  int* st_row;
-  void* st_data;
-  void* st_rowc;
-  int* ldst;
-  int* lsrc;
-  int* llen;
-  int* info;
-};
-
 template <typename T>
-int choose_panel(bemhip_ctx* c, int n, int* kb_out, int* G_out, int* rows_out) {
-  const int ncu = std::max(1, c->num_cu);
-  const size_t lds_cap = 150 * 1024;
-  int kb = (sizeof(T) == 8) ? 128 : 256;
-  for (; kb >= 16; kb /= 2) {
-    int G = std::min(ncu, std::max(1, (n + 127) / 128));
-    int rows = (n + G - 1) / G;
-    if ((size_t)rows * kb * sizeof(T) <= lds_cap) {
-      *kb_out = kb;
-      *G_out = G;
-      *rows_out = rows;
-      return BEMHIP_OK;
-    }
+int gemm_sub(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B, int ldb, T* C,
+             int ldc) {
+  if (M <= 0 || N <= 0 || K <= 0) return BEMHIP_OK;
+  dim3 grid((M + kBM - 1) / kBM, (N + kBN - 1) / kBN);
+  hipLaunchKernelGGL((gemm_kernel<T>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, B, ldb,
+                     C, ldc);
+  return BEMHIP_OK;
+}
+
+// B = L^-1 B, L unit lower w x w: split in halves, the off-diagonal block on MFMA.
+template <typename T>
+int trsm_rec(bemhip_ctx* c, int w, int ncols, const T* L, int lda, T* B, int ldb) {
+  if (w <= kTrsmBase) {
+    constexpr int NC = 64;
+    hipLaunchKernelGGL((trsm_base_kernel<T, NC>), dim3((ncols + NC - 1) / NC), dim3(256), 0,
+                       c->stream, w, ncols, L, lda, B, ldb);
+    return BEMHIP_OK;
   }
-  return fail(c, BEMHIP_ERR_ARG, "lu: matrix too large for the LDS-resident panel");
+  const int w1 = ((w / 2 + kTrsmBase - 1) / kTrsmBase) * kTrsmBase;
+  int st = trsm_rec(c, w1, ncols, L, lda, B, ldb);
+  if (st) return st;
+  st = gemm_sub(c, w - w1, ncols, w1, L + w1, lda, B, ldb, B + w1, ldb);
+  if (st) return st;
+  return trsm_rec(c, w - w1, ncols, L + (size_t)w1 + (size_t)w1 * lda, lda, B + w1, ldb);
 }
 
 template <typename T>
 int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host) {
-  int kb0, G0, rows0;
-  int st = choose_panel<T>(c, n, &kb0, &G0, &rows0);
-  if (st) return st;
+  // Panel width 128 = 4 waves x 32 columns for both precisions (the panel of
+  // 32769 rows stays register-resident with 3 workgroups per CU).
+  constexpr int NW = 4;
+  constexpr int CG = 32;
+  constexpr int KB = NW * CG;
+  const int Gmax = (n + kPR - 1) / kPR;
   // work buffers
-  const size_t need = sizeof(double) * 2 * G0 + sizeof(int) * 2 * G0 +
-                      sizeof(T) * 2 * (size_t)G0 * kb0 + sizeof(T) * 2 * kb0 +
-                      sizeof(int) * (4 * kMaxPanel + 8) + 1024;
+  const size_t need = sizeof(double) * 2 * Gmax + sizeof(int) * 2 * Gmax +
+                      sizeof(T) * 2 * (size_t)Gmax * KB + sizeof(T) * 2 * KB +
+                      sizeof(int) * (4 * kMaxPanel + 16) + 8 * 256;
   BEMHIP_HIP_TRY(c, c->lu_work.ensure(need));
   char* wp = c->lu_work.ptr;
   auto carve = [&](size_t bytes) {
@@ -530,90 +585,71 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     wp += (bytes + 255) & ~size_t(255);
     return p;
   };
-  LuWork w;
-  w.st_val = (double*)carve(sizeof(double) * 2 * G0);
-  w.st_row = (int*)carve(sizeof(int) * 2 * G0);
-  w.st_data = carve(sizeof(T) * 2 * (size_t)G0 * kb0);
-  w.st_rowc = carve(sizeof(T) * 2 * kb0);
-  w.ldst = (int*)carve(sizeof(int) * 2 * kMaxPanel);
-  w.lsrc = (int*)carve(sizeof(int) * 2 * kMaxPanel);
-  w.llen = (int*)carve(sizeof(int) * 8);
-  w.info = w.llen + 4;
+  double* st_val = (double*)carve(sizeof(double) * 2 * Gmax);
+  int* st_row = (int*)carve(sizeof(int) * 2 * Gmax);
+  T* st_data = (T*)carve(sizeof(T) * 2 * (size_t)Gmax * KB);
+  T* st_rowc = (T*)carve(sizeof(T) * 2 * KB);
+  int* ldst = (int*)carve(sizeof(int) * 2 * kMaxPanel);
+  int* lsrc = (int*)carve(sizeof(int) * 2 * kMaxPanel);
+  int* llen = (int*)carve(sizeof(int) * 16);
+  int* info = llen + 4;
+  unsigned* bar = (unsigned*)(llen + 8);
   if ((size_t)(wp - c->lu_work.ptr) > c->lu_work.count)
     return fail(c, BEMHIP_ERR_STATE, "lu: work buffer carve overflow");
-  BEMHIP_HIP_TRY(c, hipMemsetAsync(w.info, 0, sizeof(int), c->stream));
-
-  size_t max_lds = 0;  // largest panel over the whole factorization
-  for (int k0 = 0; k0 < n; k0 += kb0) {
-    const int m = n - k0, kb = std::min(kb0, n - k0);
-    const int G = std::min(G0, std::max(1, (m + 127) / 128));
-    const int rows = (m + G - 1) / G;
-    max_lds = std::max(max_lds, (size_t)rows * kb * sizeof(T));
-  }
-  if (max_lds > 160 * 1024) return fail(c, BEMHIP_ERR_STATE, "lu: panel exceeds LDS");
-  BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)panel_kernel<T>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)max_lds));
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(info, 0, sizeof(int), c->stream));
   int occ = 0;
   BEMHIP_HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &occ, (const void*)panel_kernel<T>, kPanelThreads, max_lds));
-  if (occ < 1) return fail(c, BEMHIP_ERR_STATE, "lu: panel kernel cannot be resident");
-  constexpr int NC = (sizeof(T) == 8) ? 16 : 32;
-  const size_t trsm_lds = (size_t)kb0 * NC * sizeof(T);
-  BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)trsm_kernel<T, NC>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)trsm_lds));
-  double t_panel = 0, t_swap = 0, t_trsm = 0, t_gemm = 0;
-  const int npanels = (n + kb0 - 1) / kb0;
+                        &occ, (const void*)panel_kernel<T, NW, CG>, NW * 64, 0));
+  if ((int64_t)occ * std::max(1, c->num_cu) < Gmax)
+    return fail(c, BEMHIP_ERR_STATE, "lu: panel workgroups cannot all be resident");
+  c->dinv_of = nullptr;
+
+  const int npanels = (n + KB - 1) / KB;
   std::vector<hipEvent_t> ev((size_t)npanels * 5);
   for (auto& x : ev) hipEventCreate(&x);
-  const bool timing = true;
-  for (int k0 = 0; k0 < n; k0 += kb0) {
-    hipEvent_t* e = &ev[(size_t)(k0 / kb0) * 5];
-    const int kb = std::min(kb0, n - k0);
+  for (int k0 = 0; k0 < n; k0 += KB) {
+    hipEvent_t* e = &ev[(size_t)(k0 / KB) * 5];
+    const int kb = std::min(KB, n - k0);
     const int m = n - k0;
-    const int G = std::min(G0, std::max(1, (m + 127) / 128));
-    const int rows = (m + G - 1) / G;
-    if (timing) hipEventRecord(e[0], c->stream);
+    const int G = (m + kPR - 1) / kPR;
+    hipEventRecord(e[0], c->stream);
+    BEMHIP_HIP_TRY(c, hipMemsetAsync(bar, 0, sizeof(unsigned), c->stream));
     PanelArgs<T> pa;
-    pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = k0; pa.rows_per = rows; pa.G = G;
-    pa.ipiv = dipiv; pa.st_val = w.st_val; pa.st_row = w.st_row; pa.st_data = (T*)w.st_data;
-    pa.st_rowc = (T*)w.st_rowc; pa.info = w.info;
+    pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = k0; pa.G = G;
+    pa.ipiv = dipiv; pa.st_val = st_val; pa.st_row = st_row; pa.st_data = st_data;
+    pa.st_rowc = st_rowc; pa.info = info; pa.bar = bar;
     void* args[] = {&pa};
-    const size_t lds = (size_t)rows * kb * sizeof(T);
-    BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T>, dim3(G),
-                                                 dim3(kPanelThreads), args, lds, c->stream));
-    if (timing) hipEventRecord(e[1], c->stream);
+    BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, NW, CG>, dim3(G),
+                                                 dim3(NW * 64), args, 0, c->stream));
+    hipEventRecord(e[1], c->stream);
     // interchanges outside the panel
-    hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, c->stream, dipiv, k0, kb, w.ldst,
-                       w.lsrc, w.llen);
+    hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, c->stream, dipiv, k0, kb, ldst, lsrc,
+                       llen);
     const int ncols_out = n - kb;
-    if (ncols_out > 0) {
+    if (ncols_out > 0)
       hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_out + 3) / 4), dim3(256), 0, c->stream,
-                         dA, lda, 0, k0, k0 + kb, n, w.ldst, w.lsrc, w.llen);
-    }
+                         dA, lda, 0, k0, k0 + kb, n, ldst, lsrc, llen);
     BEMHIP_HIP_TRY(c, hipGetLastError());
-    if (timing) hipEventRecord(e[2], c->stream);
+    hipEventRecord(e[2], c->stream);
     const int nrest = n - k0 - kb;
     if (nrest > 0) {
       T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
       T* A12 = dA + (size_t)k0 + (size_t)(k0 + kb) * lda;
-      hipLaunchKernelGGL((trsm_kernel<T, NC>), dim3((nrest + NC - 1) / NC), dim3(256),
-                         (size_t)kb * NC * sizeof(T), c->stream, kb, nrest, L11, lda, A12, lda);
-      if (timing) hipEventRecord(e[3], c->stream);
+      int st = trsm_rec(c, kb, nrest, (const T*)L11, lda, A12, lda);
+      if (st) return st;
+      hipEventRecord(e[3], c->stream);
       T* L21 = dA + (size_t)(k0 + kb) + (size_t)k0 * lda;
       T* A22 = dA + (size_t)(k0 + kb) + (size_t)(k0 + kb) * lda;
-      dim3 grid((nrest + kBM - 1) / kBM, (nrest + kBN - 1) / kBN);
-      hipLaunchKernelGGL((gemm_kernel<T>), grid, dim3(256), 0, c->stream, nrest, nrest, kb, L21,
-                         lda, A12, lda, A22, lda);
+      gemm_sub(c, nrest, nrest, kb, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
       BEMHIP_HIP_TRY(c, hipGetLastError());
-      if (timing) hipEventRecord(e[4], c->stream);
+      hipEventRecord(e[4], c->stream);
     }
   }
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  for (int k0 = 0; k0 < n; k0 += kb0) {
-    hipEvent_t* e = &ev[(size_t)(k0 / kb0) * 5];
-    const int nrest = n - k0 - std::min(kb0, n - k0);
+  double t_panel = 0, t_swap = 0, t_trsm = 0, t_gemm = 0;
+  for (int k0 = 0; k0 < n; k0 += KB) {
+    hipEvent_t* e = &ev[(size_t)(k0 / KB) * 5];
+    const int nrest = n - k0 - std::min(KB, n - k0);
     float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
     hipEventElapsedTime(&t01, e[0], e[1]);
     hipEventElapsedTime(&t12, e[1], e[2]);
@@ -624,42 +660,57 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     t_panel += t01; t_swap += t12; t_trsm += t23; t_gemm += t34;
   }
   for (auto& x : ev) hipEventDestroy(x);
-  c->ms[sizeof(T) == 8 ? "lu64_panel" : "lu32_panel"] = t_panel;
-  c->ms[sizeof(T) == 8 ? "lu64_swap" : "lu32_swap"] = t_swap;
-  c->ms[sizeof(T) == 8 ? "lu64_trsm" : "lu32_trsm"] = t_trsm;
-  c->ms[sizeof(T) == 8 ? "lu64_gemm" : "lu32_gemm"] = t_gemm;
-  c->stats[sizeof(T) == 8 ? "lu64_kb" : "lu32_kb"] = kb0;
-  c->stats[sizeof(T) == 8 ? "lu64_panel_wgs" : "lu32_panel_wgs"] = G0;
-  int info = 0;
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(&info, w.info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  const char* pfx = (sizeof(T) == 8) ? "lu64_" : "lu32_";
+  c->ms[std::string(pfx) + "panel"] = t_panel;
+  c->ms[std::string(pfx) + "swap"] = t_swap;
+  c->ms[std::string(pfx) + "trsm"] = t_trsm;
+  c->ms[std::string(pfx) + "gemm"] = t_gemm;
+  c->stats[std::string(pfx) + "kb"] = KB;
+  c->stats[std::string(pfx) + "panel_wgs"] = Gmax;
+  int infoh = 0;
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(&infoh, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (info_host) *info_host = info;
+  if (info_host) *info_host = infoh;
   return BEMHIP_OK;
 }
 
 template <typename T>
 int lu_solve_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dperm, double* d_b) {
-  BEMHIP_HIP_TRY(c, c->vec64c.ensure((size_t)n));
+  const int nblk = (n + kSolveNB - 1) / kSolveNB;
+  const size_t blk_elems = (size_t)kSolveNB * kSolveNB;
+  BEMHIP_HIP_TRY(c, c->dinv.ensure(2 * nblk * blk_elems * sizeof(T)));
+  T* DL = (T*)c->dinv.ptr;
+  T* DU = DL + nblk * blk_elems;
+  if (c->dinv_of != (const void*)dLU || c->dinv_n != n) {
+    const size_t lds = blk_elems * sizeof(T);
+    BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)tri_inv_kernel<T, true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)tri_inv_kernel<T, false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(nblk), dim3(kSolveNB), lds, c->stream, n,
+                       dLU, lda, DL);
+    hipLaunchKernelGGL((tri_inv_kernel<T, false>), dim3(nblk), dim3(kSolveNB), lds, c->stream, n,
+                       dLU, lda, DU);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    c->dinv_of = dLU;
+    c->dinv_n = n;
+  }
+  BEMHIP_HIP_TRY(c, c->vec64c.ensure(2 * (size_t)n));
   double* y = c->vec64c.ptr;
+  double* x = c->vec64c.ptr + n;
   const int tb = 256;
   hipLaunchKernelGGL(gather_kernel, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n, dperm,
                      d_b, y);
-  for (int s0 = 0; s0 < n; s0 += kSolveNB) {
-    const int nb = std::min(kSolveNB, n - s0);
-    hipLaunchKernelGGL((trsv_diag<T, true>), dim3(1), dim3(256), 0, c->stream, nb, s0, dLU, lda, y);
-    const int r0 = s0 + nb;
-    if (r0 < n)
-      hipLaunchKernelGGL((trsv_update<T>), dim3((n - r0 + 255) / 256), dim3(256), 0, c->stream,
-                         r0, n, s0, nb, dLU, lda, y);
+  for (int bi = 0; bi < nblk; ++bi) {  // forward: L x = y
+    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
+    const int rows = n - s0 - nb;
+    hipLaunchKernelGGL((trsv_block_kernel<T, true>), dim3(std::max(1, (rows + 255) / 256)),
+                       dim3(256), 0, c->stream, n, s0, nb, dLU, lda, DL + bi * blk_elems, y, x);
   }
-  const int nblk = (n + kSolveNB - 1) / kSolveNB;
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    const int s0 = bi * kSolveNB;
-    const int nb = std::min(kSolveNB, n - s0);
-    hipLaunchKernelGGL((trsv_diag<T, false>), dim3(1), dim3(256), 0, c->stream, nb, s0, dLU, lda, y);
-    if (s0 > 0)
-      hipLaunchKernelGGL((trsv_update<T>), dim3((s0 + 255) / 256), dim3(256), 0, c->stream, 0,
-                         s0, s0, nb, dLU, lda, y);
+  for (int bi = nblk - 1; bi >= 0; --bi) {  // backward: U y = x
+    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
+    hipLaunchKernelGGL((trsv_block_kernel<T, false>), dim3(std::max(1, (s0 + 255) / 256)),
+                       dim3(256), 0, c->stream, n, s0, nb, dLU, lda, DU + bi * blk_elems, x, y);
   }
   BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(d_b, y, sizeof(double) * n, hipMemcpyDeviceToDevice,
