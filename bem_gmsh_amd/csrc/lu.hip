@@ -4,7 +4,7 @@
 // This is the LU seam of bem_linsolve (src/Core/LinearSolveUtils.jl:44-49),
 // whose default for a dense square system is an LAPACK getrf/getrs.
 //
-// Blocked right-looking algorithm, panel width 256:
+// Blocked right-looking algorithm, panel width 128:
 //   panel_kernel   cooperative launch, one workgroup per 64 panel rows, the
 //                  panel held in registers; one grid barrier per column for the
 //                  partial-pivot search (GEPP, ties -> lowest row).
@@ -38,7 +38,7 @@ namespace {
 //
 // Workgroup = NW waves x 64 lanes. Lane l of every wave owns panel row
 // blockIdx.x*64 + l; wave w owns the CG panel columns [w*CG, (w+1)*CG), so a
-// thread holds CG values of one row. Panel width KB = NW*CG = 256.
+// thread holds CG values of one row. Panel width KB = NW*CG (4 x 32 = 128).
 // Per column c: the owning wave finds the local candidate (max |a|, lowest
 // row on ties), stages it, one grid barrier, every wave picks the global
 // winner, the pivot row's values (uniform across lanes) drive the interchange
