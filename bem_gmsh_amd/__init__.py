@@ -8,20 +8,23 @@
 """
 
 from .problem import BEMData, Element, Laplace
-from .mesh import (Mesh2D, SplitMix64, ellipse_mesh, format2d, formatdata, n_eff, quadrado,
-                   read_msh, star_mesh, star_params, write_msh)
+from .mesh import (Mesh2D, SplitMix64, ellipse_mesh, format2d, formatdata, move_geometric_node,
+                   n_eff, quadrado, read_msh, star_mesh, star_param_jacobian, star_params,
+                   write_msh)
 from .analytical import (AnalyticalSolution, ana_exp_cos, ana_laplace_linear,
                          ana_quadratic_mixed, analytical, analytical_flux, apply_analytical_bc,
                          apply_bc_from_physical, attach_analytical, rel_error, rel_error_flux)
-from .api import (ArgumentError, HipContext, apply_bc, apply_operator, assemble, get_HG,
-                  lu_factor, lu_solve, near_pairs, solve, split_sol)
+from .api import (ArgumentError, HipContext, adjoint, apply_bc, apply_operator, assemble,
+                  get_HG, lu_factor, lu_solve, near_pairs, solve, split_sol,
+                  unique_node_gradient)
 
 __all__ = [
     "BEMData", "Element", "Laplace", "Mesh2D", "SplitMix64", "ellipse_mesh", "format2d",
-    "formatdata", "n_eff", "quadrado", "read_msh", "star_mesh", "star_params", "write_msh",
+    "formatdata", "move_geometric_node", "n_eff", "quadrado", "read_msh", "star_mesh",
+    "star_param_jacobian", "star_params", "write_msh",
     "AnalyticalSolution", "ana_exp_cos", "ana_laplace_linear", "ana_quadratic_mixed",
     "analytical", "analytical_flux", "apply_analytical_bc", "apply_bc_from_physical",
     "attach_analytical", "rel_error", "rel_error_flux", "ArgumentError", "HipContext",
-    "apply_bc", "apply_operator", "assemble", "get_HG", "lu_factor", "lu_solve", "near_pairs",
-    "solve", "split_sol",
+    "adjoint", "apply_bc", "apply_operator", "assemble", "get_HG", "lu_factor", "lu_solve",
+    "near_pairs", "solve", "split_sol", "unique_node_gradient",
 ]

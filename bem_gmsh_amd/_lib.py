@@ -46,6 +46,7 @@ SIGNATURES = [
     ("bemhip_lu_factor", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p]),
     ("bemhip_lu_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p,
                                   c_double_p, C.c_int]),
+    ("bemhip_adjoint", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p]),
     ("bemhip_counters", C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     ("bemhip_synchronize", C.c_int, [C.c_void_p]),
 ]

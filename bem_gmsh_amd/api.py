@@ -180,6 +180,8 @@ def split_sol(dad: BEMData, x: np.ndarray):
 def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: int = 30):
     """Apply BCs and solve on the GPU. ``mode`` 'fp64' (FP64 LU) or 'mixed'
     (FP32 LU + FP64 refinement); default follows the assembly precision."""
+    if dad.hip is None or "assembly_info" not in dad.cache:
+        raise RuntimeError("cache.H is not set — assemble/solve first?")
     if mode is None:
         mode = "fp64" if dad.assembly_info["precision"] == "fp64" else "mixed"
     if mode not in MODES:
@@ -221,11 +223,52 @@ def lu_factor(A: np.ndarray, ctx: Optional[HipContext] = None):
 
 
 def lu_solve(LU: np.ndarray, ipiv: np.ndarray, b: np.ndarray,
-             ctx: Optional[HipContext] = None) -> np.ndarray:
+             ctx: Optional[HipContext] = None, trans: int = 0) -> np.ndarray:
+    """Solve A x = b (trans=0) or A^T x = b (trans=1) with the factors of lu_factor."""
     ctx = ctx or HipContext(0)
     dtype = 0 if LU.dtype == np.float64 else 1
     LUc = np.asfortranarray(LU)
     x = np.array(b, dtype=np.float64, copy=True)
     ctx.check(lib().bemhip_lu_solve(ctx.h, LU.shape[0], dtype, LUc.ctypes.data_as(C.c_void_p),
-                                    iptr(np.ascontiguousarray(ipiv, dtype=np.int32)), dptr(x), 0))
+                                    iptr(np.ascontiguousarray(ipiv, dtype=np.int32)), dptr(x),
+                                    int(trans)))
     return x
+
+
+def adjoint(dad: BEMData, q_target: np.ndarray, w: Optional[np.ndarray] = None):
+    """Adjoint shape gradient of J = 1/2 sum_j w_j (q_j - q_target_j)^2 for the
+    system of the last ``solve`` (no counterpart in the reference).
+
+    ``w`` defaults to the boundary quadrature weights J_j w_k, which are then
+    differentiated too; BC values and internal points are held fixed.
+    Returns (J, dJ_dgeo) with dJ_dgeo shaped like ``dad.elem_geo``
+    (ne, p+1, 2): the derivative with respect to every element's copy of its
+    geometric nodes (see ``unique_node_gradient`` for shared end nodes).
+    """
+    ctx = dad.hip
+    if ctx is None or "solve_info" not in dad.cache:
+        raise RuntimeError("cache.T is not set — assemble/solve first?")
+    qt = np.ascontiguousarray(q_target, dtype=np.float64)
+    if qt.shape != (dad.n,):
+        raise ArgumentError(f"q_target must have n = {dad.n} entries")
+    wv = None
+    if w is not None:
+        wv = np.ascontiguousarray(w, dtype=np.float64)
+        if wv.shape != (dad.n,):
+            raise ArgumentError(f"w must have n = {dad.n} entries")
+    out = np.empty((dad.ne, dad.p + 1, 2), dtype=np.float64)
+    J = C.c_double(0.0)
+    ctx.check(lib().bemhip_adjoint(ctx.h, dptr(qt), dptr(wv) if wv is not None else None,
+                                   dptr(out), C.byref(J)))
+    return J.value, out
+
+
+def unique_node_gradient(dad: BEMData, dgeo: np.ndarray):
+    """Sum per-element geometric-node gradients over coincident nodes.
+    Returns (nodes (m, 2), grad (m, 2), index (ne, p+1)) with
+    dgeo[e, k] contributing to grad[index[e, k]]."""
+    pts = np.ascontiguousarray(dad.elem_geo.reshape(-1, 2))
+    nodes, index = np.unique(pts, axis=0, return_inverse=True)
+    grad = np.zeros_like(nodes)
+    np.add.at(grad, index.reshape(-1), dgeo.reshape(-1, 2))
+    return nodes, grad, index.reshape(dad.ne, dad.p + 1)

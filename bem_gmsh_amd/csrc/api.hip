@@ -111,6 +111,7 @@ static void build_quadconst(bemhip_ctx* c, QuadConst* q) {
     q->eq_bw[j] = ((odd_n != even_j) ? 1.0 : -1.0) * std::round(binom);
   }
   diff_matrix(nn, q->eq_x, q->eq_bw, q->eq_D);
+  for (int j = 0; j < nn && j < (int)c->h_elem_w.size(); ++j) q->ew[j] = c->h_elem_w[j];
   q->k = c->k;
   q->near_factor = c->near_factor;
 }
@@ -241,6 +242,7 @@ int bemhip_set_geometry(bemhip_ctx* c, int p, int ne, int n, int ni, const doubl
   c->assembled = false;
   c->has_bc = false;
   c->factored32 = c->factored64 = false;
+  c->has_sol = false;
   c->p = p; c->nn = nn; c->ne = ne; c->n = n; c->ni = ni; c->nt = nt; c->k = k;
   c->h_elem_w.assign(elem_w, elem_w + nn);
   std::vector<double> hx(nt), hy(nt), hnx(n), hny(n), hjw(n), hgx((size_t)ne * nn),
@@ -265,6 +267,8 @@ int bemhip_set_geometry(bemhip_ctx* c, int p, int ne, int n, int ni, const doubl
     return st;
   std::vector<double> hl(elen, elen + ne);
   if ((st = up(c->elen, hl))) return st;
+  std::vector<double> hJ(J, J + n);
+  if ((st = up(c->Jn, hJ))) return st;
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->has_geo = true;
   return BEMHIP_OK;
@@ -281,6 +285,7 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   c->assembled = false;
   c->has_bc = false;
   c->factored32 = c->factored64 = false;
+  c->has_sol = false;
   c->npg = npg;
   c->near_factor = near_factor;
   c->precision = precision;
@@ -346,6 +351,7 @@ int bemhip_apply_bc(bemhip_ctx* c, const int64_t* BC, const double* BV, double* 
   hipSetDevice(c->device);
   c->has_bc = false;
   c->factored32 = c->factored64 = false;
+  c->has_sol = false;
   BEMHIP_HIP_TRY(c, c->d_bc.ensure(c->n));
   BEMHIP_HIP_TRY(c, c->d_bv.ensure(c->n));
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->d_bc.ptr, hbc.data(), sizeof(int) * c->n,

@@ -10,33 +10,9 @@
 #include <vector>
 
 #include "../../include/bemhip.h"
+#include "quadconst.hpp"
 
 namespace bemhip {
-
-constexpr int kWave = 64;          // CDNA wavefront width
-constexpr int kMaxNpg = 64;        // largest near-field Gauss rule
-constexpr int kNinterp = 20;       // Guiggiani interpolation nodes (reference default)
-constexpr int kEB = 32;            // elements per far-field column block (one bit word)
-constexpr int kFarThreads = 256;   // rows per far-field workgroup
-
-// Quadrature constants shared by the near/singular kernels (device buffer).
-struct QuadConst {
-  int npg;
-  int p;
-  double gl_x[kMaxNpg];     // near-field Gauss-Legendre rule
-  double gl_w[kMaxNpg];
-  double g20_x[kNinterp];   // interpolated-Guiggiani rule (ninterp = 20)
-  double g20_w[kNinterp];
-  double g20_bw[kNinterp];  // barycentric weights 1/prod(x_i - x_j)
-  double lp_x[4];           // Legendre(p) collocation nodes
-  double lp_bw[4];          // Legendre(p) barycentric weights
-  double lp_D[16];          // Legendre(p) differentiation matrix, row-major
-  double eq_x[4];           // Equispaced(p) nodes
-  double eq_bw[4];          // Equispaced(p) barycentric weights
-  double eq_D[16];          // Equispaced(p) differentiation matrix, row-major
-  double k;                 // conductivity
-  double near_factor;
-};
 
 // Device geometry (structure of arrays).
 struct Geo {
@@ -114,9 +90,16 @@ struct bemhip_ctx {
   bemhip::DevBuf<double> part1, part2;     // [ncb][nt] operator partial sums
   bemhip::DevBuf<double> A64;
   bemhip::DevBuf<float> A32;
-  bemhip::DevBuf<int> ipiv;
-  bemhip::DevBuf<int> perm;
+  bemhip::DevBuf<int> ipiv, ipiv32;   // pivots of the FP64 / FP32 factorization
+  bemhip::DevBuf<int> perm, perm32;   // gather permutations built from them
   bool factored64 = false, factored32 = false;
+
+  // solution of the last bemhip_solve (kept for the adjoint)
+  bool has_sol = false;
+  int sol_mode = 0;
+  bemhip::DevBuf<double> xsol;
+  bemhip::DevBuf<double> Jn;        // n |dx/dxi| at each boundary node
+  bemhip::DevBuf<double> adj_work;  // adjoint scratch (carved per call)
 
   // scratch for LU and solves
   bemhip::DevBuf<char> lu_work;
@@ -168,7 +151,10 @@ int lu_factor_device(bemhip_ctx* c, int n, int dtype, void* dA, int lda, int* di
                      int* info_host);
 int lu_solve_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int lda,
                     const int* dperm, double* d_b);
+int lu_solve_trans_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int lda,
+                          const int* dperm, double* d_b);
 int build_perm(bemhip_ctx* c, int n, const int* dipiv, int* dperm);
+int operator_apply_trans(bemhip_ctx* c, const double* d_v, double* d_y);
 
 // Small host helpers
 void gauss_legendre_host(int n, double* x, double* w);

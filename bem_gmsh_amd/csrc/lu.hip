@@ -17,7 +17,8 @@
 //                  v_mfma_f32_16x16x4f32), 128x128 block tiles, LDS double buffer.
 // Solves: row permutation as a gather, then blocked forward / backward
 // substitution with precomputed inverses of the 128x128 diagonal blocks and
-// FP64 accumulation (one launch per block).
+// FP64 accumulation (one launch per block). Transposed solves (the adjoint)
+// use the dot-product form over the factor's contiguous columns.
 
 #include "common.hpp"
 
@@ -536,6 +537,50 @@ __global__ __launch_bounds__(256) void trsv_block_kernel(int n, int s0, int nb,
   y[r] -= acc;
 }
 
+// Transposed solves A^T x = b with A = P^T L U: U^T w = b (forward), L^T z = w
+// (backward), x = P^T z. Left-looking dot-product form: block bi needs, for
+// each of its columns j, s_j = sum_r F[r, j] v[r] over the already solved
+// rows r, which reads column j of the factor contiguously (one workgroup per
+// column), then v_b = Dinv_b^T (b_b - s).
+template <typename T>
+__global__ __launch_bounds__(256) void dot_cols_kernel(const T* __restrict__ F, int lda, int c0,
+                                                       int r0, int r1,
+                                                       const double* __restrict__ v,
+                                                       double* __restrict__ s) {
+  __shared__ double red[4];
+  const int j = c0 + blockIdx.x;
+  const T* Fc = F + (size_t)j * lda;
+  double acc = 0.0;
+  for (int r = r0 + threadIdx.x; r < r1; r += 256) acc += (double)Fc[r] * v[r];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) s[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// v[s0 + i] = sum_k D[k + i*NB] (b[s0 + k] - s[k]), i < nb (D^T applied).
+template <typename T>
+__global__ __launch_bounds__(kSolveNB) void diag_t_kernel(int s0, int nb, const T* __restrict__ D,
+                                                          const double* __restrict__ b,
+                                                          const double* __restrict__ s,
+                                                          double* __restrict__ v) {
+  __shared__ double t[kSolveNB];
+  const int i = threadIdx.x;
+  if (i < nb) t[i] = b[s0 + i] - s[i];
+  __syncthreads();
+  if (i >= nb) return;
+  const T* Dc = D + (size_t)i * kSolveNB;
+  double acc = 0.0;
+  for (int k = 0; k < nb; ++k) acc += (double)Dc[k] * t[k];
+  v[s0 + i] = acc;
+}
+
+__global__ void scatter_kernel(int n, const int* __restrict__ perm, const double* __restrict__ z,
+                               double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[perm[i]] = z[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
@@ -675,7 +720,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
 }
 
 template <typename T>
-int lu_solve_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dperm, double* d_b) {
+int ensure_dinv(bemhip_ctx* c, int n, const T* dLU, int lda) {
   const int nblk = (n + kSolveNB - 1) / kSolveNB;
   const size_t blk_elems = (size_t)kSolveNB * kSolveNB;
   BEMHIP_HIP_TRY(c, c->dinv.ensure(2 * nblk * blk_elems * sizeof(T)));
@@ -695,6 +740,52 @@ int lu_solve_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dperm, do
     c->dinv_of = dLU;
     c->dinv_n = n;
   }
+  return BEMHIP_OK;
+}
+
+// A^T x = b in place of d_b, with P A = L U (perm: (P v)[i] = v[perm[i]]).
+template <typename T>
+int lu_solve_trans_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dperm,
+                     double* d_b) {
+  int st = ensure_dinv<T>(c, n, dLU, lda);
+  if (st) return st;
+  const int nblk = (n + kSolveNB - 1) / kSolveNB;
+  const size_t blk_elems = (size_t)kSolveNB * kSolveNB;
+  const T* DL = (const T*)c->dinv.ptr;
+  const T* DU = DL + nblk * blk_elems;
+  BEMHIP_HIP_TRY(c, c->vec64c.ensure(2 * (size_t)n + kSolveNB));
+  double* w = c->vec64c.ptr;
+  double* z = c->vec64c.ptr + n;
+  double* s = c->vec64c.ptr + 2 * (size_t)n;
+  for (int bi = 0; bi < nblk; ++bi) {  // U^T w = b: rows [0, s0) of U's columns
+    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
+    hipLaunchKernelGGL((dot_cols_kernel<T>), dim3(nb), dim3(256), 0, c->stream, dLU, lda, s0, 0,
+                       s0, w, s);
+    hipLaunchKernelGGL((diag_t_kernel<T>), dim3(1), dim3(kSolveNB), 0, c->stream, s0, nb,
+                       DU + bi * blk_elems, d_b, s, w);
+  }
+  for (int bi = nblk - 1; bi >= 0; --bi) {  // L^T z = w: rows [s0+nb, n) of L's columns
+    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
+    hipLaunchKernelGGL((dot_cols_kernel<T>), dim3(nb), dim3(256), 0, c->stream, dLU, lda, s0,
+                       s0 + nb, n, z, s);
+    hipLaunchKernelGGL((diag_t_kernel<T>), dim3(1), dim3(kSolveNB), 0, c->stream, s0, nb,
+                       DL + bi * blk_elems, w, s, z);
+  }
+  const int tb = 256;
+  hipLaunchKernelGGL(scatter_kernel, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n, dperm,
+                     z, d_b);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  return BEMHIP_OK;
+}
+
+template <typename T>
+int lu_solve_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dperm, double* d_b) {
+  int st = ensure_dinv<T>(c, n, dLU, lda);
+  if (st) return st;
+  const int nblk = (n + kSolveNB - 1) / kSolveNB;
+  const size_t blk_elems = (size_t)kSolveNB * kSolveNB;
+  const T* DL = (const T*)c->dinv.ptr;
+  const T* DU = DL + nblk * blk_elems;
   BEMHIP_HIP_TRY(c, c->vec64c.ensure(2 * (size_t)n));
   double* y = c->vec64c.ptr;
   double* x = c->vec64c.ptr + n;
@@ -730,6 +821,12 @@ int lu_solve_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int lda, c
                     double* d_b) {
   if (dtype == 0) return lu_solve_t<double>(c, n, (const double*)dLU, lda, dperm, d_b);
   return lu_solve_t<float>(c, n, (const float*)dLU, lda, dperm, d_b);
+}
+
+int lu_solve_trans_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int lda,
+                          const int* dperm, double* d_b) {
+  if (dtype == 0) return lu_solve_trans_t<double>(c, n, (const double*)dLU, lda, dperm, d_b);
+  return lu_solve_trans_t<float>(c, n, (const float*)dLU, lda, dperm, d_b);
 }
 
 // Sequential LAPACK-style interchanges -> gather permutation Pb[i] = b[perm[i]].

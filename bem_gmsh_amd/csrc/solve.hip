@@ -38,6 +38,19 @@ __global__ void round_f32_kernel(size_t count, const double* __restrict__ s, flo
     d[t] = (float)s[t];
 }
 
+// A32 = FP32 of the column swap of H64, G64 (used once A64 holds FP64 factors).
+__global__ void swap_round_kernel(int nt, int n, const int* __restrict__ bc,
+                                  const double* __restrict__ H, const double* __restrict__ G,
+                                  float* __restrict__ A) {
+  const int j = blockIdx.y;
+  const size_t ld = (size_t)nt;
+  const bool dir = (j < n) && bc[j] == 0;
+  const double* src = dir ? G + (size_t)j * ld : H + (size_t)j * ld;
+  const double sg = dir ? -1.0 : 1.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
+    A[(size_t)j * ld + i] = (float)(sg * src[i]);
+}
+
 // max |v| over n entries into out[slot] (one workgroup).
 __global__ __launch_bounds__(1024) void absmax_kernel(int n, const double* __restrict__ v,
                                                       double* __restrict__ out, int slot) {
@@ -87,6 +100,8 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
   const size_t nn2 = (size_t)nt * nt;
   BEMHIP_HIP_TRY(c, c->ipiv.ensure(nt));
   BEMHIP_HIP_TRY(c, c->perm.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->ipiv32.ensure(nt));
+  BEMHIP_HIP_TRY(c, c->perm32.ensure(nt));
   BEMHIP_HIP_TRY(c, c->vec64a.ensure(nt));  // x
   BEMHIP_HIP_TRY(c, c->vec64b.ensure(std::max(nt, 4)));
   BEMHIP_HIP_TRY(c, c->part2.ensure(2 * (size_t)nt));  // r, y
@@ -125,16 +140,22 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
   } else {
     if (!c->factored32) {
       if (c->precision == BEMHIP_FP64) {
-        // FP32 copy of the FP64-assembled system
+        // FP32 copy of the FP64-assembled system (from H, G once A64 is factored)
         BEMHIP_HIP_TRY(c, c->A32.ensure(nn2));
-        hipLaunchKernelGGL(round_f32_kernel, dim3(4096), dim3(256), 0, c->stream, nn2,
-                           c->A64.ptr, c->A32.ptr);
+        if (c->factored64)
+          hipLaunchKernelGGL(swap_round_kernel,
+                             dim3((unsigned)std::min((nt + 255) / 256, 64), (unsigned)nt),
+                             dim3(256), 0, c->stream, nt, c->n, c->d_bc.ptr, c->H64.ptr,
+                             c->G64.ptr, c->A32.ptr);
+        else
+          hipLaunchKernelGGL(round_f32_kernel, dim3(4096), dim3(256), 0, c->stream, nn2,
+                             c->A64.ptr, c->A32.ptr);
       }
       hipEvent_t f0, f1;
       hipEventCreate(&f0);
       hipEventCreate(&f1);
       hipEventRecord(f0, c->stream);
-      st = lu_factor_device(c, nt, 1, c->A32.ptr, nt, c->ipiv.ptr, &info);
+      st = lu_factor_device(c, nt, 1, c->A32.ptr, nt, c->ipiv32.ptr, &info);
       hipEventRecord(f1, c->stream);
       hipEventSynchronize(f1);
       float ms = 0;
@@ -144,7 +165,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
       hipEventDestroy(f1);
       if (st) return st;
       if (info) return fail(c, BEMHIP_ERR_SINGULAR, "solve: zero pivot in the FP32 LU");
-      if ((st = build_perm(c, nt, c->ipiv.ptr, c->perm.ptr))) return st;
+      if ((st = build_perm(c, nt, c->ipiv32.ptr, c->perm32.ptr))) return st;
       c->factored32 = true;
     }
     hipEvent_t r0, r1;
@@ -153,7 +174,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     hipEventRecord(r0, c->stream);
     BEMHIP_HIP_TRY(c, hipMemcpyAsync(dx, c->b64.ptr, sizeof(double) * nt, hipMemcpyDeviceToDevice,
                                      c->stream));
-    if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm.ptr, dx))) return st;
+    if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm32.ptr, dx))) return st;
     bool converged = false;
     double dn_prev = INFINITY;
     c->stats.erase("refine_stagnated");
@@ -162,7 +183,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     for (it = 0; it < maxit; ++it) {
       if ((st = operator_apply(c, dx, dy))) return st;
       hipLaunchKernelGGL(sub_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, c->b64.ptr, dy, dr);
-      if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm.ptr, dr))) return st;
+      if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm32.ptr, dr))) return st;
       double dn = 0, xn = 0;
       if ((st = absmax2(c, nt, dr, dx, &dn, &xn))) return st;
       hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dr, dx);
@@ -211,6 +232,11 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
   if ((st = absmax2(c, nt, dr, c->b64.ptr, &rn, &bn))) return st;
   rel = bn > 0 ? rn / bn : rn;
   c->stats["relres"] = rel;
+  BEMHIP_HIP_TRY(c, c->xsol.ensure(nt));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->xsol.ptr, dx, sizeof(double) * nt, hipMemcpyDeviceToDevice,
+                                   c->stream));
+  c->has_sol = true;
+  c->sol_mode = mode;
   if (x) BEMHIP_HIP_TRY(c, hipMemcpy(x, dx, sizeof(double) * nt, hipMemcpyDeviceToHost));
   if (iters) *iters = it;
   if (relres) *relres = rel;
@@ -255,7 +281,7 @@ extern "C" int bemhip_lu_solve(bemhip_ctx* c, int n, int dtype, const void* LU, 
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "lu_solve: NULL context");
   if (n < 1 || !LU || !ipiv || !b || (dtype != 0 && dtype != 1))
     return fail(c, BEMHIP_ERR_ARG, "lu_solve: need n >= 1, dtype 0/1 and non-NULL buffers");
-  if (trans != 0) return fail(c, BEMHIP_ERR_ARG, "lu_solve: only trans = 0 is supported");
+  if (trans != 0 && trans != 1) return fail(c, BEMHIP_ERR_ARG, "lu_solve: trans must be 0 or 1");
   for (int i = 0; i < n; ++i)
     if (ipiv[i] < i || ipiv[i] >= n) return fail(c, BEMHIP_ERR_ARG, "lu_solve: invalid ipiv");
   hipSetDevice(c->device);
@@ -273,7 +299,9 @@ extern "C" int bemhip_lu_solve(bemhip_ctx* c, int n, int dtype, const void* LU, 
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(db.ptr, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   int st = build_perm(c, n, dp.ptr, dperm.ptr);
   if (st) return st;
-  if ((st = lu_solve_device(c, n, dtype, dA.ptr, n, dperm.ptr, db.ptr))) return st;
+  st = trans ? lu_solve_trans_device(c, n, dtype, dA.ptr, n, dperm.ptr, db.ptr)
+             : lu_solve_device(c, n, dtype, dA.ptr, n, dperm.ptr, db.ptr);
+  if (st) return st;
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(b, db.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   return BEMHIP_OK;

@@ -317,6 +317,8 @@ def _curve_mesh(name: str, theta_ends: np.ndarray, curve, bc_of_element,
         th = np.linspace(theta_ends[e], theta_ends[e + 1], order + 1)
         th[0], th[-1] = theta_ends[e], theta_ends[e + 1]
         lines[e] = curve(th)
+    if abs(theta_ends[-1] - theta_ends[0] - 2.0 * np.pi) < 1e-12:
+        lines[-1, -1] = lines[0, 0]   # closed curve: the last end node is the first one
     phys = [bc_of_element(e, 0.5 * (theta_ends[e] + theta_ends[e + 1])) for e in range(ne)]
     return Mesh2D(name, order, _to_gmsh_order(lines), phys, np.ones(ne, dtype=np.int64))
 
@@ -383,6 +385,40 @@ def star_mesh(ne: int, seed: int = 1, jitter: float = 0.1, arc: int = 64,
     m = _curve_mesh(f"star_{ne}_{seed}", th, curve, bc, order)
     m.extras.update(am=am, ph=ph, theta=th)
     return m
+
+
+def star_param_jacobian(mesh: Mesh2D) -> np.ndarray:
+    """d(geometric nodes)/d(a_m, phi_m) of a star mesh, shaped (ne, p+1, 2, 14):
+    columns 0..6 are a_2..a_8, 7..13 phi_2..phi_8. A node at angle t sits at
+    r(t) (cos t, sin t), so dX/da_m = cos(m t + phi_m) (cos t, sin t) and
+    dX/dphi_m = -a_m sin(m t + phi_m) (cos t, sin t)."""
+    am, ph = mesh.extras["am"], mesh.extras["ph"]
+    th = mesh.extras["theta"]
+    p = int(mesh.order)
+    ne = th.size - 1
+    t = np.zeros((ne, p + 1))
+    for e in range(ne):
+        te = np.linspace(th[e], th[e + 1], p + 1)
+        te[0], te[-1] = th[e], th[e + 1]
+        t[e] = te                                   # parametric order (xi = -1..1)
+    dirv = np.stack([np.cos(t), np.sin(t)], axis=-1)  # (ne, p+1, 2)
+    Jac = np.zeros((ne, p + 1, 2, 14))
+    for idx, m in enumerate(range(2, 9)):
+        Jac[..., idx] = (np.cos(m * t + ph[idx]))[..., None] * dirv
+        Jac[..., 7 + idx] = (-am[idx] * np.sin(m * t + ph[idx]))[..., None] * dirv
+    return Jac
+
+
+def move_geometric_node(mesh: Mesh2D, node, axis: int, delta: float) -> Mesh2D:
+    """Copy of ``mesh`` with every copy of the geometric node ``node`` (x, y)
+    moved by ``delta`` along ``axis`` (a shared end node moves in both elements)."""
+    lines = np.array(mesh.lines, dtype=np.float64, copy=True)
+    hit = np.all(lines == np.asarray(node, dtype=np.float64), axis=-1)
+    if not hit.any():
+        raise ValueError("move_geometric_node: node not found")
+    lines[hit, axis] += delta
+    return Mesh2D(mesh.name, mesh.order, lines, list(mesh.physical), np.array(mesh.region),
+                  [np.array(c) for c in mesh.cells], dict(mesh.extras))
 
 
 def n_eff(N: int, p: int = 2) -> int:

@@ -2,6 +2,7 @@
 """Benchmark of the dense 2-D Laplace BEM path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--ne NE] [--no-cpu] [--no-fp64]
+                    [--no-adjoint] [--phase all|assemble]
 
 Workload: random star r(t) = 1 + sum_{m=2..8} a_m cos(m t + phi_m) (seed 1),
 quadratic elements with +-10% jittered angular widths, N = 32768 requested ->
@@ -18,6 +19,12 @@ the accuracy check (mixed vs FP64, per field).
 value = H,G entries per second of the assembly phase (2 nt n entries per
 assembly / assembly seconds, HIP events on the library stream, averaged over
 the timed steps). ms_per_step = wall time of a whole step.
+
+After the timed steps (outside them): the adjoint shape gradient on the
+N = 4095 star (1365 elements, npg = 12, target fluxes from the FP64 solve on
+the shape with a_m shifted by U(-0.01, 0.01)) after an FP64 and after a mixed
+solve, and the CPU baseline (the FP64 restatement of the reference algorithm
+on a bounded sample, host cores).
 """
 
 from __future__ import annotations
@@ -41,6 +48,7 @@ from bem_gmsh_amd._lib import dptr, i64ptr  # noqa: E402
 PEAK_HBM = 8.0e12          # B/s
 PEAK_MFMA_F32 = 157.3e12   # FLOP/s dense
 PEAK_MFMA_F64 = 78.6e12
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_star_N32769.json")
 
 
 def log(msg):
@@ -131,6 +139,54 @@ def cpu_baseline(dad, npg, rows, lu_n, threads):
     }
 
 
+def load_traffic(nt):
+    """HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE + WRITE_SIZE,
+    scaled by a plain-copy calibration), from the committed summary of the
+    same workload; None when absent or for another size."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if int(t.get("nt", -1)) != nt:
+        return None
+    return {k: v["bytes_per_launch"] for k, v in t.get("kernels", {}).items()}
+
+
+def adjoint_leg(ne=1365, seed=1, npg=12):
+    """Adjoint shape gradient on the N = 4095 star: times (ms, HIP events) of the
+    transposed solve and of the gradient pass, after an FP64 and a mixed solve."""
+    mesh = B.star_mesh(ne, seed=seed)
+    dad = B.format2d(mesh, B.Laplace(1.0))
+    B.apply_bc_from_physical(dad, B.ana_exp_cos())
+    mp = B.star_mesh(ne, seed=seed, params=B.star_params(seed, perturb=True))
+    dp = B.format2d(mp, B.Laplace(1.0))
+    B.apply_bc_from_physical(dp, B.ana_exp_cos())
+    B.assemble(dp, npg, fetch=False)
+    B.solve(dp, mode="fp64")
+    qt = dp.q.copy()
+    out = {"workload": f"star N={dad.n} ({ne} quadratic elements), npg={npg}"}
+    grads = {}
+    for prec, mode in (("fp64", "fp64"), ("mixed", "mixed")):
+        B.assemble(dad, npg, precision=prec, fetch=False)
+        B.solve(dad, mode=mode)
+        B.adjoint(dad, qt)                       # warm-up (kernel loading)
+        t = time.perf_counter()
+        J, g = B.adjoint(dad, qt)
+        wall = time.perf_counter() - t
+        ms = dad.hip.counters()["ms"]
+        grads[mode] = g
+        out[mode] = {"J": J, "adjoint_ms": ms["adjoint"],
+                     "transposed_solve_ms": ms["adjoint_solve"],
+                     "gradient_pass_ms": ms["adjoint_grad"], "wall_s": wall,
+                     "solve_ms": ms.get("solve64" if mode == "fp64" else "solve_mixed")}
+    out["grad_mixed_vs_fp64"] = float(np.abs(grads["mixed"] - grads["fp64"]).max()
+                                      / np.abs(grads["fp64"]).max())
+    log(f"[bench] adjoint N={dad.n}: fp64 {out['fp64']['adjoint_ms']:.2f} ms, mixed "
+        f"{out['mixed']['adjoint_ms']:.2f} ms, mixed vs fp64 {out['grad_mixed_vs_fp64']:.1e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +196,7 @@ def main():
     ap.add_argument("--npg", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--phase", choices=["all", "assemble"], default="all",
                     help="'assemble': only assembly + BC passes (for counter runs)")
     args = ap.parse_args()
@@ -226,31 +283,37 @@ def main():
             f" mixed vs fp64: T {eT:.2e} q {eq:.2e}")
     ctx.close()
 
+    if not args.no_adjoint:
+        detail["adjoint"] = adjoint_leg()
+
     # roofline of the dominant kernel (FP32 MFMA trailing update of the LU)
     kb = int(counters["stats"].get("lu32_kb", 256))
     gflops, glaunch = gemm_flops(nt, kb)
     gemm_s = float(np.mean(gemm_ms)) * 1e-3
     ach = gflops / gemm_s
+    traffic = load_traffic(nt)
     roofline = {"kernel": "gemm_kernel<float> (LU trailing update)", "bound": "mfma",
                 "achieved": ach / 1e12, "peak": PEAK_MFMA_F32 / 1e12, "unit": "TFLOP/s",
-                "frac": ach / PEAK_MFMA_F32, "traffic": None, "launches": glaunch,
-                "flops_total": gflops}
+                "frac": ach / PEAK_MFMA_F32,
+                "traffic": traffic.get("gemm_kernel<float>") if traffic else None,
+                "launches": glaunch, "flops_total": gflops}
     far_bytes = entries * 4.0
     far_s = float(np.mean(far_ms)) * 1e-3
     roofline_far = {"kernel": "far_kernel<float,3> (H,G far field)", "bound": "hbm",
                     "achieved": far_bytes / far_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                    "frac": far_bytes / far_s / PEAK_HBM, "traffic": None,
+                    "frac": far_bytes / far_s / PEAK_HBM,
+                    "traffic": traffic.get("far_kernel<float, 3>") if traffic else None,
                     "algorithmic_bytes": far_bytes}
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         log(f"[bench] cpu baseline on {threads} threads ...")
-        cb = cpu_baseline(dad, args.npg, rows=min(nt, 4096), lu_n=min(nt, 8192),
+        cb = cpu_baseline(dad, args.npg, rows=min(nt, 16384), lu_n=min(nt, 10240),
                           threads=threads)
         cpu = {"value": cb["entries_per_s"], "unit": "entries/s", "cores": threads,
                "kind": "port",
-               "sample": f"oracle assembly of rows 0..{min(nt, 4096)} of this workload "
+               "sample": f"oracle assembly of rows 0..{min(nt, 16384)} of this workload "
                          f"({cb['assembly_sample_s']:.2f} s); LAPACK LU n={cb['lu_sample_n']} "
                          f"({cb['lu_sample_s']:.2f} s) -> {cb['lu_extrapolated_s']:.1f} s at "
                          f"n={nt} (n^3)",

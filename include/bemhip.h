@@ -141,11 +141,30 @@ int bemhip_apply_operator(bemhip_ctx* ctx, const double* x, double* y);
  * n x n), the LU seam of bem_linsolve (src/Core/LinearSolveUtils.jl:44-49).
  *   dtype 0: A is double*, 1: A is float*. A is overwritten by L\U (unit L)
  *   and ipiv (n, 0-based: row i was swapped with row ipiv[i]).
- * bemhip_lu_solve solves with those factors in place of b (nrhs = 1); trans
- * must be 0. */
+ * bemhip_lu_solve solves A x = b (trans = 0) or A^T x = b (trans = 1) with
+ * those factors, in place of b (nrhs = 1). */
 int bemhip_lu_factor(bemhip_ctx* ctx, int n, int dtype, void* A, int* ipiv);
 int bemhip_lu_solve(bemhip_ctx* ctx, int n, int dtype, const void* LU, const int* ipiv,
                     double* b, int trans);
+
+/* Adjoint shape gradient of a boundary-flux misfit. No counterpart in the
+ * reference: it differentiates assemble -> applyBC -> solve
+ * (src/Core/Assembly_full.jl:61-85, src/Core/Boundary_conditions.jl:20-75,
+ * src/Core/Solver.jl:24-64) for the system of the last bemhip_solve:
+ *   J = 1/2 sum_{j<n} w_j (q_j - q_target_j)^2
+ *   q_target (n) target fluxes
+ *   w        (n) weights, or NULL for the boundary quadrature weights
+ *            J_j * elem_w[k] (then differentiated as well)
+ *   dJ_dgeo  (out, ne x (p+1) x 2) dJ/d geo_xy, per element copy of each
+ *            geometric node (an end node shared by two elements gets the sum
+ *            of both entries)
+ *   J        (out, may be NULL) the objective
+ * BC values and internal points are held fixed. One transposed solve with the
+ * factors of the last solve (FP64, or FP32 with FP64 refinement), then one
+ * pass over the element kernels recomputed with their derivatives (near and
+ * on-element rules in forward-mode dual numbers). */
+int bemhip_adjoint(bemhip_ctx* ctx, const double* q_target, const double* w, double* dJ_dgeo,
+                   double* J);
 
 /* Timings (ms, HIP events on the context stream) and sizes of the last calls
  * as a JSON object. */
