@@ -52,6 +52,7 @@ Geo make_geo_adj(const bemhip_ctx* c) {
   g.p = c->p; g.nn = c->nn; g.ne = c->ne; g.n = c->n; g.ni = c->ni; g.nt = c->nt;
   g.x = c->x.ptr; g.y = c->y.ptr; g.nx = c->nx.ptr; g.ny = c->ny.ptr; g.jw = c->jw.ptr;
   g.elen = c->elen.ptr; g.gx = c->gx.ptr; g.gy = c->gy.ptr;
+  g.ld = (c->precision == BEMHIP_FP64) ? c->ld64 : c->ld32;
   return g;
 }
 
@@ -408,7 +409,7 @@ static int solve_transposed(bemhip_ctx* c, double* d_g, double tol, int maxit, i
   const int nt = c->nt;
   if (c->sol_mode == BEMHIP_SOLVE_FP64) {
     if (iters) *iters = 0;
-    return lu_solve_trans_device(c, nt, 0, c->A64.ptr, nt, c->perm.ptr, d_g);
+    return lu_solve_trans_device(c, nt, 0, c->A64.ptr, (int)c->ld64, c->perm.ptr, d_g);
   }
   BEMHIP_HIP_TRY(c, c->vec64b.ensure(3 * (size_t)nt));
   double* rhs = c->vec64b.ptr;
@@ -416,7 +417,7 @@ static int solve_transposed(bemhip_ctx* c, double* d_g, double tol, int maxit, i
   double* r = y + nt;
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(rhs, d_g, sizeof(double) * nt, hipMemcpyDeviceToDevice,
                                    c->stream));
-  int st = lu_solve_trans_device(c, nt, 1, c->A32.ptr, nt, c->perm32.ptr, d_g);
+  int st = lu_solve_trans_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, d_g);
   if (st) return st;
   const int tb = 256, nb = (nt + tb - 1) / tb;
   std::vector<double> hr(nt), hx(nt);
@@ -426,7 +427,8 @@ static int solve_transposed(bemhip_ctx* c, double* d_g, double tol, int maxit, i
   for (it = 0; it < maxit; ++it) {
     if ((st = operator_apply_trans(c, d_g, y))) return st;
     hipLaunchKernelGGL(resid_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, rhs, y, r);
-    if ((st = lu_solve_trans_device(c, nt, 1, c->A32.ptr, nt, c->perm32.ptr, r))) return st;
+    if ((st = lu_solve_trans_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, r)))
+      return st;
     BEMHIP_HIP_TRY(c, hipMemcpyAsync(hr.data(), r, sizeof(double) * nt, hipMemcpyDeviceToHost,
                                      c->stream));
     BEMHIP_HIP_TRY(c, hipMemcpyAsync(hx.data(), d_g, sizeof(double) * nt, hipMemcpyDeviceToHost,

@@ -116,15 +116,19 @@ static void build_quadconst(bemhip_ctx* c, QuadConst* q) {
   q->near_factor = c->near_factor;
 }
 
+// Device matrix (rows x cols, leading dimension ld) -> dense host column-major
+// (leading dimension rows), converting the element type when TD != TS.
 template <typename TD, typename TS>
-int copy_out(bemhip_ctx* c, TD* dst, const TS* src, size_t count) {
+int copy_out(bemhip_ctx* c, TD* dst, const TS* src, size_t rows, size_t cols, size_t ld) {
   if (std::is_same<TD, TS>::value) {
-    BEMHIP_HIP_TRY(c, hipMemcpy(dst, src, sizeof(TS) * count, hipMemcpyDeviceToHost));
+    BEMHIP_HIP_TRY(c, hipMemcpy2D(dst, rows * sizeof(TS), src, ld * sizeof(TS), rows * sizeof(TS),
+                                  cols, hipMemcpyDeviceToHost));
     return BEMHIP_OK;
   }
-  std::vector<TS> tmp(count);
-  BEMHIP_HIP_TRY(c, hipMemcpy(tmp.data(), src, sizeof(TS) * count, hipMemcpyDeviceToHost));
-  for (size_t t = 0; t < count; ++t) dst[t] = (TD)tmp[t];
+  std::vector<TS> tmp(rows * cols);
+  BEMHIP_HIP_TRY(c, hipMemcpy2D(tmp.data(), rows * sizeof(TS), src, ld * sizeof(TS),
+                                rows * sizeof(TS), cols, hipMemcpyDeviceToHost));
+  for (size_t t = 0; t < rows * cols; ++t) dst[t] = (TD)tmp[t];
   return BEMHIP_OK;
 }
 
@@ -134,14 +138,14 @@ int get_HG(bemhip_ctx* c, TD* H, TD* G) {
   if (!c->assembled) return fail(c, BEMHIP_ERR_STATE, "get_HG: assemble first");
   hipSetDevice(c->device);
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  const size_t nt = c->nt;
+  const size_t nt = c->nt, n = c->n;
   int st = 0;
   if (c->precision == BEMHIP_FP64) {
-    if (H && (st = copy_out(c, H, c->H64.ptr, nt * nt))) return st;
-    if (G && (st = copy_out(c, G, c->G64.ptr, nt * (size_t)c->n))) return st;
+    if (H && (st = copy_out(c, H, c->H64.ptr, nt, nt, c->ld64))) return st;
+    if (G && (st = copy_out(c, G, c->G64.ptr, nt, n, c->ld64))) return st;
   } else {
-    if (H && (st = copy_out(c, H, c->H32.ptr, nt * nt))) return st;
-    if (G && (st = copy_out(c, G, c->G32.ptr, nt * (size_t)c->n))) return st;
+    if (H && (st = copy_out(c, H, c->H32.ptr, nt, nt, c->ld32))) return st;
+    if (G && (st = copy_out(c, G, c->G32.ptr, nt, n, c->ld32))) return st;
   }
   return BEMHIP_OK;
 }
@@ -244,6 +248,8 @@ int bemhip_set_geometry(bemhip_ctx* c, int p, int ne, int n, int ni, const doubl
   c->factored32 = c->factored64 = false;
   c->has_sol = false;
   c->p = p; c->nn = nn; c->ne = ne; c->n = n; c->ni = ni; c->nt = nt; c->k = k;
+  c->ld32 = padded_ld(nt, 64);
+  c->ld64 = padded_ld(nt, 32);
   c->h_elem_w.assign(elem_w, elem_w + nn);
   std::vector<double> hx(nt), hy(nt), hnx(n), hny(n), hjw(n), hgx((size_t)ne * nn),
       hgy((size_t)ne * nn);
@@ -368,9 +374,9 @@ int bemhip_apply_bc(bemhip_ctx* c, const int64_t* BC, const double* BV, double* 
   if (b) BEMHIP_HIP_TRY(c, hipMemcpy(b, c->b64.ptr, sizeof(double) * nt, hipMemcpyDeviceToHost));
   if (A) {
     if (c->precision == BEMHIP_FP64) {
-      if ((st = copy_out(c, A, c->A64.ptr, nt * nt))) return st;
+      if ((st = copy_out(c, A, c->A64.ptr, nt, nt, c->ld64))) return st;
     } else {
-      if ((st = copy_out(c, A, c->A32.ptr, nt * nt))) return st;
+      if ((st = copy_out(c, A, c->A32.ptr, nt, nt, c->ld32))) return st;
     }
   }
   return BEMHIP_OK;

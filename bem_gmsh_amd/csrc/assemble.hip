@@ -78,10 +78,41 @@ struct Consts {
   double nf;  // near factor
 };
 
+// Conservative FP32 pre-test: true only when the pair is far by a relative
+// margin (lim2m = lim^2 (1 + 1e-3)) on both the node-distance and the
+// bounding-box tests, so that the exact FP64 test (is_near) would also call
+// it far. r*f are the FP32 roundings of the FP64 node - source offsets.
+template <int NN>
+__device__ __forceinline__ bool clearly_far(const float* rxf, const float* ryf, const float* d2f,
+                                            float lim2m) {
+  float dn2 = d2f[0], lox = rxf[0], hix = rxf[0], loy = ryf[0], hiy = ryf[0];
+#pragma unroll
+  for (int k = 1; k < NN; ++k) {
+    dn2 = fminf(dn2, d2f[k]);
+    lox = fminf(lox, rxf[k]); hix = fmaxf(hix, rxf[k]);
+    loy = fminf(loy, ryf[k]); hiy = fmaxf(hiy, ryf[k]);
+  }
+  const float dx = lox > 0.f ? lox : (hix < 0.f ? -hix : 0.f);
+  const float dy = loy > 0.f ? loy : (hiy < 0.f ? -hiy : 0.f);
+  return dn2 > lim2m && (dx * dx + dy * dy) > lim2m;
+}
+
+// 1/x in FP64 from the FP32 reciprocal and two Newton steps (about 1 ulp).
+__device__ __forceinline__ double rcp_f64(double x) {
+  double r = (double)__builtin_amdgcn_rcpf((float)x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
 // ---------------------------------------------------------------------------
 // Far-field tile kernel.
 // Block = 256 rows x 32 elements. Element data staged in LDS; each lane owns
 // one row i, so every store H[i + j*nt] of a wave is 64 consecutive entries.
+// Classification: the FP32 pre-test settles the clearly-far pairs; the rest
+// (near pairs and those within 0.1% of the cutoff) take the exact FP64 test.
+// FP64 storage: the entries with the FP64 operation order of the reference.
+// Mixed storage: FP32 entries (FP32 reciprocal and logarithm) and the FP64
+// row sum of the far H entries (Newton reciprocal) for the diagonal.
 // ---------------------------------------------------------------------------
 template <typename TS, int NN>
 __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* __restrict__ H,
@@ -90,6 +121,7 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
                                                           double* __restrict__ hpart) {
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
       s_jw[kEB * NN], s_L[kEB];
+  __shared__ float s_nxf[kEB * NN], s_nyf[kEB * NN], s_jwf[kEB * NN], s_lim2f[kEB];
   const int cb = blockIdx.y;
   const int e0 = cb * kEB;
   const int ecount = min(kEB, g.ne - e0);
@@ -98,28 +130,45 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
     s_cx[t] = g.x[j]; s_cy[t] = g.y[j];
     s_nx[t] = g.nx[j]; s_ny[t] = g.ny[j];
     s_jw[t] = g.jw[j];
+    s_nxf[t] = (float)g.nx[j]; s_nyf[t] = (float)g.ny[j];
+    s_jwf[t] = (float)g.jw[j];
   }
-  for (int t = threadIdx.x; t < ecount; t += blockDim.x) s_L[t] = g.elen[e0 + t];
+  for (int t = threadIdx.x; t < ecount; t += blockDim.x) {
+    const double L = g.elen[e0 + t];
+    s_L[t] = L;
+    const double lim = cs.nf * L;
+    // +inf (no far lumping) when the cutoff is disabled or overflows
+    s_lim2f[t] = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
+  }
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;
   const double px = g.x[i], py = g.y[i];
-  const size_t ld = (size_t)g.nt;
+  const size_t ld = g.ld;
   double rowsum = 0.0;
   uint32_t bits = 0u;
   for (int el = 0; el < ecount; ++el) {
     const double* cx = s_cx + el * NN;
     const double* cy = s_cy + el * NN;
-    double rx[NN], ry[NN], d2[NN];
+    double rx[NN], ry[NN];
+    float rxf[NN], ryf[NN], d2f[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       rx[k] = cx[k] - px;
       ry[k] = cy[k] - py;
-      d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
+      rxf[k] = (float)rx[k];
+      ryf[k] = (float)ry[k];
+      d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
     }
     const int col0 = (e0 + el) * NN;
     const bool onel = (i >= col0) && (i < col0 + NN);
-    const bool nr = onel || is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf);
+    bool nr = onel;
+    if (!onel && !clearly_far<NN>(rxf, ryf, d2f, s_lim2f[el])) {
+      double d2[NN];
+#pragma unroll
+      for (int k = 0; k < NN; ++k) d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
+      nr = is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf);
+    }
     TS* Hc = H + (size_t)col0 * ld + i;
     TS* Gc = G + (size_t)col0 * ld + i;
     if (nr) {
@@ -132,9 +181,10 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
     } else if constexpr (sizeof(TS) == 8) {
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
-        const double U = -log(d2[k]) * cs.cU;
+        const double d2 = rx[k] * rx[k] + ry[k] * ry[k];
+        const double U = -log(d2) * cs.cU;
         const double T = (rx[k] * s_nx[el * NN + k] + ry[k] * s_ny[el * NN + k]) *
-                         (cs.cT / d2[k]);
+                         (cs.cT / d2);
         const double w = s_jw[el * NN + k];
         const double h = T * w;
         Hc[k * ld] = h;
@@ -142,27 +192,23 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
         rowsum += h;
       }
     } else {
-      // Mixed precision: the regular far quadrature in FP32; the FP64 row sum
-      // of H (rigid-body diagonal) uses the FP64 double-layer value, which
-      // needs no logarithm, with the operations of the FP64 kernel.
+      const float cUf = (float)cs.cU, cTf = (float)cs.cT;
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
-        const double T64 = (rx[k] * s_nx[el * NN + k] + ry[k] * s_ny[el * NN + k]) *
-                           (cs.cT / d2[k]);
-        rowsum += T64 * s_jw[el * NN + k];
-        const float fx = (float)rx[k], fy = (float)ry[k];
-        const float r2 = fx * fx + fy * fy;
-        const float U = -logf(r2) * (float)cs.cU;
-        const float T = (fx * (float)s_nx[el * NN + k] + fy * (float)s_ny[el * NN + k]) *
-                        ((float)cs.cT / r2);
-        const float w = (float)s_jw[el * NN + k];
-        Hc[k * ld] = T * w;
-        Gc[k * ld] = U * w;
+        const int t = el * NN + k;
+        const double d2 = fma(rx[k], rx[k], ry[k] * ry[k]);
+        const double T64 = fma(rx[k], s_nx[t], ry[k] * s_ny[t]) * (cs.cT * rcp_f64(d2));
+        rowsum = fma(T64, s_jw[t], rowsum);
+        const float rinv = __builtin_amdgcn_rcpf(d2f[k]);
+        const float U = -__logf(d2f[k]) * cUf;
+        const float T = (rxf[k] * s_nxf[t] + ryf[k] * s_nyf[t]) * (cTf * rinv);
+        Hc[k * ld] = T * s_jwf[t];
+        Gc[k * ld] = U * s_jwf[t];
       }
     }
   }
-  nearbits[(size_t)cb * ld + i] = bits;
-  if (hpart) hpart[(size_t)cb * ld + i] = rowsum;
+  nearbits[(size_t)cb * g.nt + i] = bits;
+  if (hpart) hpart[(size_t)cb * g.nt + i] = rowsum;
 }
 
 // ---------------------------------------------------------------------------
@@ -178,6 +224,7 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
       s_jw[kEB * NN], s_L[kEB];
   __shared__ double s_w[4][kEB * NN];
+  __shared__ float s_lim2f[kEB];
   const int cb = blockIdx.y;
   const int e0 = cb * kEB;
   const int ecount = min(kEB, g.ne - e0);
@@ -190,7 +237,12 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
     s_w[1][t] = wG1[j];
     if (nvec == 2) { s_w[2][t] = wH2[j]; s_w[3][t] = wG2[j]; }
   }
-  for (int t = threadIdx.x; t < ecount; t += blockDim.x) s_L[t] = g.elen[e0 + t];
+  for (int t = threadIdx.x; t < ecount; t += blockDim.x) {
+    const double L = g.elen[e0 + t];
+    s_L[t] = L;
+    const double lim = cs.nf * L;
+    s_lim2f[t] = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
+  }
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;
@@ -200,15 +252,22 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
     const double* cx = s_cx + el * NN;
     const double* cy = s_cy + el * NN;
     double rx[NN], ry[NN], d2[NN];
+    float rxf[NN], ryf[NN], d2f[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       rx[k] = cx[k] - px;
       ry[k] = cy[k] - py;
       d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
+      rxf[k] = (float)rx[k];
+      ryf[k] = (float)ry[k];
+      d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
     }
     const int col0 = (e0 + el) * NN;
     const bool onel = (i >= col0) && (i < col0 + NN);
-    if (onel || is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf)) continue;
+    if (onel) continue;
+    if (!clearly_far<NN>(rxf, ryf, d2f, s_lim2f[el]) &&
+        is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf))
+      continue;
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       const int t = el * NN + k;
@@ -231,7 +290,7 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
 //   mode 1 (apply):     y_i = far + near (self column: G_ii wG_i + diag_i wH_i)
 //                       + diag_i wH_i for internal rows.
 template <typename TS>
-__global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode,
+__global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode, size_t ldh,
                               const double* __restrict__ part, const int* __restrict__ row_ptr,
                               const int* __restrict__ pair_elem,
                               const double* __restrict__ pair_h,
@@ -251,7 +310,7 @@ __global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode,
     }
     const double d = -s;
     diag[i] = d;
-    if (Hdiag) Hdiag[(size_t)i * nt + i] = TS(d);
+    if (Hdiag) Hdiag[(size_t)i * ldh + i] = TS(d);
   } else {
     for (int pr = p0; pr < p1; ++pr) {
       const int c0 = pair_elem[pr] * nn;
@@ -855,18 +914,17 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     ph[(size_t)pair * NN + lane] = hv;
     pg[(size_t)pair * NN + lane] = gv;
     const size_t col = (size_t)e * NN + lane;
-    H[i + col * (size_t)g.nt] = TS(hv);
-    G[i + col * (size_t)g.nt] = TS(gv);
+    H[i + col * g.ld] = TS(hv);
+    G[i + col * g.ld] = TS(gv);
   }
 }
 
 // Column swap A = H with Dirichlet columns -G (src/Core/Boundary_conditions.jl:54-64).
 template <typename TS>
-__global__ void bc_swap_kernel(int nt, int n, const int* __restrict__ bc,
+__global__ void bc_swap_kernel(int nt, int n, size_t ld, const int* __restrict__ bc,
                                const TS* __restrict__ H, const TS* __restrict__ G,
                                TS* __restrict__ A) {
   const int j = blockIdx.y;
-  const size_t ld = (size_t)nt;
   const bool dir = (j < n) && bc[j] == 0;
   const TS* src = dir ? G + (size_t)j * ld : H + (size_t)j * ld;
   TS* dst = A + (size_t)j * ld;
@@ -902,6 +960,7 @@ Geo make_geo(bemhip_ctx* c) {
   g.p = c->p; g.nn = c->nn; g.ne = c->ne; g.n = c->n; g.ni = c->ni; g.nt = c->nt;
   g.x = c->x.ptr; g.y = c->y.ptr; g.nx = c->nx.ptr; g.ny = c->ny.ptr; g.jw = c->jw.ptr;
   g.elen = c->elen.ptr; g.gx = c->gx.ptr; g.gy = c->gy.ptr;
+  g.ld = (c->precision == BEMHIP_FP64) ? c->ld64 : c->ld32;
   return g;
 }
 
@@ -928,18 +987,18 @@ int assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, c->row_ptr.ensure(nt + 1));
   if (fp64) {
     c->H32.release(); c->G32.release();
-    BEMHIP_HIP_TRY(c, c->H64.ensure(nt * nt));
-    BEMHIP_HIP_TRY(c, c->G64.ensure(nt * (size_t)c->n));
-    if (c->ni > 0)
-      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + (size_t)c->n * nt, 0,
-                                       sizeof(double) * nt * c->ni, c->stream));
+    BEMHIP_HIP_TRY(c, c->H64.ensure(c->ld64 * nt));
+    BEMHIP_HIP_TRY(c, c->G64.ensure(c->ld64 * (size_t)c->n));
+    if (c->ni > 0)  // columns of the internal nodes: zero but the diagonal
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + (size_t)c->n * c->ld64, 0,
+                                       sizeof(double) * c->ld64 * c->ni, c->stream));
   } else {
     c->H64.release(); c->G64.release();
-    BEMHIP_HIP_TRY(c, c->H32.ensure(nt * nt));
-    BEMHIP_HIP_TRY(c, c->G32.ensure(nt * (size_t)c->n));
+    BEMHIP_HIP_TRY(c, c->H32.ensure(c->ld32 * nt));
+    BEMHIP_HIP_TRY(c, c->G32.ensure(c->ld32 * (size_t)c->n));
     if (c->ni > 0)
-      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + (size_t)c->n * nt, 0,
-                                       sizeof(float) * nt * c->ni, c->stream));
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + (size_t)c->n * c->ld32, 0,
+                                       sizeof(float) * c->ld32 * c->ni, c->stream));
   }
   int st;
   {
@@ -990,14 +1049,14 @@ int assemble_impl(bemhip_ctx* c) {
     const int nb = (int)((nt + tb - 1) / tb);
     if (fp64) {
       hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt,
-                         c->n, c->nn, ncb, 0, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
-                         c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr,
-                         nullptr, c->H64.ptr);
+                         c->n, c->nn, ncb, 0, c->ld64, c->hpart.ptr, c->row_ptr.ptr,
+                         c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr,
+                         c->diag64.ptr, nullptr, c->H64.ptr);
     } else {
       hipLaunchKernelGGL((op64_finalize<float>), dim3(nb), dim3(tb), 0, c->stream, (int)nt,
-                         c->n, c->nn, ncb, 0, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
-                         c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr,
-                         nullptr, c->H32.ptr);
+                         c->n, c->nn, ncb, 0, c->ld32, c->hpart.ptr, c->row_ptr.ptr,
+                         c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr,
+                         c->diag64.ptr, nullptr, c->H32.ptr);
     }
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
@@ -1044,7 +1103,7 @@ static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, doub
   const int tb = 256;
   const int nb = (int)((nt + tb - 1) / tb);
   hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
-                     c->nn, ncb, 1, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                     c->nn, ncb, 1, (size_t)0, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                      c->pair_h.ptr, c->pair_g.ptr, wH, wG, c->diag64.ptr, d_y,
                      (double*)nullptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
@@ -1084,14 +1143,14 @@ int apply_bc_impl(bemhip_ctx* c) {
     dim3 grid((unsigned)std::min<size_t>((nt + 255) / 256, 64), (unsigned)nt);
     if (c->precision == BEMHIP_FP64) {
       c->A32.release();
-      BEMHIP_HIP_TRY(c, c->A64.ensure(nt * nt));
+      BEMHIP_HIP_TRY(c, c->A64.ensure(c->ld64 * nt));
       hipLaunchKernelGGL((bc_swap_kernel<double>), grid, dim3(256), 0, c->stream, c->nt, c->n,
-                         c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A64.ptr);
+                         c->ld64, c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A64.ptr);
     } else {
       c->A64.release();
-      BEMHIP_HIP_TRY(c, c->A32.ensure(nt * nt));
+      BEMHIP_HIP_TRY(c, c->A32.ensure(c->ld32 * nt));
       hipLaunchKernelGGL((bc_swap_kernel<float>), grid, dim3(256), 0, c->stream, c->nt, c->n,
-                         c->d_bc.ptr, c->H32.ptr, c->G32.ptr, c->A32.ptr);
+                         c->ld32, c->d_bc.ptr, c->H32.ptr, c->G32.ptr, c->A32.ptr);
     }
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }

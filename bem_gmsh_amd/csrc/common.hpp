@@ -17,6 +17,7 @@ namespace bemhip {
 // Device geometry (structure of arrays).
 struct Geo {
   int p, nn, ne, n, ni, nt;
+  size_t ld;           // leading dimension of the stored H, G (padded, see padded_ld)
   const double* x;     // nt collocation x
   const double* y;     // nt collocation y
   const double* nx;    // n normals
@@ -26,6 +27,17 @@ struct Geo {
   const double* gx;    // ne*nn geometric nodes (equispaced order)
   const double* gy;
 };
+
+// Leading dimension of the device-resident matrices: the smallest multiple of
+// `unit` elements >= nt whose column stride is an odd multiple of 256 bytes
+// (unit = 64 for FP32, 32 for FP64). Column strides near a power of two
+// (ld = nt = 32769 -> 2^17 + 4 bytes) put the columns of a tile on the same
+// HBM channels; the odd multiple spreads them.
+inline size_t padded_ld(int nt, int unit) {
+  size_t m = ((size_t)nt + unit - 1) / unit;
+  if ((m & 1) == 0) ++m;
+  return m * unit;
+}
 
 template <typename T>
 struct DevBuf {
@@ -59,6 +71,7 @@ struct bemhip_ctx {
   // geometry
   bool has_geo = false;
   int p = 0, nn = 0, ne = 0, n = 0, ni = 0, nt = 0;
+  size_t ld32 = 0, ld64 = 0;  // leading dimensions of the FP32 / FP64 matrices
   double k = 1.0;
   std::vector<double> h_elem_w;
   bemhip::DevBuf<double> x, y, nx, ny, jw, elen, gx, gy;

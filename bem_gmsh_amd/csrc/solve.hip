@@ -32,23 +32,24 @@ __global__ void axpy_kernel(int n, const double* __restrict__ d, double* __restr
   if (i < n) x[i] += d[i];
 }
 
-__global__ void round_f32_kernel(size_t count, const double* __restrict__ s, float* __restrict__ d) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
-       t += (size_t)gridDim.x * blockDim.x)
-    d[t] = (float)s[t];
-}
-
 // A32 = FP32 of the column swap of H64, G64 (used once A64 holds FP64 factors).
-__global__ void swap_round_kernel(int nt, int n, const int* __restrict__ bc,
+__global__ void swap_round_kernel(int nt, int n, size_t lds, size_t ldd, const int* __restrict__ bc,
                                   const double* __restrict__ H, const double* __restrict__ G,
                                   float* __restrict__ A) {
   const int j = blockIdx.y;
-  const size_t ld = (size_t)nt;
   const bool dir = (j < n) && bc[j] == 0;
-  const double* src = dir ? G + (size_t)j * ld : H + (size_t)j * ld;
+  const double* src = dir ? G + (size_t)j * lds : H + (size_t)j * lds;
   const double sg = dir ? -1.0 : 1.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
-    A[(size_t)j * ld + i] = (float)(sg * src[i]);
+    A[(size_t)j * ldd + i] = (float)(sg * src[i]);
+}
+
+// A32 = FP32 of A64 (nt x nt; leading dimensions lds, ldd).
+__global__ void round_cols_kernel(int nt, size_t lds, size_t ldd, const double* __restrict__ s,
+                                  float* __restrict__ d) {
+  const int j = blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
+    d[(size_t)j * ldd + i] = (float)s[(size_t)j * lds + i];
 }
 
 // max |v| over n entries into out[slot] (one workgroup).
@@ -97,7 +98,6 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     return fail(c, BEMHIP_ERR_ARG, "solve: tol must be > 0 and maxit >= 0");
   hipSetDevice(c->device);
   const int nt = c->nt;
-  const size_t nn2 = (size_t)nt * nt;
   BEMHIP_HIP_TRY(c, c->ipiv.ensure(nt));
   BEMHIP_HIP_TRY(c, c->perm.ensure(nt));
   BEMHIP_HIP_TRY(c, c->ipiv32.ensure(nt));
@@ -121,7 +121,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
       hipEventCreate(&f0);
       hipEventCreate(&f1);
       hipEventRecord(f0, c->stream);
-      st = lu_factor_device(c, nt, 0, c->A64.ptr, nt, c->ipiv.ptr, &info);
+      st = lu_factor_device(c, nt, 0, c->A64.ptr, (int)c->ld64, c->ipiv.ptr, &info);
       hipEventRecord(f1, c->stream);
       hipEventSynchronize(f1);
       float ms = 0;
@@ -136,26 +136,25 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     }
     BEMHIP_HIP_TRY(c, hipMemcpyAsync(dx, c->b64.ptr, sizeof(double) * nt, hipMemcpyDeviceToDevice,
                                      c->stream));
-    if ((st = lu_solve_device(c, nt, 0, c->A64.ptr, nt, c->perm.ptr, dx))) return st;
+    if ((st = lu_solve_device(c, nt, 0, c->A64.ptr, (int)c->ld64, c->perm.ptr, dx))) return st;
   } else {
     if (!c->factored32) {
       if (c->precision == BEMHIP_FP64) {
         // FP32 copy of the FP64-assembled system (from H, G once A64 is factored)
-        BEMHIP_HIP_TRY(c, c->A32.ensure(nn2));
+        BEMHIP_HIP_TRY(c, c->A32.ensure(c->ld32 * nt));
+        const dim3 grid((unsigned)std::min((nt + 255) / 256, 64), (unsigned)nt);
         if (c->factored64)
-          hipLaunchKernelGGL(swap_round_kernel,
-                             dim3((unsigned)std::min((nt + 255) / 256, 64), (unsigned)nt),
-                             dim3(256), 0, c->stream, nt, c->n, c->d_bc.ptr, c->H64.ptr,
-                             c->G64.ptr, c->A32.ptr);
+          hipLaunchKernelGGL(swap_round_kernel, grid, dim3(256), 0, c->stream, nt, c->n, c->ld64,
+                             c->ld32, c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A32.ptr);
         else
-          hipLaunchKernelGGL(round_f32_kernel, dim3(4096), dim3(256), 0, c->stream, nn2,
-                             c->A64.ptr, c->A32.ptr);
+          hipLaunchKernelGGL(round_cols_kernel, grid, dim3(256), 0, c->stream, nt, c->ld64,
+                             c->ld32, c->A64.ptr, c->A32.ptr);
       }
       hipEvent_t f0, f1;
       hipEventCreate(&f0);
       hipEventCreate(&f1);
       hipEventRecord(f0, c->stream);
-      st = lu_factor_device(c, nt, 1, c->A32.ptr, nt, c->ipiv32.ptr, &info);
+      st = lu_factor_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->ipiv32.ptr, &info);
       hipEventRecord(f1, c->stream);
       hipEventSynchronize(f1);
       float ms = 0;
@@ -174,7 +173,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     hipEventRecord(r0, c->stream);
     BEMHIP_HIP_TRY(c, hipMemcpyAsync(dx, c->b64.ptr, sizeof(double) * nt, hipMemcpyDeviceToDevice,
                                      c->stream));
-    if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm32.ptr, dx))) return st;
+    if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, dx))) return st;
     bool converged = false;
     double dn_prev = INFINITY;
     c->stats.erase("refine_stagnated");
@@ -183,7 +182,8 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     for (it = 0; it < maxit; ++it) {
       if ((st = operator_apply(c, dx, dy))) return st;
       hipLaunchKernelGGL(sub_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, c->b64.ptr, dy, dr);
-      if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, nt, c->perm32.ptr, dr))) return st;
+      if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, dr)))
+        return st;
       double dn = 0, xn = 0;
       if ((st = absmax2(c, nt, dr, dx, &dn, &xn))) return st;
       hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dr, dx);
