@@ -116,6 +116,7 @@ struct bemhip_ctx {
 
   // scratch for LU and solves
   bemhip::DevBuf<char> lu_work;
+  unsigned panel_epoch = 0;           // tag base of the LU panel exchange words
   bemhip::DevBuf<char> dinv;          // inverses of the diagonal blocks of L and U
   const void* dinv_of = nullptr;      // factor the inverses belong to
   int dinv_n = 0;

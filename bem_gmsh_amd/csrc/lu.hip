@@ -37,198 +37,357 @@ namespace {
 // ---------------------------------------------------------------------------
 // Panel factorization (GEPP) with the panel held in registers.
 //
-// Workgroup = NW waves x 64 lanes. Lane l of every wave owns panel row
-// blockIdx.x*64 + l; wave w owns the CG panel columns [w*CG, (w+1)*CG), so a
-// thread holds CG values of one row. Panel width KB = NW*CG (4 x 32 = 128).
-// Per column c: the owning wave finds the local candidate (max |a|, lowest
-// row on ties), stages it, one grid barrier, every wave picks the global
-// winner, the pivot row's values (uniform across lanes) drive the interchange
-// and the rank-1 update of the rows below.
+// Workgroup = 8 waves x 64 lanes = kPR = 512 panel rows; thread (wave w, lane
+// l) holds the whole panel row blockIdx.x*512 + 64w + l (W values: W = 128 for
+// FP32, 64 for FP64, 128 VGPRs either way), rotated so that register k holds
+// column (c + k) mod W at column step c: the current column is always
+// register 0 and no register is ever indexed at run time (a run-time index
+// would send the row to scratch memory). LDS copies of rows use the same
+// rotated order; the exchanged records use natural column order. Per column c:
+//   1. each wave finds its candidate (max |a| over its rows >= c, lowest row
+//      on ties) and stages the candidate row in LDS; the row at position c is
+//      staged by its owner;
+//   2. wave 0 picks the workgroup candidate and publishes it: a 64-bit key
+//      word (|a| key, tag, row) and the candidate row as tagged data words,
+//      plus (workgroup 0) row c;
+//   3. wave 0 polls the key words of all workgroups until every one carries
+//      this column's tag, picks the winner (max |a|, lowest row), and reads the
+//      winner's row (and, in the workgroup that owns the pivot row, row c);
+//   4. every wave interchanges, scales and applies the rank-1 update.
+// Exchange protocol: every cross-workgroup word is written and read with
+// relaxed agent-scope atomics (coherent across XCDs without cache-wide
+// write-back or invalidation) and validates itself by its tag, so no ordering
+// between words is needed. Key words and records are double-buffered by column
+// parity: a workgroup can only overwrite a buffer after every other workgroup
+// has published the next column, i.e. has finished reading this one. Key tags
+// are c + 1 (the key buffer is cleared before each launch); data-word tags add
+// a per-launch epoch. Every wait is bounded: past kSpinLimit polls the waiting
+// workgroup raises the abort word, every workgroup leaves at its next wait,
+// and the host reports an error.
+//
+// FP32 keys are the exact |a| bits. FP64 keys are |a| rounded toward zero to
+// FP32 (monotone); workgroups tied on the maximum key are resolved on the exact
+// FP64 values read from their records.
 // ---------------------------------------------------------------------------
-constexpr int kPB = 256;   // largest panel width handled by the interchange lists
-constexpr int kPR = 64;    // rows per workgroup
+constexpr int kPW = 8;          // waves per panel workgroup
+constexpr int kPR = kPW * 64;   // rows per panel workgroup
+constexpr unsigned kSpinLimit = 1u << 20;
 
 template <typename T>
 struct PanelArgs {
   T* A;
   int lda;
   int m;          // panel rows (global rows k0 .. k0+m-1)
-  int kb;         // panel width (<= KB of the kernel)
+  int kb;         // panel width (<= W of the kernel)
   int k0;         // global index of the panel's first row/column
   int G;          // workgroups
+  int gstride;    // key / record buffer stride in workgroups (>= G)
+  unsigned epoch; // per-launch tag base of the data words
   int* ipiv;      // ipiv[k0 + c] = global pivot row of column k0 + c
-  double* st_val; // [2][G] |candidate|
-  int* st_row;    // [2][G] candidate row (panel-relative), INT_MAX if none
-  T* st_data;     // [2][G][KB] candidate row values
-  T* st_rowc;     // [2][KB] values of row c before the interchange
   int* info;      // first zero pivot: global column + 1 (0 if none)
-  unsigned* bar;  // grid barrier counter (zero at launch)
+  unsigned long long* keys;   // [2][gstride] key words
+  unsigned long long* recs;   // [2][gstride][RW] candidate rows (tagged words)
+  unsigned long long* rowc;   // [2][RW] row c before the interchange (tagged words)
+  unsigned* abort_flag;       // raised by a workgroup whose wait ran out
 };
 
-__device__ __forceinline__ bool better(double ov, int orr, double v, int r) {
+__device__ __forceinline__ void st_ag(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_ag(const unsigned long long* p) {
+  return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_ag32(const unsigned* p) {
+  return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Tagged data words: FP32 one word per value, FP64 two (low, high halves).
+template <typename T>
+struct TagWords;
+template <>
+struct TagWords<float> {
+  static constexpr int kWords = 1;
+  static __device__ __forceinline__ void put(unsigned long long* w, float v, unsigned tag) {
+    st_ag(w, ((unsigned long long)tag << 32) | __float_as_uint(v));
+  }
+  // returns false if the word does not carry `tag` yet
+  static __device__ __forceinline__ bool get(const unsigned long long* w, unsigned tag, float& v) {
+    const unsigned long long x = ld_ag(w);
+    v = __uint_as_float((unsigned)x);
+    return (unsigned)(x >> 32) == tag;
+  }
+  static __device__ __forceinline__ unsigned key(float a) { return __float_as_uint(a); }
+};
+template <>
+struct TagWords<double> {
+  static constexpr int kWords = 2;
+  static __device__ __forceinline__ void put(unsigned long long* w, double v, unsigned tag) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    st_ag(w, ((unsigned long long)tag << 32) | (b & 0xffffffffull));
+    st_ag(w + 1, ((unsigned long long)tag << 32) | (b >> 32));
+  }
+  static __device__ __forceinline__ bool get(const unsigned long long* w, unsigned tag, double& v) {
+    const unsigned long long lo = ld_ag(w), hi = ld_ag(w + 1);
+    v = __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+    return (unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag;
+  }
+  static __device__ __forceinline__ unsigned key(double a) {
+    return __float_as_uint(__double2float_rz(a));
+  }
+};
+
+// ranking of (|a|, row): larger |a| first, then the lower row
+template <typename T>
+__device__ __forceinline__ bool better(T ov, int orr, T v, int r) {
   return ov > v || (ov == v && orr < r);
 }
 
-// Grid-wide barrier on a monotonic counter: every workgroup adds 1 per use
-// and waits until the counter reaches `target` (= uses so far x workgroups).
-// Every wave first waits for its own outstanding global stores (they are then
-// in L2); after the workgroup barrier, thread 0's agent-scope release writes L2
-// back and publishes them. Thread 0's acquire (which also invalidates the CU's
-// caches) precedes every wave's later loads through the second barrier.
-__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned target) {
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(1);
+// Reads the kb-value tagged record at `w` into s in the rotated order of column
+// step c (s[k] = column (c + k) mod W, zeros past kb); wave 0 only. Returns
+// false if the wait ran out (abort raised).
+template <typename T, int W>
+__device__ bool read_record(const unsigned long long* w, unsigned tag, int kb, int c, T* s,
+                            unsigned* abort_flag) {
+  constexpr int NW = TagWords<T>::kWords;
+  const int lane = threadIdx.x & 63;
+  for (unsigned spin = 0;; ++spin) {
+    bool ok = true;
+    T val[W / 64 > 0 ? W / 64 : 1];
+#pragma unroll
+    for (int q = 0; q < (W + 63) / 64; ++q) {
+      const int j = lane + 64 * q;
+      val[q] = T(0);
+      if (j < kb && j < W) ok = TagWords<T>::get(w + (size_t)j * NW, tag, val[q]) && ok;
+    }
+    if (__all(ok)) {
+#pragma unroll
+      for (int q = 0; q < (W + 63) / 64; ++q) {
+        const int j = lane + 64 * q;
+        if (j < W) s[(j - c + W) % W] = val[q];
+      }
+      return true;
+    }
+    if (spin > kSpinLimit || ld_ag32(abort_flag)) {
+      if (lane == 0) atomicExch(abort_flag, 1u);
+      return false;
+    }
   }
-  __syncthreads();
 }
 
-template <typename T, int NW, int CG>
-__global__ __launch_bounds__(NW * 64, 3) void panel_kernel(PanelArgs<T> a) {
-  constexpr int KB = NW * CG;  // panel width of this instantiation
-  __shared__ T s_l[kPR];
-  __shared__ T s_stage[2][KB];
-  __shared__ T s_pr[KB];   // pivot row (each wave fills and reads its own columns)
-  __shared__ T s_rc[KB];   // row c before the interchange
-  __shared__ int s_cand;
+template <typename T, int W>
+__global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
+  using TW = TagWords<T>;
+  constexpr int NWD = TW::kWords;
+  constexpr int RW = W * NWD;  // data words per record
+  __shared__ __align__(16) T s_wrow[kPW][W];  // each wave's candidate row
+  __shared__ __align__(16) T s_rowc[W];       // row at position c (workgroup 0)
+  __shared__ __align__(16) T s_pr[W];         // pivot row
+  __shared__ __align__(16) T s_rc[W];         // row c, for the owner of the pivot row
+  __shared__ T s_wval[kPW];
+  __shared__ int s_wrowi[kPW];
+  __shared__ int s_p, s_ok, s_abort;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int blk = blockIdx.x;
   const int rbase = blk * kPR;
-  const int r = rbase + lane;  // panel-relative row
+  const int r = rbase + w * 64 + lane;  // panel-relative row
   const bool valid = r < a.m;
   const size_t lda = (size_t)a.lda;
   T* Ap = a.A + (size_t)a.k0 + (size_t)a.k0 * lda;
-  const int G = a.G;
-  T v[CG];
+  const int G = a.G, kb = a.kb;
+  T v[W];
 #pragma unroll
-  for (int j = 0; j < CG; ++j) {
-    const int col = w * CG + j;
-    v[j] = (valid && col < a.kb) ? Ap[(size_t)r + (size_t)col * lda] : T(0);
-  }
-  unsigned target = 0;
-  for (int g = 0; g < NW; ++g) {
-    if (g * CG >= a.kb) break;
+  for (int j = 0; j < W; ++j) v[j] = (valid && j < kb) ? Ap[(size_t)r + (size_t)j * lda] : T(0);
+  if (tid == 0) s_abort = 0;
+  int rot = 0;  // column steps done: register k holds column (rot + k) mod W
+
+  for (int c = 0; c < kb; ++c) {
+    const int par = c & 1;
+    const unsigned tag8 = (unsigned)(c + 1);
+    const unsigned tag32 = (a.epoch << 8) | tag8;
+    // 1. wave candidate over rows >= c (column c is register 0)
+    T bv = (valid && r >= c) ? fabs(v[0]) : T(-1);
+    int br = (valid && r >= c) ? r : INT_MAX;
 #pragma unroll
-    for (int jc = 0; jc < CG; ++jc) {
-      const int c = g * CG + jc;
-      if (c < a.kb) {
-      const int par = c & 1;
-      // 1. local candidate in wave g
-      if (w == g) {
-        double bv = (valid && r >= c) ? fabs((double)v[jc]) : -1.0;
-        int br = (valid && r >= c) ? r : INT_MAX;
+    for (int off = 32; off >= 1; off >>= 1) {
+      const T ov = __shfl_xor(bv, off, 64);
+      const int orr = __shfl_xor(br, off, 64);
+      if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+    }
+    if (lane == 0) { s_wval[w] = bv; s_wrowi[w] = br; }
+    if (br != INT_MAX && r == br) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) s_wrow[w][j] = v[j];
+    }
+    if (r == c) {
+#pragma unroll
+      for (int j = 0; j < W; ++j) s_rowc[j] = v[j];
+    }
+    __syncthreads();
+    // 2.-3. wave 0: publish, poll, fetch the pivot row
+    if (w == 0) {
+      T cv = T(-1);
+      int cr = INT_MAX, cw = 0;
+      if (lane < kPW) { cv = s_wval[lane]; cr = s_wrowi[lane]; cw = lane; }
+#pragma unroll
+      for (int off = 4; off >= 1; off >>= 1) {
+        const T ov = __shfl_xor(cv, off, 64);
+        const int orr = __shfl_xor(cr, off, 64);
+        const int ow = __shfl_xor(cw, off, 64);
+        if (better(ov, orr, cv, cr)) { cv = ov; cr = orr; cw = ow; }
+      }
+      cv = __shfl(cv, 0, 64);
+      cr = __shfl(cr, 0, 64);
+      cw = __shfl(cw, 0, 64);
+      unsigned long long* rec = a.recs + ((size_t)par * a.gstride + blk) * RW;
+      if (cr != INT_MAX) {
+        for (int j = lane; j < kb; j += 64)
+          TW::put(rec + (size_t)j * NWD, s_wrow[cw][(j - c + W) % W], tag32);
+      }
+      if (blk == 0) {
+        unsigned long long* rc = a.rowc + (size_t)par * RW;
+        for (int j = lane; j < kb; j += 64)
+          TW::put(rc + (size_t)j * NWD, s_rowc[(j - c + W) % W], tag32);
+      }
+      if (lane == 0) {
+        const unsigned k32 = (cr != INT_MAX) ? TW::key(cv) : 0u;
+        const unsigned row24 = (cr != INT_MAX) ? (unsigned)cr : 0xffffffu;
+        st_ag(&a.keys[(size_t)par * a.gstride + blk],
+              ((unsigned long long)k32 << 32) | ((unsigned long long)tag8 << 24) | row24);
+      }
+      // poll the keys of all workgroups
+      unsigned long long bk = 0;  // (key << 24) | (0xffffff - row): larger is better
+      int bb = 0;
+      bool aborted = false;
+      for (unsigned spin = 0;; ++spin) {
+        bool ok = true;
+        unsigned long long lk = 0;
+        int lb = 0;
+        for (int q = lane; q < G; q += 64) {
+          const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
+          if (((unsigned)(x >> 24) & 0xffu) != tag8) ok = false;
+          const unsigned long long ck = ((x >> 32) << 24) | (0xffffffull - (x & 0xffffffull));
+          if (ck > lk) { lk = ck; lb = q; }
+        }
+        if (__all(ok)) { bk = lk; bb = lb; break; }
+        if (spin > kSpinLimit || ld_ag32(a.abort_flag)) {
+          if (lane == 0) atomicExch(a.abort_flag, 1u);
+          aborted = true;
+          break;
+        }
+      }
+      int p = INT_MAX;
+      bool nz = false;
+      if (!aborted) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
-          const double ov = __shfl_xor(bv, off, 64);
-          const int orr = __shfl_xor(br, off, 64);
-          if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+          const unsigned long long ok_ = __shfl_xor(bk, off, 64);
+          const int ob = __shfl_xor(bb, off, 64);
+          if (ok_ > bk) { bk = ok_; bb = ob; }
         }
-        if (lane == 0) {
-          a.st_val[par * G + blk] = bv;
-          a.st_row[par * G + blk] = br;
-          s_cand = (br == INT_MAX) ? -1 : br - rbase;
-        }
-      }
-      __syncthreads();
-      const int cl = s_cand;
-      const bool own_c = (c >= rbase) && (c < rbase + kPR);
-      if (cl >= 0 && lane == cl) {
+        const unsigned kmax = (unsigned)(bk >> 24);
+        int pb = bb;
+        p = (int)(0xffffffull - (bk & 0xffffffull));
+        if (sizeof(T) == 8) {
+          // workgroups tied on the FP32-rounded key: decide on the exact values
+          int ntie = 0;
+          for (int q = lane; q < G; q += 64) {
+            const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
+            if ((unsigned)(x >> 32) == kmax && (x & 0xffffffull) != 0xffffffull) ++ntie;
+          }
+          for (int off = 32; off >= 1; off >>= 1) ntie += __shfl_xor(ntie, off, 64);
+          if (ntie > 1) {
+            T ev = T(-1);
+            int er = INT_MAX, eb = 0;
+            for (int q = lane; q < G && !aborted; q += 64) {
+              const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
+              if ((unsigned)(x >> 32) != kmax || (x & 0xffffffull) == 0xffffffull) continue;
+              const unsigned long long* wq =
+                  a.recs + ((size_t)par * a.gstride + q) * RW + (size_t)c * NWD;
+              T val = T(0);
+              unsigned spin = 0;
+              while (!TW::get(wq, tag32, val)) {
+                if (++spin > kSpinLimit || ld_ag32(a.abort_flag)) { aborted = true; break; }
+              }
+              const int rq = (int)(x & 0xffffffull);
+              if (better(fabs(val), rq, ev, er)) { ev = fabs(val); er = rq; eb = q; }
+            }
+            if (__any(aborted)) {
+              if (lane == 0) atomicExch(a.abort_flag, 1u);
+              aborted = true;
+            } else {
 #pragma unroll
-        for (int j = 0; j < CG; ++j) s_stage[0][w * CG + j] = v[j];
-      }
-      if (own_c && lane == c - rbase) {
-#pragma unroll
-        for (int j = 0; j < CG; ++j) s_stage[1][w * CG + j] = v[j];
-      }
-      __syncthreads();
-      if (cl >= 0)
-        for (int t = tid; t < KB; t += NW * 64)
-          a.st_data[((size_t)par * G + blk) * KB + t] = s_stage[0][t];
-      if (own_c)
-        for (int t = tid; t < KB; t += NW * 64) a.st_rowc[(size_t)par * KB + t] = s_stage[1][t];
-      // 2. grid barrier
-      target += (unsigned)G;
-      grid_barrier(a.bar, target);
-      // 3. global winner (each wave independently)
-      double bv = -1.0;
-      int br = INT_MAX, bb = 0;
-      for (int q = lane; q < G; q += 64) {
-        const double vq = a.st_val[par * G + q];
-        const int rq = a.st_row[par * G + q];
-        if (rq != INT_MAX && better(vq, rq, bv, br)) { bv = vq; br = rq; bb = q; }
-      }
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double ov = __shfl_xor(bv, off, 64);
-        const int orr = __shfl_xor(br, off, 64);
-        const int ob = __shfl_xor(bb, off, 64);
-        if (better(ov, orr, bv, br)) { bv = ov; br = orr; bb = ob; }
-      }
-      const int p = __builtin_amdgcn_readfirstlane(br);
-      const int pb = __builtin_amdgcn_readfirstlane(bb);
-      if (!(bv > 0.0)) {  // zero column: no interchange, no scaling (LAPACK)
-        if (blk == 0 && tid == 0) {
-          a.ipiv[a.k0 + c] = a.k0 + c;
-          atomicCAS(a.info, 0, a.k0 + c + 1);
-        }
-      } else {
-      // 4. pivot row of this wave's columns into LDS (lane j loads column j;
-      //    LDS operations of one wave complete in order), interchange
-      T* pr = s_pr + w * CG;
-      if (lane < CG) pr[lane] = a.st_data[((size_t)par * G + pb) * KB + w * CG + lane];
-      if (p != c) {
-        T* rc = s_rc + w * CG;
-        if (lane < CG) rc[lane] = a.st_rowc[(size_t)par * KB + w * CG + lane];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && r == p) {
-#pragma unroll
-          for (int j = 0; j < CG; ++j) v[j] = rc[j];
-        }
-        if (valid && r == c) {
-#pragma unroll
-          for (int j = 0; j < CG; ++j) v[j] = pr[j];
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-      if (blk == 0 && tid == 0) a.ipiv[a.k0 + c] = a.k0 + p;
-      // 5. multipliers (wave g) and rank-1 update of rows below c
-      if (w == g) {
-        const bool below = valid && r > c;
-        const T l = below ? v[jc] / pr[jc] : T(0);
-        if (below) v[jc] = l;
-        s_l[lane] = l;
-      }
-      __syncthreads();
-      const T l = s_l[lane];
-      // (pivot-row reads in chunks of 8 columns: bounded register live ranges)
-      if (w > g) {
-#pragma unroll
-        for (int j0 = 0; j0 < CG; j0 += 8) {
-#pragma unroll
-          for (int j = j0; j < j0 + 8; ++j) v[j] -= l * pr[j];
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      } else if (w == g) {
-#pragma unroll
-        for (int j0 = 0; j0 < CG; j0 += 8) {
-          if (j0 + 8 > jc + 1) {
-#pragma unroll
-            for (int j = (j0 > jc + 1 ? j0 : jc + 1); j < j0 + 8; ++j) v[j] -= l * pr[j];
-            __builtin_amdgcn_sched_barrier(0);
+              for (int off = 32; off >= 1; off >>= 1) {
+                const T ov = __shfl_xor(ev, off, 64);
+                const int orr = __shfl_xor(er, off, 64);
+                const int ob = __shfl_xor(eb, off, 64);
+                if (better(ov, orr, ev, er)) { ev = ov; er = orr; eb = ob; }
+              }
+              p = er;
+              pb = eb;
+            }
           }
         }
+        if (!aborted) {
+          const unsigned long long* wrec = a.recs + ((size_t)par * a.gstride + pb) * RW;
+          aborted = !read_record<T, W>(wrec, tag32, kb, c, s_pr, a.abort_flag);
+        }
+        if (!aborted) {
+          nz = s_pr[0] != T(0);
+          if (nz && p != c && p >= rbase && p < rbase + kPR)
+            aborted = !read_record<T, W>(a.rowc + (size_t)par * RW, tag32, kb, c, s_rc,
+                                         a.abort_flag);
+        }
       }
-      }  // nonzero pivot
-      }  // c < kb
+      if (lane == 0) {
+        s_p = p;
+        s_ok = nz ? 1 : 0;
+        s_abort = aborted ? 1 : 0;
+        if (!aborted && blk == 0) {
+          a.ipiv[a.k0 + c] = a.k0 + (nz ? p : c);
+          if (!nz) atomicCAS(a.info, 0, a.k0 + c + 1);
+        }
+      }
     }
-  }
+    __syncthreads();
+    if (s_abort) break;
+    // 4. interchange, multipliers, rank-1 update (zero pivot: nothing, as LAPACK)
+    if (s_ok) {
+      const int p = s_p;
+      if (p != c) {
+        if (r == p) {
 #pragma unroll
-  for (int j = 0; j < CG; ++j) {
-    const int col = w * CG + j;
-    if (valid && col < a.kb) Ap[(size_t)r + (size_t)col * lda] = v[j];
+          for (int k = 0; k < W; ++k) v[k] = s_rc[k];
+        } else if (r == c) {
+#pragma unroll
+          for (int k = 0; k < W; ++k) v[k] = s_pr[k];
+        }
+      }
+      const T piv = s_pr[0];
+      const bool below = valid && r > c;
+      const T l = below ? v[0] / piv : T(0);
+      if (below) v[0] = l;
+      // register k >= 1 holds column c + k while c + k < W (later columns)
+#pragma unroll
+      for (int k = 1; k < W; ++k) {
+        const T lk = (k < W - c) ? l : T(0);
+        v[k] -= lk * s_pr[k];
+      }
+    }
+    // rotate: column c + 1 into register 0
+    {
+      const T t0 = v[0];
+#pragma unroll
+      for (int k = 0; k < W - 1; ++k) v[k] = v[k + 1];
+      v[W - 1] = t0;
+    }
+    ++rot;
+  }
+  // register k holds column (rot + k) mod W
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int j = (rot + k) % W;
+    if (valid && j < kb) Ap[(size_t)r + (size_t)j * lda] = v[k];
   }
 }
 
@@ -467,6 +626,183 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, const
 }
 
 // ---------------------------------------------------------------------------
+// C -= A B for 16-byte aligned operands: the trailing update of the LU.
+//
+// Workgroup tile BM x BN, NWM x NWN waves, each wave (BM/NWM) x (BN/NWN) built
+// from 16x16 MFMA blocks; K in steps of BK = 16 through a double-buffered LDS
+// stage. The accumulators start at zero and the epilogue subtracts them from
+// C (read once, written once per launch).
+//
+// LDS layouts: A as [k][m] (m contiguous: 16-byte stores from the column-major
+// source, conflict-free 4-byte fragment reads), B as [n][k] (k contiguous: the
+// four k of one MFMA lane come in one 16-byte read). Within a K-step of 16 the
+// MFMA k-slot q of a lane (lane / 16) takes source k = 4q + s at sub-step s,
+// identically for A and B, so each product term appears once.
+//
+// Tile order: workgroups are dealt to the 8 XCDs round-robin; block b is
+// mapped to tile xcd_chunk(b) so each XCD walks a contiguous range of
+// column-major tiles (shared B columns and neighbouring A rows in its L2).
+// ---------------------------------------------------------------------------
+template <typename T, int BM, int BN, int NWM, int NWN>
+__global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
+    int M, int N, int K, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
+    T* __restrict__ C, int ldc, int tiles_m, int ntiles) {
+  using acc_t = typename Mfma<T>::acc_t;
+  constexpr int NT = NWM * NWN * 64;
+  constexpr int BK = 16;
+  constexpr int VEC = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  constexpr int WTM = BM / NWM, WTN = BN / NWN;
+  constexpr int TI = WTM / 16, TJ = WTN / 16;
+  constexpr int PADA = VEC, PADB = VEC;
+  constexpr int LA = BM * BK / VEC / NT, LB = BN * BK / VEC / NT;
+  static_assert(LA * VEC * NT == BM * BK && LB * VEC * NT == BN * BK, "tile/thread mismatch");
+  __shared__ __align__(16) T As[2][BK][BM + PADA];
+  __shared__ __align__(16) T Bs[2][BN][BK + PADB];
+
+  // XCD-chunked tile index
+  const int bid = blockIdx.x;
+  int tile = bid;
+  if (ntiles >= 8) {
+    const int q = ntiles >> 3, rem = ntiles & 7, x = bid & 7, j = bid >> 3;
+    tile = (x < rem) ? x * (q + 1) + j : rem * (q + 1) + (x - rem) * q + j;
+  }
+  const int bm = (tile % tiles_m) * BM, bn = (tile / tiles_m) * BN;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave % NWM) * WTM, wn = (wave / NWM) * WTN;
+  const int l16 = lane & 15, q4 = lane >> 4;
+
+  // Lane element (ti, tj, r) of the tile sits at row bm + wm + 16 ti + l16 and
+  // column bn + wn + 16 tj + cofs(tj, r): a per-lane base plus a wave-uniform
+  // offset.
+  const bool interior = (bm + BM <= M) && (bn + BN <= N);
+  const int lane_col = (sizeof(T) == 4) ? q4 * 4 : q4;
+  T* Cl = C + (size_t)(bm + wm + l16) + (size_t)(bn + wn + lane_col) * ldc;
+  auto cofs = [](int tj, int r) { return tj * 16 + ((sizeof(T) == 4) ? r : r * 4); };
+  acc_t acc[TI][TJ];
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj) acc[ti][tj] = acc_t{0, 0, 0, 0};
+
+  vec_t ra[LA], rb[LB];
+  auto gload = [&](int kt) {
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+      const int v = tid + u * NT;
+      const int m4 = (v % (BM / VEC)) * VEC, k = v / (BM / VEC);
+      const int r = bm + m4, kk = kt + k;
+      const T* src = A + (size_t)r + (size_t)kk * lda;
+      if (kk < K && r + VEC <= M) {
+        ra[u] = *reinterpret_cast<const vec_t*>(src);
+      } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) ra[u][e] = (kk < K && r + e < M) ? src[e] : T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int v = tid + u * NT;
+      const int k4 = (v % (BK / VEC)) * VEC, n = v / (BK / VEC);
+      const int cb = bn + n, kk = kt + k4;
+      const T* src = B + (size_t)kk + (size_t)cb * ldb;
+      if (cb < N && kk + VEC <= K) {
+        rb[u] = *reinterpret_cast<const vec_t*>(src);
+      } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) rb[u][e] = (cb < N && kk + e < K) ? src[e] : T(0);
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < LA; ++u) {
+      const int v = tid + u * NT;
+      const int m4 = (v % (BM / VEC)) * VEC, k = v / (BM / VEC);
+      *reinterpret_cast<vec_t*>(&As[buf][k][m4]) = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < LB; ++u) {
+      const int v = tid + u * NT;
+      const int k4 = (v % (BK / VEC)) * VEC, n = v / (BK / VEC);
+      *reinterpret_cast<vec_t*>(&Bs[buf][n][k4]) = rb[u];
+    }
+  };
+
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  int buf = 0;
+  for (int kt = 0; kt < K; kt += BK) {
+    const bool more = kt + BK < K;
+    if (more) gload(kt + BK);
+    T fb[TJ][4];
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj) {
+      const T* p = &Bs[buf][wn + tj * 16 + l16][q4 * 4];
+#pragma unroll
+      for (int h = 0; h < 4; h += VEC) {
+        const vec_t x = *reinterpret_cast<const vec_t*>(p + h);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) fb[tj][h + e] = x[e];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      T fa[TI];
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti) fa[ti] = As[buf][q4 * 4 + s][wm + ti * 16 + l16];
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < TJ; ++tj) acc[ti][tj] = Mfma<T>::op(fb[tj][s], fa[ti], acc[ti][tj]);
+    }
+    if (more) {
+      sstore(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+  // C -= acc: the product is summed on its own and subtracted with one
+  // rounding (as the reference GEMM update does), one MFMA row block of C at a
+  // time with the next block's loads in flight.
+  if (interior) {
+    T cv[2][TJ][4];
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[0][tj][r] = Cl[(size_t)cofs(tj, r) * ldc];
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+      if (ti + 1 < TI) {
+#pragma unroll
+        for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            cv[(ti + 1) & 1][tj][r] = Cl[(ti + 1) * 16 + (size_t)cofs(tj, r) * ldc];
+      }
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Cl[ti * 16 + (size_t)cofs(tj, r) * ldc] = cv[ti & 1][tj][r] - acc[ti][tj][r];
+    }
+  } else {
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+      const int i = bm + wm + ti * 16 + l16;
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = bn + wn + lane_col + cofs(tj, r);
+          if (i < M && j < N) Cl[ti * 16 + (size_t)cofs(tj, r) * ldc] -= acc[ti][tj][r];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Triangular solves with FP64 accumulation (factors in T), blocked by kSolveNB
 // with the inverses of the diagonal blocks precomputed once per factorization.
 // ---------------------------------------------------------------------------
@@ -584,10 +920,41 @@ __global__ void scatter_kernel(int n, const int* __restrict__ perm, const double
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
+template <typename T, int BM, int BN, int NWM, int NWN>
+void launch_tile_gemm(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B,
+                      int ldb, T* C, int ldc) {
+  const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
+  const int nt = tm * tn;
+  hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, NWM, NWN>), dim3(nt), dim3(NWM * NWN * 64), 0,
+                     c->stream, M, N, K, A, lda, B, ldb, C, ldc, tm, nt);
+}
+
+// C -= A B. 16-byte aligned operands (every LU call on the padded matrices)
+// take the tile kernel: large products on 256-wide tiles, small ones on 128x128
+// tiles for more workgroups; anything else takes the generic kernel.
 template <typename T>
 int gemm_sub(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B, int ldb, T* C,
              int ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return BEMHIP_OK;
+  constexpr int VEC = 16 / (int)sizeof(T);
+  const bool aligned = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) &&
+                       (lda % VEC == 0) && (ldb % VEC == 0);
+  if (aligned) {
+    const int cus = std::max(1, c->num_cu);
+    const long big = (long)((M + 255) / 256) * (long)((N + 255) / 256);
+    if constexpr (sizeof(T) == 4) {
+      if (big >= 2L * cus)
+        launch_tile_gemm<T, 256, 256, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
+      else
+        launch_tile_gemm<T, 128, 128, 2, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
+    } else {
+      if (big >= 2L * cus)
+        launch_tile_gemm<T, 256, 128, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
+      else
+        launch_tile_gemm<T, 128, 128, 2, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
+    }
+    return BEMHIP_OK;
+  }
   dim3 grid((M + kBM - 1) / kBM, (N + kBN - 1) / kBN);
   hipLaunchKernelGGL((gemm_kernel<T>), grid, dim3(256), 0, c->stream, M, N, K, A, lda, B, ldb,
                      C, ldc);
@@ -613,15 +980,13 @@ int trsm_rec(bemhip_ctx* c, int w, int ncols, const T* L, int lda, T* B, int ldb
 
 template <typename T>
 int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host) {
-  // Panel width 128 = 4 waves x 32 columns for both precisions (the panel of
-  // 32769 rows stays register-resident with 3 workgroups per CU).
-  constexpr int NW = 4;
-  constexpr int CG = 32;
-  constexpr int KB = NW * CG;
+  // Register-resident panel width: 128 FP32 or 64 FP64 values per thread.
+  constexpr int KB = (sizeof(T) == 4) ? 128 : 64;
+  constexpr int RW = KB * TagWords<T>::kWords;
   const int Gmax = (n + kPR - 1) / kPR;
+  if (n >= (1 << 24)) return fail(c, BEMHIP_ERR_ARG, "lu: n must be below 2^24");
   // work buffers
-  const size_t need = sizeof(double) * 2 * Gmax + sizeof(int) * 2 * Gmax +
-                      sizeof(T) * 2 * (size_t)Gmax * KB + sizeof(T) * 2 * KB +
+  const size_t need = 8 * 2 * (size_t)Gmax + 8 * 2 * (size_t)Gmax * RW + 8 * 2 * (size_t)RW +
                       sizeof(int) * (4 * kMaxPanel + 16) + 8 * 256;
   BEMHIP_HIP_TRY(c, c->lu_work.ensure(need));
   char* wp = c->lu_work.ptr;
@@ -630,92 +995,121 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     wp += (bytes + 255) & ~size_t(255);
     return p;
   };
-  double* st_val = (double*)carve(sizeof(double) * 2 * Gmax);
-  int* st_row = (int*)carve(sizeof(int) * 2 * Gmax);
-  T* st_data = (T*)carve(sizeof(T) * 2 * (size_t)Gmax * KB);
-  T* st_rowc = (T*)carve(sizeof(T) * 2 * KB);
+  unsigned long long* keys = (unsigned long long*)carve(8 * 2 * (size_t)Gmax);
+  unsigned long long* recs = (unsigned long long*)carve(8 * 2 * (size_t)Gmax * RW);
+  unsigned long long* rowc = (unsigned long long*)carve(8 * 2 * (size_t)RW);
   int* ldst = (int*)carve(sizeof(int) * 2 * kMaxPanel);
   int* lsrc = (int*)carve(sizeof(int) * 2 * kMaxPanel);
   int* llen = (int*)carve(sizeof(int) * 16);
   int* info = llen + 4;
-  unsigned* bar = (unsigned*)(llen + 8);
+  unsigned* abort_flag = (unsigned*)(llen + 8);
   if ((size_t)(wp - c->lu_work.ptr) > c->lu_work.count)
     return fail(c, BEMHIP_ERR_STATE, "lu: work buffer carve overflow");
-  BEMHIP_HIP_TRY(c, hipMemsetAsync(info, 0, sizeof(int), c->stream));
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(llen, 0, sizeof(int) * 16, c->stream));
   int occ = 0;
   BEMHIP_HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &occ, (const void*)panel_kernel<T, NW, CG>, NW * 64, 0));
+                        &occ, (const void*)panel_kernel<T, KB>, kPR, 0));
   if ((int64_t)occ * std::max(1, c->num_cu) < Gmax)
     return fail(c, BEMHIP_ERR_STATE, "lu: panel workgroups cannot all be resident");
   c->dinv_of = nullptr;
 
-  const int npanels = (n + KB - 1) / KB;
-  std::vector<hipEvent_t> ev((size_t)npanels * 5);
-  for (auto& x : ev) hipEventCreate(&x);
-  for (int k0 = 0; k0 < n; k0 += KB) {
-    hipEvent_t* e = &ev[(size_t)(k0 / KB) * 5];
-    const int kb = std::min(KB, n - k0);
-    const int m = n - k0;
-    const int G = (m + kPR - 1) / kPR;
-    hipEventRecord(e[0], c->stream);
-    BEMHIP_HIP_TRY(c, hipMemsetAsync(bar, 0, sizeof(unsigned), c->stream));
-    PanelArgs<T> pa;
-    pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = k0; pa.G = G;
-    pa.ipiv = dipiv; pa.st_val = st_val; pa.st_row = st_row; pa.st_data = st_data;
-    pa.st_rowc = st_rowc; pa.info = info; pa.bar = bar;
-    void* args[] = {&pa};
-    BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, NW, CG>, dim3(G),
-                                                 dim3(NW * 64), args, 0, c->stream));
-    hipEventRecord(e[1], c->stream);
-    // interchanges outside the panel
-    hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, c->stream, dipiv, k0, kb, ldst, lsrc,
-                       llen);
-    const int ncols_out = n - kb;
-    if (ncols_out > 0)
-      hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_out + 3) / 4), dim3(256), 0, c->stream,
-                         dA, lda, 0, k0, k0 + kb, n, ldst, lsrc, llen);
-    BEMHIP_HIP_TRY(c, hipGetLastError());
-    hipEventRecord(e[2], c->stream);
-    const int nrest = n - k0 - kb;
+  // Two-level blocking (the structure of LAPACK getrf over getrf2): outer
+  // blocks of OB = 2 KB columns, each factored as KB-wide register-resident
+  // panels with their interchanges applied to every other column and the rest
+  // of the outer block updated right away; the trailing matrix is updated once
+  // per outer block, a K = OB product.
+  constexpr int OB = 256;
+  enum { kPanel = 0, kSwap = 1, kTrsm = 2, kGemm = 3 };
+  std::vector<hipEvent_t> ev;
+  std::vector<int> cat;
+  ev.reserve((size_t)(n / KB + 2) * 8);
+  auto mark = [&](int category) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    hipEventRecord(e, c->stream);
+    ev.push_back(e);
+    cat.push_back(category);  // category of the segment that ends at this event
+  };
+  mark(-1);
+  for (int k0 = 0; k0 < n; k0 += OB) {
+    const int ob = std::min(OB, n - k0);
+    for (int s = 0; s < ob; s += KB) {
+      const int kc = k0 + s;
+      const int kb = std::min(KB, ob - s);
+      const int m = n - kc;
+      const int G = (m + kPR - 1) / kPR;
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(keys, 0, 8 * 2 * (size_t)Gmax, c->stream));
+      PanelArgs<T> pa;
+      pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = kc; pa.G = G; pa.gstride = Gmax;
+      pa.epoch = (++c->panel_epoch) & 0xffffffu;
+      pa.ipiv = dipiv; pa.info = info; pa.keys = keys; pa.recs = recs; pa.rowc = rowc;
+      pa.abort_flag = abort_flag;
+      void* args[] = {&pa};
+      BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, KB>, dim3(G),
+                                                   dim3(kPR), args, 0, c->stream));
+      mark(kPanel);
+      // interchanges outside the panel
+      hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, c->stream, dipiv, kc, kb, ldst, lsrc,
+                         llen);
+      const int ncols_out = n - kb;
+      if (ncols_out > 0)
+        hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_out + 3) / 4), dim3(256), 0, c->stream,
+                           dA, lda, 0, kc, kc + kb, n, ldst, lsrc, llen);
+      BEMHIP_HIP_TRY(c, hipGetLastError());
+      mark(kSwap);
+      // the rest of this outer block
+      const int rest = k0 + ob - (kc + kb);
+      if (rest > 0) {
+        T* L11 = dA + (size_t)kc + (size_t)kc * lda;
+        T* A12 = dA + (size_t)kc + (size_t)(kc + kb) * lda;
+        int st = trsm_rec(c, kb, rest, (const T*)L11, lda, A12, lda);
+        if (st) return st;
+        mark(kTrsm);
+        T* L21 = dA + (size_t)(kc + kb) + (size_t)kc * lda;
+        T* A22 = dA + (size_t)(kc + kb) + (size_t)(kc + kb) * lda;
+        gemm_sub(c, n - kc - kb, rest, kb, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
+        BEMHIP_HIP_TRY(c, hipGetLastError());
+        mark(kGemm);
+      }
+    }
+    const int nrest = n - k0 - ob;
     if (nrest > 0) {
       T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
-      T* A12 = dA + (size_t)k0 + (size_t)(k0 + kb) * lda;
-      int st = trsm_rec(c, kb, nrest, (const T*)L11, lda, A12, lda);
+      T* A12 = dA + (size_t)k0 + (size_t)(k0 + ob) * lda;
+      int st = trsm_rec(c, ob, nrest, (const T*)L11, lda, A12, lda);
       if (st) return st;
-      hipEventRecord(e[3], c->stream);
-      T* L21 = dA + (size_t)(k0 + kb) + (size_t)k0 * lda;
-      T* A22 = dA + (size_t)(k0 + kb) + (size_t)(k0 + kb) * lda;
-      gemm_sub(c, nrest, nrest, kb, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
+      mark(kTrsm);
+      T* L21 = dA + (size_t)(k0 + ob) + (size_t)k0 * lda;
+      T* A22 = dA + (size_t)(k0 + ob) + (size_t)(k0 + ob) * lda;
+      gemm_sub(c, nrest, nrest, ob, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
       BEMHIP_HIP_TRY(c, hipGetLastError());
-      hipEventRecord(e[4], c->stream);
+      mark(kGemm);
     }
   }
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  double t_panel = 0, t_swap = 0, t_trsm = 0, t_gemm = 0;
-  for (int k0 = 0; k0 < n; k0 += KB) {
-    hipEvent_t* e = &ev[(size_t)(k0 / KB) * 5];
-    const int nrest = n - k0 - std::min(KB, n - k0);
-    float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
-    hipEventElapsedTime(&t01, e[0], e[1]);
-    hipEventElapsedTime(&t12, e[1], e[2]);
-    if (nrest > 0) {
-      hipEventElapsedTime(&t23, e[2], e[3]);
-      hipEventElapsedTime(&t34, e[3], e[4]);
-    }
-    t_panel += t01; t_swap += t12; t_trsm += t23; t_gemm += t34;
+  double tcat[4] = {0, 0, 0, 0};
+  for (size_t i = 1; i < ev.size(); ++i) {
+    float t = 0;
+    hipEventElapsedTime(&t, ev[i - 1], ev[i]);
+    tcat[cat[i]] += t;
   }
   for (auto& x : ev) hipEventDestroy(x);
+  const double t_panel = tcat[kPanel], t_swap = tcat[kSwap], t_trsm = tcat[kTrsm],
+               t_gemm = tcat[kGemm];
   const char* pfx = (sizeof(T) == 8) ? "lu64_" : "lu32_";
   c->ms[std::string(pfx) + "panel"] = t_panel;
   c->ms[std::string(pfx) + "swap"] = t_swap;
   c->ms[std::string(pfx) + "trsm"] = t_trsm;
   c->ms[std::string(pfx) + "gemm"] = t_gemm;
   c->stats[std::string(pfx) + "kb"] = KB;
+  c->stats[std::string(pfx) + "ob"] = OB;
   c->stats[std::string(pfx) + "panel_wgs"] = Gmax;
-  int infoh = 0;
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(&infoh, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int flags[16];
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(flags, llen, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (info_host) *info_host = infoh;
+  if (flags[8] != 0)
+    return fail(c, BEMHIP_ERR_HIP, "lu: panel exchange wait ran out (factorization abandoned)");
+  if (info_host) *info_host = flags[4];
   return BEMHIP_OK;
 }
 

@@ -248,19 +248,21 @@ extern "C" int bemhip_lu_factor(bemhip_ctx* c, int n, int dtype, void* A, int* i
   if (n < 1 || !A || !ipiv || (dtype != 0 && dtype != 1))
     return fail(c, BEMHIP_ERR_ARG, "lu_factor: need n >= 1, dtype 0/1 and non-NULL buffers");
   hipSetDevice(c->device);
+  // device copy with the padded leading dimension of the resident matrices
   const size_t es = dtype == 0 ? 8 : 4;
-  const size_t bytes = es * (size_t)n * n;
+  const size_t ld = padded_ld(n, dtype == 0 ? 32 : 64);
   DevBuf<char> dA;
   DevBuf<int> dp;
-  BEMHIP_HIP_TRY(c, dA.ensure(bytes));
+  BEMHIP_HIP_TRY(c, dA.ensure(es * ld * n));
   BEMHIP_HIP_TRY(c, dp.ensure(n));
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(dA.ptr, A, bytes, hipMemcpyHostToDevice, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpy2DAsync(dA.ptr, es * ld, A, es * n, es * n, n, hipMemcpyHostToDevice,
+                                     c->stream));
   int info = 0;
   hipEvent_t f0, f1;
   hipEventCreate(&f0);
   hipEventCreate(&f1);
   hipEventRecord(f0, c->stream);
-  int st = lu_factor_device(c, n, dtype, dA.ptr, n, dp.ptr, &info);
+  int st = lu_factor_device(c, n, dtype, dA.ptr, (int)ld, dp.ptr, &info);
   hipEventRecord(f1, c->stream);
   hipEventSynchronize(f1);
   float ms = 0;
@@ -269,7 +271,8 @@ extern "C" int bemhip_lu_factor(bemhip_ctx* c, int n, int dtype, void* A, int* i
   hipEventDestroy(f0);
   hipEventDestroy(f1);
   if (st) return st;
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(A, dA.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpy2DAsync(A, es * n, dA.ptr, es * ld, es * n, n, hipMemcpyDeviceToHost,
+                                     c->stream));
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(ipiv, dp.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (info) return fail(c, BEMHIP_ERR_SINGULAR, "lu_factor: zero pivot");
@@ -286,21 +289,22 @@ extern "C" int bemhip_lu_solve(bemhip_ctx* c, int n, int dtype, const void* LU, 
     if (ipiv[i] < i || ipiv[i] >= n) return fail(c, BEMHIP_ERR_ARG, "lu_solve: invalid ipiv");
   hipSetDevice(c->device);
   const size_t es = dtype == 0 ? 8 : 4;
-  const size_t bytes = es * (size_t)n * n;
+  const size_t ld = padded_ld(n, dtype == 0 ? 32 : 64);
   DevBuf<char> dA;
   DevBuf<int> dp, dperm;
   DevBuf<double> db;
-  BEMHIP_HIP_TRY(c, dA.ensure(bytes));
+  BEMHIP_HIP_TRY(c, dA.ensure(es * ld * n));
   BEMHIP_HIP_TRY(c, dp.ensure(n));
   BEMHIP_HIP_TRY(c, dperm.ensure(n));
   BEMHIP_HIP_TRY(c, db.ensure(n));
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(dA.ptr, LU, bytes, hipMemcpyHostToDevice, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpy2DAsync(dA.ptr, es * ld, LU, es * n, es * n, n, hipMemcpyHostToDevice,
+                                     c->stream));
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(dp.ptr, ipiv, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(db.ptr, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   int st = build_perm(c, n, dp.ptr, dperm.ptr);
   if (st) return st;
-  st = trans ? lu_solve_trans_device(c, n, dtype, dA.ptr, n, dperm.ptr, db.ptr)
-             : lu_solve_device(c, n, dtype, dA.ptr, n, dperm.ptr, db.ptr);
+  st = trans ? lu_solve_trans_device(c, n, dtype, dA.ptr, (int)ld, dperm.ptr, db.ptr)
+             : lu_solve_device(c, n, dtype, dA.ptr, (int)ld, dperm.ptr, db.ptr);
   if (st) return st;
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(b, db.ptr, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));

@@ -102,13 +102,24 @@ class Ctx:
         self.L.bemhip_destroy(self.h)
 
 
-def gemm_flops(n, kb):
+def gemm_flops(n, kb, ob):
+    """FP32 flops and launches of the LU's trailing-update GEMMs: per outer block
+    of ob columns, the kb-wide panels' updates of the rest of the block, then
+    the K = ob update of the trailing matrix."""
     tot = 0.0
     launches = 0
-    for k0 in range(0, n, kb):
-        r = n - k0 - min(kb, n - k0)
-        if r > 0:
-            tot += 2.0 * r * r * min(kb, n - k0)
+    for k0 in range(0, n, ob):
+        blk = min(ob, n - k0)
+        for s in range(0, blk, kb):
+            kc = k0 + s
+            w = min(kb, blk - s)
+            rest = k0 + blk - (kc + w)
+            if rest > 0 and n - kc - w > 0:
+                tot += 2.0 * (n - kc - w) * rest * w
+                launches += 1
+        nrest = n - k0 - blk
+        if nrest > 0:
+            tot += 2.0 * nrest * nrest * blk
             launches += 1
     return tot, launches
 
@@ -287,8 +298,9 @@ def main():
         detail["adjoint"] = adjoint_leg()
 
     # roofline of the dominant kernel (FP32 MFMA trailing update of the LU)
-    kb = int(counters["stats"].get("lu32_kb", 256))
-    gflops, glaunch = gemm_flops(nt, kb)
+    kb = int(counters["stats"].get("lu32_kb", 128))
+    ob = int(counters["stats"].get("lu32_ob", kb))
+    gflops, glaunch = gemm_flops(nt, kb, ob)
     gemm_s = float(np.mean(gemm_ms)) * 1e-3
     ach = gflops / gemm_s
     traffic = load_traffic(nt)

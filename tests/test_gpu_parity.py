@@ -149,7 +149,8 @@ def test_reference_square_T_equals_x():
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("n", [1, 7, 130, 517, 1000])
+# 6149: trailing updates large enough for the 256-wide GEMM tiles (>= 2 tiles per CU)
+@pytest.mark.parametrize("n", [1, 7, 130, 517, 1000, 6149])
 def test_lu_matches_lapack(n, dtype):
     import scipy.linalg as sl
     rng = np.random.default_rng(n)
