@@ -1,17 +1,21 @@
 // Influence-matrix assembly for the dense 2-D Laplace BEM path (gfx950).
 //
-// Kernels (one launch each, all on the context stream):
-//   far_kernel      rows x 32-element column blocks: near/far classification of
-//                   every (row, element) pair (src/Core/Assembly_full.jl:736-754),
-//                   far nodal lumping (:801-810) written column-major with lanes
-//                   over rows (coalesced), a near bit per pair, FP64 row sums.
-//   near_count / near_scan / near_fill   near bits -> row-sorted CSR of pairs.
+// Kernels (far_kernel on a second stream, concurrent with the near chain):
+//   near_classify   near/far classification of every (row, element) pair
+//                   (src/Core/Assembly_full.jl:736-754) into a bit per pair and
+//                   per-row pair counts; column blocks far from the rows'
+//                   bounding box are settled without pair tests.
+//   far_kernel      rows x 32-element column blocks: the same classification
+//                   and the far nodal lumping (:801-810) written column-major
+//                   with lanes over rows (coalesced), FP64 row sums.
+//   near_scan / near_fill   counts and near bits -> row-sorted CSR of pairs.
 //   near_kernel     one wavefront per near pair: tan-p3c rule for off-element
 //                   pairs (:649-695, src/Core/Integration.jl:68-93, :1117-1365),
 //                   interpolated Guiggiani for on-element pairs (:397-424,
 //                   src/Core/Integration.jl:1597-1845); lanes over quadrature
 //                   points, wave reductions.
-//   diag_kernel     rigid-body diagonal H_ii = -sum_{j!=i} H_ij (:384-390).
+//   op64_finalize   rigid-body diagonal H_ii = -sum_{j!=i} H_ij (:384-390) and
+//                   the near / on-element entries into H and G.
 //   op64_kernel / op64_finalize   FP64 operator passes that recompute the far
 //                   entries on the fly (row sums, right-hand side, A*x).
 //
@@ -104,6 +108,155 @@ __device__ __forceinline__ double rcp_f64(double x) {
   return fma(r, fma(-x, r, 1.0), r);
 }
 
+// Classification of pair (row at (px, py), element with collocation nodes
+// cx/cy): the node - source offsets (FP64 and their FP32 roundings) and the
+// near decision: on-element pairs are near; otherwise the FP32 pre-test settles
+// the clearly-far pairs and the rest take the exact FP64 test. Shared by
+// near_classify and far_kernel so that both make the same decision.
+template <int NN>
+__device__ __forceinline__ bool classify_pair(double px, double py, const double* cx,
+                                              const double* cy, double L, float lim2f,
+                                              double nf, bool onel, double* rx, double* ry,
+                                              float* rxf, float* ryf, float* d2f) {
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    rx[k] = cx[k] - px;
+    ry[k] = cy[k] - py;
+    rxf[k] = (float)rx[k];
+    ryf[k] = (float)ry[k];
+    d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
+  }
+  if (onel) return true;
+  if (clearly_far<NN>(rxf, ryf, d2f, lim2f)) return false;
+  double d2[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
+  return is_near<NN>(px, py, cx, cy, d2, L, nf);
+}
+
+// Element data of one 32-element column block in LDS for the classification.
+template <int NN>
+__device__ __forceinline__ void stage_class_block(const Geo& g, const Consts& cs, int e0,
+                                                  int ecount, double* s_cx, double* s_cy,
+                                                  double* s_L, float* s_lim2f) {
+  for (int t = threadIdx.x; t < ecount * NN; t += blockDim.x) {
+    const int j = e0 * NN + t;
+    s_cx[t] = g.x[j];
+    s_cy[t] = g.y[j];
+  }
+  for (int t = threadIdx.x; t < ecount; t += blockDim.x) {
+    const double L = g.elen[e0 + t];
+    s_L[t] = L;
+    const double lim = cs.nf * L;
+    // +inf (no far lumping) when the cutoff is disabled or overflows
+    s_lim2f[t] = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bounding box of the collocation nodes of each 32-element column block and
+// its largest cutoff nf * L: box[cb] = {lox, loy, hix, hiy, maxlim}.
+// ---------------------------------------------------------------------------
+template <int NN>
+__global__ void elem_block_boxes(Geo g, Consts cs, int ncb, double* __restrict__ box) {
+  const int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= ncb) return;
+  const int e0 = cb * kEB, e1 = min(g.ne, e0 + kEB);
+  double lox = INFINITY, loy = INFINITY, hix = -INFINITY, hiy = -INFINITY, ml = 0.0;
+  for (int e = e0; e < e1; ++e) {
+    for (int k = 0; k < NN; ++k) {
+      const double x = g.x[e * NN + k], y = g.y[e * NN + k];
+      lox = fmin(lox, x); hix = fmax(hix, x);
+      loy = fmin(loy, y); hiy = fmax(hiy, y);
+    }
+    ml = fmax(ml, cs.nf * g.elen[e]);
+  }
+  double* b = box + (size_t)cb * 5;
+  b[0] = lox; b[1] = loy; b[2] = hix; b[3] = hiy; b[4] = ml;
+}
+
+// ---------------------------------------------------------------------------
+// Near bits of every (row, element) pair (src/Core/Assembly_full.jl:736-754)
+// and the per-row near-pair counts: the classification of far_kernel without
+// the far entries, so that the near-pair list and the near/singular integrals
+// do not wait for the far field.
+// Block = 256 rows x a range of 32-element column blocks. A column block whose
+// node bounding box lies farther than (1 + 1e-6) x its largest cutoff from the
+// rows' bounding box holds only far pairs (every node distance and every
+// element bounding-box distance is at least the box distance, so the exact
+// test answers far); its bit words are written as zero without the pair tests.
+// Other column blocks take the pair tests of far_kernel (same arithmetic, same
+// decisions). row_cnt must be zero on entry; tiles[row block][column block]
+// records which column blocks took the pair tests (the others hold no near bit).
+// ---------------------------------------------------------------------------
+template <int NN>
+__global__ __launch_bounds__(kFarThreads) void near_classify(Geo g, Consts cs, int ncb,
+                                                             int cb_per_block,
+                                                             const double* __restrict__ box,
+                                                             uint32_t* __restrict__ nearbits,
+                                                             int* __restrict__ row_cnt,
+                                                             uint8_t* __restrict__ tiles) {
+  __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_L[kEB];
+  __shared__ float s_lim2f[kEB];
+  __shared__ double s_red[4][kFarThreads / 64];
+  const int i = blockIdx.x * kFarThreads + threadIdx.x;
+  const bool ok = i < g.nt;
+  const double px = ok ? g.x[i] : 0.0, py = ok ? g.y[i] : 0.0;
+  // bounding box of this block's rows
+  double rlox = ok ? px : INFINITY, rhix = ok ? px : -INFINITY;
+  double rloy = ok ? py : INFINITY, rhiy = ok ? py : -INFINITY;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    rlox = fmin(rlox, __shfl_xor(rlox, off, 64));
+    rhix = fmax(rhix, __shfl_xor(rhix, off, 64));
+    rloy = fmin(rloy, __shfl_xor(rloy, off, 64));
+    rhiy = fmax(rhiy, __shfl_xor(rhiy, off, 64));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_red[0][wv] = rlox; s_red[1][wv] = rhix; s_red[2][wv] = rloy; s_red[3][wv] = rhiy;
+  }
+  __syncthreads();
+  rlox = s_red[0][0]; rhix = s_red[1][0]; rloy = s_red[2][0]; rhiy = s_red[3][0];
+#pragma unroll
+  for (int q = 1; q < kFarThreads / 64; ++q) {
+    rlox = fmin(rlox, s_red[0][q]); rhix = fmax(rhix, s_red[1][q]);
+    rloy = fmin(rloy, s_red[2][q]); rhiy = fmax(rhiy, s_red[3][q]);
+  }
+  const int cb0 = blockIdx.y * cb_per_block, cb1 = min(ncb, cb0 + cb_per_block);
+  int cnt = 0;
+  for (int cb = cb0; cb < cb1; ++cb) {
+    const double* bx = box + (size_t)cb * 5;
+    const double dx = fmax(0.0, fmax(bx[0] - rhix, rlox - bx[2]));
+    const double dy = fmax(0.0, fmax(bx[1] - rhiy, rloy - bx[3]));
+    const double ml = bx[4];
+    const bool all_far = !(cs.nf > 1e20) && (dx * dx + dy * dy) > (ml * ml) * (1.0 + 1e-6);
+    if (threadIdx.x == 0) tiles[(size_t)blockIdx.x * ncb + cb] = all_far ? 0 : 1;
+    uint32_t bits = 0u;
+    if (!all_far) {  // uniform over the block
+      const int e0 = cb * kEB;
+      const int ecount = min(kEB, g.ne - e0);
+      __syncthreads();
+      stage_class_block<NN>(g, cs, e0, ecount, s_cx, s_cy, s_L, s_lim2f);
+      __syncthreads();
+      if (ok) {
+        for (int el = 0; el < ecount; ++el) {
+          double rx[NN], ry[NN];
+          float rxf[NN], ryf[NN], d2f[NN];
+          const int col0 = (e0 + el) * NN;
+          const bool onel = (i >= col0) && (i < col0 + NN);
+          if (classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el], s_lim2f[el],
+                                cs.nf, onel, rx, ry, rxf, ryf, d2f))
+            bits |= 1u << el;
+        }
+      }
+    }
+    if (ok) nearbits[(size_t)cb * g.nt + i] = bits;
+    cnt += __popc(bits);
+  }
+  if (ok && cnt) atomicAdd(&row_cnt[i], cnt);
+}
+
 // ---------------------------------------------------------------------------
 // Far-field tile kernel.
 // Block = 256 rows x 32 elements. Element data staged in LDS; each lane owns
@@ -117,7 +270,6 @@ __device__ __forceinline__ double rcp_f64(double x) {
 template <typename TS, int NN>
 __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* __restrict__ H,
                                                           TS* __restrict__ G,
-                                                          uint32_t* __restrict__ nearbits,
                                                           double* __restrict__ hpart) {
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
       s_jw[kEB * NN], s_L[kEB];
@@ -125,20 +277,13 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
   const int cb = blockIdx.y;
   const int e0 = cb * kEB;
   const int ecount = min(kEB, g.ne - e0);
+  stage_class_block<NN>(g, cs, e0, ecount, s_cx, s_cy, s_L, s_lim2f);
   for (int t = threadIdx.x; t < ecount * NN; t += blockDim.x) {
     const int j = e0 * NN + t;
-    s_cx[t] = g.x[j]; s_cy[t] = g.y[j];
     s_nx[t] = g.nx[j]; s_ny[t] = g.ny[j];
     s_jw[t] = g.jw[j];
     s_nxf[t] = (float)g.nx[j]; s_nyf[t] = (float)g.ny[j];
     s_jwf[t] = (float)g.jw[j];
-  }
-  for (int t = threadIdx.x; t < ecount; t += blockDim.x) {
-    const double L = g.elen[e0 + t];
-    s_L[t] = L;
-    const double lim = cs.nf * L;
-    // +inf (no far lumping) when the cutoff is disabled or overflows
-    s_lim2f[t] = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
   }
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
@@ -146,33 +291,16 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
   const double px = g.x[i], py = g.y[i];
   const size_t ld = g.ld;
   double rowsum = 0.0;
-  uint32_t bits = 0u;
   for (int el = 0; el < ecount; ++el) {
-    const double* cx = s_cx + el * NN;
-    const double* cy = s_cy + el * NN;
     double rx[NN], ry[NN];
     float rxf[NN], ryf[NN], d2f[NN];
-#pragma unroll
-    for (int k = 0; k < NN; ++k) {
-      rx[k] = cx[k] - px;
-      ry[k] = cy[k] - py;
-      rxf[k] = (float)rx[k];
-      ryf[k] = (float)ry[k];
-      d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
-    }
     const int col0 = (e0 + el) * NN;
     const bool onel = (i >= col0) && (i < col0 + NN);
-    bool nr = onel;
-    if (!onel && !clearly_far<NN>(rxf, ryf, d2f, s_lim2f[el])) {
-      double d2[NN];
-#pragma unroll
-      for (int k = 0; k < NN; ++k) d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
-      nr = is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf);
-    }
+    const bool nr = classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el],
+                                      s_lim2f[el], cs.nf, onel, rx, ry, rxf, ryf, d2f);
     TS* Hc = H + (size_t)col0 * ld + i;
     TS* Gc = G + (size_t)col0 * ld + i;
     if (nr) {
-      bits |= 1u << el;
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
         Hc[k * ld] = TS(0);
@@ -207,7 +335,6 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
       }
     }
   }
-  nearbits[(size_t)cb * g.nt + i] = bits;
   if (hpart) hpart[(size_t)cb * g.nt + i] = rowsum;
 }
 
@@ -296,7 +423,8 @@ __global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode, size_t l
                               const double* __restrict__ pair_h,
                               const double* __restrict__ pair_g, const double* __restrict__ wH,
                               const double* __restrict__ wG, double* __restrict__ diag,
-                              double* __restrict__ y, TS* __restrict__ Hdiag) {
+                              double* __restrict__ y, TS* __restrict__ Hm,
+                              TS* __restrict__ Gm) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
   double s = 0.0;
@@ -310,7 +438,17 @@ __global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode, size_t l
     }
     const double d = -s;
     diag[i] = d;
-    if (Hdiag) Hdiag[(size_t)i * ldh + i] = TS(d);
+    if (Hm) {  // the near / on-element entries of row i, and the diagonal
+      for (int pr = p0; pr < p1; ++pr) {
+        const int c0 = pair_elem[pr] * nn;
+        for (int k = 0; k < nn; ++k) {
+          const size_t at = (size_t)(c0 + k) * ldh + i;
+          Hm[at] = TS((c0 + k == i) ? d : pair_h[(size_t)pr * nn + k]);
+          Gm[at] = TS(pair_g[(size_t)pr * nn + k]);
+        }
+      }
+      Hm[(size_t)i * ldh + i] = TS(d);
+    }
   } else {
     for (int pr = p0; pr < p1; ++pr) {
       const int c0 = pair_elem[pr] * nn;
@@ -328,15 +466,6 @@ __global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode, size_t l
 // ---------------------------------------------------------------------------
 // Near-pair CSR from the bit matrix [word][row].
 // ---------------------------------------------------------------------------
-__global__ void near_count(int nt, int nwords, const uint32_t* __restrict__ bits,
-                           int* __restrict__ cnt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nt) return;
-  int c = 0;
-  for (int w = 0; w < nwords; ++w) c += __popc(bits[(size_t)w * nt + i]);
-  cnt[i] = c;
-}
-
 // Exclusive scan of cnt[0..nt) into ptr[0..nt] with one workgroup of 1024.
 __global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict__ cnt,
                                                   int* __restrict__ ptr) {
@@ -362,14 +491,20 @@ __global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict_
   if (tid == 1023) ptr[nt] = s[1023];
 }
 
-__global__ void near_fill(int nt, int nwords, const uint32_t* __restrict__ bits,
-                          const int* __restrict__ ptr, int* __restrict__ prow,
-                          int* __restrict__ pelem) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nt) return;
-  int off = ptr[i];
+// One thread per row, in the row blocks of near_classify: only the column
+// blocks that took the pair tests can hold near bits.
+__global__ __launch_bounds__(kFarThreads) void near_fill(int nt, int nwords,
+                                                         const uint32_t* __restrict__ bits,
+                                                         const uint8_t* __restrict__ tiles,
+                                                         const int* __restrict__ ptr,
+                                                         int* __restrict__ prow,
+                                                         int* __restrict__ pelem) {
+  const int i = blockIdx.x * kFarThreads + threadIdx.x;
+  const uint8_t* tl = tiles + (size_t)blockIdx.x * nwords;
+  int off = (i < nt) ? ptr[i] : 0;
   for (int w = 0; w < nwords; ++w) {
-    uint32_t b = bits[(size_t)w * nt + i];
+    if (!tl[w]) continue;  // uniform over the block
+    uint32_t b = (i < nt) ? bits[(size_t)w * nt + i] : 0u;
     while (b) {
       const int t = __ffs(b) - 1;
       b &= b - 1u;
@@ -687,6 +822,15 @@ __device__ void tanp3c_node(const TanPlan& P, int q, int npg, const double* u, c
 // ---------------------------------------------------------------------------
 // near_kernel: one wavefront per near pair.
 // ---------------------------------------------------------------------------
+// Sum over the 32 lanes of a half-wavefront group (xor offsets < 32 stay in
+// the group).
+constexpr int kGroup = 32;
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = kGroup / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
 template <typename TS, int NN>
 __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __restrict__ qc,
                                                    int64_t npairs, const int* __restrict__ prow,
@@ -694,9 +838,14 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
                                                    double* __restrict__ ph,
                                                    double* __restrict__ pg, TS* __restrict__ H,
                                                    TS* __restrict__ G) {
-  const int64_t pair = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (pair >= npairs) return;
+  // two pairs per wavefront: lanes 0-31 and 32-63 (kGroup lanes each)
+  const int64_t pair_raw =
+      ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * 2 +
+      ((threadIdx.x & (kWave - 1)) >> 5);
+  const bool active = pair_raw < npairs;
+  const int64_t pair = active ? pair_raw : npairs - 1;  // idle groups redo the last pair
+  const int lane = threadIdx.x & (kGroup - 1);
+  const int gshift = threadIdx.x & kGroup;  // 0 or 32: bit offset of this group in a ballot
   const int i = prow[pair];
   const int e = pelem[pair];
   const double px = g.x[i], py = g.y[i];
@@ -742,7 +891,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     double ha[NN], ga[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) { ha[k] = 0.0; ga[k] = 0.0; }
-    for (int q0 = 0; q0 < nq; q0 += kWave) {
+    for (int q0 = 0; q0 < nq; q0 += kGroup) {
       const int q = q0 + lane;
       if (q < nq) {
         double xq, wq;
@@ -773,7 +922,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
       }
     }
 #pragma unroll
-    for (int k = 0; k < NN; ++k) { hs[k] = wave_sum(ha[k]); gs[k] = wave_sum(ga[k]); }
+    for (int k = 0; k < NN; ++k) { hs[k] = group_sum(ha[k]); gs[k] = group_sum(ga[k]); }
   } else {
     // closest point on the collocation interpolant (src/Core/Integration.jl:323-349)
     double a;
@@ -829,8 +978,8 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     const bool lane_on = lane < kNinterp;
     const double xm = lane_on ? qc->g20_x[lane] : 0.0;
     const bool keep = lane_on && !(fabs(xm - a) <= 1e-12);
-    const uint64_t keep_mask = __ballot(keep);
-    const int m = __popcll(keep_mask);
+    const uint32_t keep_mask = (uint32_t)(__ballot(keep) >> gshift);
+    const int m = __popc(keep_mask);
     double bw = 0.0;
     if (keep) {
       if (m == kNinterp) {
@@ -847,7 +996,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     }
     const double dm = a - xm;
     const double bnum = keep ? bw / dm : 0.0;
-    const double msum = wave_sum(bnum);
+    const double msum = group_sum(bnum);
     const double Ni = keep ? bnum * (1.0 / msum) : 0.0;
     const double delta = xm - a;
     // kernel samples F = (U, T) * N_k * J  (src/Core/Assembly_full.jl:629-643)
@@ -883,9 +1032,9 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     // log-coefficient fit for G (order 0) and CPV coefficient for H (order -1)
     const bool used = keep && !(fabs(delta) < 1e-14);
     const double lg = used ? log(fabs(delta)) : 0.0;
-    const double sL = wave_sum(lg);
-    const double sL2 = wave_sum(lg * lg);
-    const int nused = __popcll(__ballot(used));
+    const double sL = group_sum(lg);
+    const double sL2 = group_sum(lg * lg);
+    const int nused = __popc((uint32_t)(__ballot(used) >> gshift));
     const double detA = sL2 * nused - sL * sL;
     const double wq = keep ? qc->g20_w[lane] : 0.0;
     const double hfp1 = log(fabs((1.0 - a) / (1.0 + a)));
@@ -893,29 +1042,31 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     const double lgd = keep ? log(fabs(delta)) : 0.0;
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
-      const double Fm1g = wave_sum(keep ? Ni * (Fg[k] * delta) : 0.0);
-      const double F1h = wave_sum(keep ? Ni * (Fh[k] * delta) : 0.0);
+      const double Fm1g = group_sum(keep ? Ni * (Fg[k] * delta) : 0.0);
+      const double F1h = group_sum(keep ? Ni * (Fh[k] * delta) : 0.0);
       const double wi = keep ? Fg[k] - Fm1g / delta : 0.0;
-      const double sW = wave_sum(used ? wi : 0.0);
-      const double sWL = wave_sum(used ? wi * lg : 0.0);
+      const double sW = group_sum(used ? wi : 0.0);
+      const double sWL = group_sum(used ? wi * lg : 0.0);
       double F0 = 0.0;
       if (nused >= 2 && !(fabs(detA) < 1e-30)) F0 = (nused * sWL - sL * sW) / detA;
       const double regg = keep ? wq * ((Fg[k] - Fm1g / delta) - F0 * lgd) : 0.0;
       const double regh = keep ? wq * (Fh[k] - F1h / delta) : 0.0;
-      gs[k] = wave_sum(regg) + (Fm1g * hfp1 + F0 * logp);
-      hs[k] = wave_sum(regh) + F1h * hfp1;
+      gs[k] = group_sum(regg) + (Fm1g * hfp1 + F0 * logp);
+      hs[k] = group_sum(regh) + F1h * hfp1;
     }
   }
-  if (lane < NN) {
+  if (active && lane < NN) {
     double hv = hs[0], gv = gs[0];
 #pragma unroll
     for (int k = 1; k < NN; ++k)
       if (lane == k) { hv = hs[k]; gv = gs[k]; }
     ph[(size_t)pair * NN + lane] = hv;
     pg[(size_t)pair * NN + lane] = gv;
-    const size_t col = (size_t)e * NN + lane;
-    H[i + col * g.ld] = TS(hv);
-    G[i + col * g.ld] = TS(gv);
+    if (H) {  // null: the values stay in the pair list (written to H, G later)
+      const size_t col = (size_t)e * NN + lane;
+      H[i + col * g.ld] = TS(hv);
+      G[i + col * g.ld] = TS(gv);
+    }
   }
 }
 
@@ -933,20 +1084,21 @@ __global__ void bc_swap_kernel(int nt, int n, size_t ld, const int* __restrict__
 }
 
 template <typename TS, int NN>
-int launch_far(bemhip_ctx* c, const Geo& g, const Consts& cs, TS* H, TS* G, double* hpart) {
+int launch_far(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Consts& cs, TS* H, TS* G,
+               double* hpart) {
   dim3 grid((g.nt + kFarThreads - 1) / kFarThreads, (g.ne + kEB - 1) / kEB);
-  hipLaunchKernelGGL((far_kernel<TS, NN>), grid, dim3(kFarThreads), 0, c->stream, g, cs, H,
-                     G, c->nearbits.ptr, hpart);
+  hipLaunchKernelGGL((far_kernel<TS, NN>), grid, dim3(kFarThreads), 0, stream, g, cs, H, G,
+                     hpart);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   return BEMHIP_OK;
 }
 
 template <typename TS, int NN>
-int launch_near(bemhip_ctx* c, const Geo& g, TS* H, TS* G) {
+int launch_near(bemhip_ctx* c, hipStream_t stream, const Geo& g, TS* H, TS* G) {
   if (c->npairs == 0) return BEMHIP_OK;
-  const int waves_per_block = 256 / kWave;
-  const int64_t blocks = (c->npairs + waves_per_block - 1) / waves_per_block;
-  hipLaunchKernelGGL((near_kernel<TS, NN>), dim3((unsigned)blocks), dim3(256), 0, c->stream, g,
+  const int pairs_per_block = 2 * (256 / kWave);  // two pairs per wavefront
+  const int64_t blocks = (c->npairs + pairs_per_block - 1) / pairs_per_block;
+  hipLaunchKernelGGL((near_kernel<TS, NN>), dim3((unsigned)blocks), dim3(256), 0, stream, g,
                      c->qc.ptr, c->npairs, c->pair_row.ptr, c->pair_elem.ptr, c->pair_h.ptr,
                      c->pair_g.ptr, H, G);
   BEMHIP_HIP_TRY(c, hipGetLastError());
@@ -973,7 +1125,13 @@ static Consts make_consts(const bemhip_ctx* c) {
   return cs;
 }
 
-// Full assembly: far tile kernel, near-pair CSR, near/singular kernel, diagonal.
+// Full assembly on two streams:
+//   stream 1: near_classify -> near-pair CSR -> near_kernel --+-> finalize
+//   stream 2:               \-> far_kernel -------------------/
+// The far field (HBM-write bound) and the near/singular integrals (FP64 VALU
+// bound) run concurrently. far_kernel writes 0 into the near entries, so
+// near_kernel keeps its values in the pair list and the finalize pass, after
+// both, writes them into H and G together with the rigid-body diagonal.
 int assemble_impl(bemhip_ctx* c) {
   const Geo g = make_geo(c);
   const Consts cs = make_consts(c);
@@ -1000,66 +1158,111 @@ int assemble_impl(bemhip_ctx* c) {
       BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + (size_t)c->n * c->ld32, 0,
                                        sizeof(float) * c->ld32 * c->ni, c->stream));
   }
-  int st;
+  // events: 0 start, 1 classified, 2 far done, 3 pairs start, 4 pairs done,
+  // 5 near done, 6 diagonal done, 7 far start
+  hipEvent_t ev[8];
+  for (auto& e : ev) BEMHIP_HIP_TRY(c, hipEventCreate(&e));
+  struct Destroy {
+    hipEvent_t* e;
+    ~Destroy() { for (int k = 0; k < 8; ++k) hipEventDestroy(e[k]); }
+  } destroy{ev};
+  const int tb = 256;
+  const int nb = (int)((nt + tb - 1) / tb);
+  hipStream_t s2 = c->stream2;
+
+  // stream 1: classification with the per-row counts, near-pair CSR, near
+  // integrals; the far field goes to stream 2 once the near work is queued
+  // (the near chain is the longer one, so it is dispatched first).
+  const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);  // row blocks
+  BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
+  BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
+  if (!c->h_count) BEMHIP_HIP_TRY(c, hipHostMalloc((void**)&c->h_count, sizeof(int)));
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
   {
-    ScopedTimer t(c, "assemble_far");
-    if (fp64)
-      st = (c->nn == 2) ? launch_far<double, 2>(c, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
-                        : launch_far<double, 3>(c, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
-    else
-      st = (c->nn == 2) ? launch_far<float, 2>(c, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr)
-                        : launch_far<float, 3>(c, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
-    if (st) return st;
-  }
-  {
-    ScopedTimer t(c, "assemble_pairs");
-    const int tb = 256;
-    const int nb = (int)((nt + tb - 1) / tb);
-    hipLaunchKernelGGL(near_count, dim3(nb), dim3(tb), 0, c->stream, (int)nt, ncb,
-                       c->nearbits.ptr, c->row_cnt.ptr);
-    hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
-                       c->row_ptr.ptr);
-    BEMHIP_HIP_TRY(c, hipGetLastError());
-    int total = 0;
-    BEMHIP_HIP_TRY(c, hipMemcpyAsync(&total, c->row_ptr.ptr + nt, sizeof(int),
-                                     hipMemcpyDeviceToHost, c->stream));
-    BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-    c->npairs = total;
-    BEMHIP_HIP_TRY(c, c->pair_row.ensure((size_t)std::max(total, 1)));
-    BEMHIP_HIP_TRY(c, c->pair_elem.ensure((size_t)std::max(total, 1)));
-    BEMHIP_HIP_TRY(c, c->pair_h.ensure((size_t)std::max(total, 1) * c->nn));
-    BEMHIP_HIP_TRY(c, c->pair_g.ensure((size_t)std::max(total, 1) * c->nn));
-    hipLaunchKernelGGL(near_fill, dim3(nb), dim3(tb), 0, c->stream, (int)nt, ncb,
-                       c->nearbits.ptr, c->row_ptr.ptr, c->pair_row.ptr, c->pair_elem.ptr);
-    BEMHIP_HIP_TRY(c, hipGetLastError());
-  }
-  {
-    ScopedTimer t(c, "assemble_near");
-    if (fp64)
-      st = (c->nn == 2) ? launch_near<double, 2>(c, g, c->H64.ptr, c->G64.ptr)
-                        : launch_near<double, 3>(c, g, c->H64.ptr, c->G64.ptr);
-    else
-      st = (c->nn == 2) ? launch_near<float, 2>(c, g, c->H32.ptr, c->G32.ptr)
-                        : launch_near<float, 3>(c, g, c->H32.ptr, c->G32.ptr);
-    if (st) return st;
-  }
-  {
-    ScopedTimer t(c, "assemble_diag");
-    const int tb = 256;
-    const int nb = (int)((nt + tb - 1) / tb);
-    if (fp64) {
-      hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt,
-                         c->n, c->nn, ncb, 0, c->ld64, c->hpart.ptr, c->row_ptr.ptr,
-                         c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr,
-                         c->diag64.ptr, nullptr, c->H64.ptr);
+    const int cpb = 32;  // column blocks per classify workgroup
+    dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)((ncb + cpb - 1) / cpb));
+    if (c->nn == 2) {
+      hipLaunchKernelGGL((elem_block_boxes<2>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+                         cs, ncb, c->blkbox.ptr);
+      hipLaunchKernelGGL((near_classify<2>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
+                         cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
     } else {
-      hipLaunchKernelGGL((op64_finalize<float>), dim3(nb), dim3(tb), 0, c->stream, (int)nt,
-                         c->n, c->nn, ncb, 0, c->ld32, c->hpart.ptr, c->row_ptr.ptr,
-                         c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr,
-                         c->diag64.ptr, nullptr, c->H32.ptr);
+      hipLaunchKernelGGL((elem_block_boxes<3>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+                         cs, ncb, c->blkbox.ptr);
+      hipLaunchKernelGGL((near_classify<3>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
+                         cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
     }
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[3], c->stream));
+  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
+                     c->row_ptr.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->h_count, c->row_ptr.ptr + nt, sizeof(int),
+                                   hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int total = *c->h_count;
+  c->npairs = total;
+  BEMHIP_HIP_TRY(c, c->pair_row.ensure((size_t)std::max(total, 1)));
+  BEMHIP_HIP_TRY(c, c->pair_elem.ensure((size_t)std::max(total, 1)));
+  BEMHIP_HIP_TRY(c, c->pair_h.ensure((size_t)std::max(total, 1) * c->nn));
+  BEMHIP_HIP_TRY(c, c->pair_g.ensure((size_t)std::max(total, 1) * c->nn));
+  hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, c->stream, (int)nt, ncb,
+                     c->nearbits.ptr, c->tiles.ptr, c->row_ptr.ptr, c->pair_row.ptr,
+                     c->pair_elem.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[4], c->stream));
+  // Near entries: far_kernel writes 0 into the near entries of H and G, so
+  // near_kernel keeps its values in the pair list and the finalize pass
+  // writes them into H and G after both.
+  int st;
+  if (fp64)
+    st = (c->nn == 2) ? launch_near<double, 2>(c, c->stream, g, (double*)nullptr, (double*)nullptr)
+                      : launch_near<double, 3>(c, c->stream, g, (double*)nullptr, (double*)nullptr);
+  else
+    st = (c->nn == 2) ? launch_near<float, 2>(c, c->stream, g, (float*)nullptr, (float*)nullptr)
+                      : launch_near<float, 3>(c, c->stream, g, (float*)nullptr, (float*)nullptr);
+  if (st) return st;
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[5], c->stream));
+  // stream 2: the far field (needs only the geometry; starts after the classify
+  // so that the phases can be timed apart)
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
+  if (fp64)
+    st = (c->nn == 2) ? launch_far<double, 2>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
+                      : launch_far<double, 3>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+  else
+    st = (c->nn == 2) ? launch_far<float, 2>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr)
+                      : launch_far<float, 3>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
+  if (st) return st;
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], s2));
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(c->stream, ev[2], 0));
+  // join (above): near values into H, G and the rigid-body diagonal
+  if (fp64)
+    hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
+                       c->nn, ncb, 0, c->ld64, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr, nullptr,
+                       c->H64.ptr, c->G64.ptr);
+  else
+    hipLaunchKernelGGL((op64_finalize<float>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
+                       c->nn, ncb, 0, c->ld32, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr, nullptr,
+                       c->H32.ptr, c->G32.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[6], c->stream));
+  BEMHIP_HIP_TRY(c, hipEventSynchronize(ev[6]));
+  auto el = [&](int a, int b) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, ev[a], ev[b]);
+    return (double)t;
+  };
+  c->ms["assemble_classify"] = el(0, 1);
+  c->ms["assemble_far"] = el(7, 2);
+  c->ms["assemble_pairs"] = el(3, 4);
+  c->ms["assemble_near"] = el(4, 5);
+  c->ms["assemble_diag"] = el(5, 6);
   return BEMHIP_OK;
 }
 
@@ -1105,7 +1308,7 @@ static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, doub
   hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
                      c->nn, ncb, 1, (size_t)0, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                      c->pair_h.ptr, c->pair_g.ptr, wH, wG, c->diag64.ptr, d_y,
-                     (double*)nullptr);
+                     (double*)nullptr, (double*)nullptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   return BEMHIP_OK;
 }

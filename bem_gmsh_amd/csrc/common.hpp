@@ -65,6 +65,7 @@ struct DevBuf {
 struct bemhip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // second stream (default priority) for concurrent phases
   std::string err;
   int num_cu = 0;
 
@@ -89,6 +90,9 @@ struct bemhip_ctx {
   bemhip::DevBuf<double> pair_h, pair_g;  // [npairs][nn]
   int64_t npairs = 0;
   bemhip::DevBuf<double> hpart;   // [ncb][nt] far row sums of H
+  bemhip::DevBuf<double> blkbox;  // [ncb][5] node bounding box and cutoff of each column block
+  bemhip::DevBuf<uint8_t> tiles;  // [row block][ncb] 1 where the pair tests ran
+  int* h_count = nullptr;          // pinned host word: near-pair count
   bemhip::DevBuf<double> diag64;  // [nt] rigid-body diagonal
   bemhip::DevBuf<char> scan_tmp;
 
@@ -121,6 +125,9 @@ struct bemhip_ctx {
   const void* dinv_of = nullptr;      // factor the inverses belong to
   int dinv_n = 0;
   bemhip::DevBuf<double> vec64a, vec64b, vec64c;
+  bemhip::DevBuf<unsigned> solve_sync;  // block-chain solve flags, tickets, abort word
+  unsigned solve_epoch = 0;
+  unsigned* h_flag = nullptr;           // pinned host word: solve abort flag
   bemhip::DevBuf<float> vec32a;
   bemhip::DevBuf<int> info;
 

@@ -17,7 +17,7 @@
 //                  v_mfma_f32_16x16x4f32), 128x128 block tiles, LDS double buffer.
 // Solves: row permutation as a gather, then blocked forward / backward
 // substitution with precomputed inverses of the 128x128 diagonal blocks and
-// FP64 accumulation (one launch per block). Transposed solves (the adjoint)
+// FP64 accumulation, one launch per triangle (row blocks chained by flags). Transposed solves (the adjoint)
 // use the dot-product form over the factor's contiguous columns.
 
 #include "common.hpp"
@@ -844,33 +844,99 @@ __global__ __launch_bounds__(kSolveNB) void tri_inv_kernel(int n, const T* __res
   for (int i = 0; i < nb; ++i) D[i + (size_t)c * kSolveNB] = (c < nb) ? X[i * kSolveNB + c] : T(0);
 }
 
-// One block step of a triangular solve: z = Dinv_b y_b (written to x_b), then
-// y_r -= sum_k F[r, s0+k] z_k for rows after (LOWER) or before (upper) the block.
+// Block triangular solve in one launch: forward with the unit-lower L
+// (LOWER) or backward with the upper U, FP64 accumulation over the factor T.
+// Workgroups take row blocks in dependency order from a ticket counter; a
+// workgroup only waits on blocks whose tickets were taken before its own by
+// workgroups already running, so the chain always progresses whatever the
+// residency. Row block rb: acc = y_rb - sum over the solved column blocks q of
+// F[rb, q] x_q, taking x_q as soon as its flag carries this launch's epoch;
+// then x_rb = Dinv_rb acc is published: the values as agent-scope stores,
+// completed, then the flag (agent-scope, relaxed loads on the reading side).
+// The 128x128 tile of F for the next column block is loaded before its flag
+// is awaited and Dinv_rb waits in LDS. Waits are bounded (abort word).
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void trsv_block_kernel(int n, int s0, int nb,
-                                                         const T* __restrict__ F, int lda,
-                                                         const T* __restrict__ D,
-                                                         double* __restrict__ y,
-                                                         double* __restrict__ x) {
-  __shared__ double ys[kSolveNB];
-  __shared__ double z[kSolveNB];
-  const int tid = threadIdx.x;
-  if (tid < nb) ys[tid] = y[s0 + tid];
-  __syncthreads();
-  if (tid < nb) {
-    double acc = 0.0;
-    for (int k = 0; k < nb; ++k) acc += (double)D[tid + (size_t)k * kSolveNB] * ys[k];
-    z[tid] = acc;
-    if (blockIdx.x == 0) x[s0 + tid] = acc;
+__global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restrict__ F, int lda,
+                                                         const T* __restrict__ Dinv,
+                                                         const double* __restrict__ y,
+                                                         unsigned long long* __restrict__ xw,
+                                                         unsigned* __restrict__ flags,
+                                                         unsigned* __restrict__ ticket,
+                                                         unsigned epoch,
+                                                         unsigned* __restrict__ abort_flag) {
+  constexpr int NB = kSolveNB;
+  constexpr int HB = NB / 2;  // columns per half-workgroup
+  __shared__ T sD[NB][NB];    // sD[k][i] = Dinv_rb(i, k)
+  __shared__ double s_x[NB], s_v[NB], s_z[2][NB];
+  __shared__ int s_blk, s_abort;
+  const int tid = threadIdx.x, r = tid & (NB - 1), h = tid >> 7;
+  const int nblk = (n + NB - 1) / NB;
+  if (tid == 0) {
+    s_blk = (int)atomicAdd(ticket, 1u);
+    s_abort = 0;
   }
   __syncthreads();
-  const int r = (LOWER ? s0 + nb : 0) + blockIdx.x * 256 + tid;
-  const int rend = LOWER ? n : s0;
-  if (r >= rend) return;
-  double acc = 0.0;
-  const T* Fr = F + (size_t)r + (size_t)s0 * lda;
-  for (int k = 0; k < nb; ++k) acc += (double)Fr[(size_t)k * lda] * z[k];
-  y[r] -= acc;
+  const int t = s_blk;
+  if (t >= nblk) return;
+  const int rb = LOWER ? t : nblk - 1 - t;
+  const int s0 = rb * NB, nb = min(NB, n - s0);
+  const int row = s0 + r;
+  const bool rok = r < nb;
+  {
+    const T* Db = Dinv + (size_t)rb * NB * NB;
+    for (int idx = tid; idx < NB * NB; idx += 256) sD[idx / NB][idx % NB] = Db[idx];
+  }
+  double acc = (h == 0 && rok) ? y[row] : 0.0;
+  T tv[HB];
+  auto load_tile = [&](int q) {
+    const int c0 = (LOWER ? q : nblk - 1 - q) * NB;
+    const int cn = min(NB, n - c0);
+    const T* Fr = F + (size_t)row + (size_t)(c0 + h * HB) * lda;
+#pragma unroll
+    for (int jj = 0; jj < HB; ++jj)
+      tv[jj] = (rok && h * HB + jj < cn) ? Fr[(size_t)jj * lda] : T(0);
+  };
+  if (t > 0) load_tile(0);
+  for (int q = 0; q < t; ++q) {
+    const int cbk = LOWER ? q : nblk - 1 - q;
+    if (tid == 0) {
+      unsigned spin = 0;
+      while (ld_ag32(&flags[cbk]) != epoch) {
+        if (++spin > kSpinLimit || ld_ag32(abort_flag)) {
+          atomicExch(abort_flag, 1u);
+          s_abort = 1;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_abort) return;
+    if (tid < NB) {
+      const int j = cbk * NB + tid;
+      s_x[tid] = (j < n) ? __longlong_as_double((long long)ld_ag(&xw[j])) : 0.0;
+    }
+    __syncthreads();
+    double part = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < HB; ++jj) part += (double)tv[jj] * s_x[h * HB + jj];
+    acc -= part;
+    if (q + 1 < t) load_tile(q + 1);
+    __syncthreads();
+  }
+  s_z[h][r] = acc;
+  __syncthreads();
+  if (tid < NB) s_v[tid] = (tid < nb) ? s_z[0][tid] + s_z[1][tid] : 0.0;
+  __syncthreads();
+  double zp = 0.0;
+  for (int k = h * HB; k < h * HB + HB; ++k) zp += (double)sD[k][r] * s_v[k];
+  __syncthreads();
+  s_z[h][r] = zp;
+  __syncthreads();
+  if (h == 0 && rok)
+    st_ag(&xw[row], (unsigned long long)__double_as_longlong(s_z[0][r] + s_z[1][r]));
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&flags[rb], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Transposed solves A^T x = b with A = P^T L U: U^T w = b (forward), L^T z = w
@@ -1180,26 +1246,38 @@ int lu_solve_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dperm, do
   const size_t blk_elems = (size_t)kSolveNB * kSolveNB;
   const T* DL = (const T*)c->dinv.ptr;
   const T* DU = DL + nblk * blk_elems;
-  BEMHIP_HIP_TRY(c, c->vec64c.ensure(2 * (size_t)n));
+  BEMHIP_HIP_TRY(c, c->vec64c.ensure(3 * (size_t)n));
   double* y = c->vec64c.ptr;
-  double* x = c->vec64c.ptr + n;
+  unsigned long long* w = (unsigned long long*)(c->vec64c.ptr + n);
+  unsigned long long* x = (unsigned long long*)(c->vec64c.ptr + 2 * (size_t)n);
+  // flags [2][nblk], tickets [2], abort word; zero once at allocation
+  const size_t nsync = 2 * (size_t)nblk + 4;
+  if (c->solve_sync.count < nsync) {
+    BEMHIP_HIP_TRY(c, c->solve_sync.ensure(nsync));
+    BEMHIP_HIP_TRY(c, hipMemsetAsync(c->solve_sync.ptr, 0, sizeof(unsigned) * nsync, c->stream));
+  }
+  unsigned* flags = c->solve_sync.ptr;
+  unsigned* tickets = flags + 2 * (size_t)nblk;
+  unsigned* abort_flag = tickets + 2;
+  if (!c->h_flag) BEMHIP_HIP_TRY(c, hipHostMalloc((void**)&c->h_flag, sizeof(unsigned)));
+  const unsigned epoch = ++c->solve_epoch;
+  if (epoch == 0) return fail(c, BEMHIP_ERR_STATE, "lu_solve: epoch counter wrapped");
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(tickets, 0, sizeof(unsigned) * 3, c->stream));  // + abort
   const int tb = 256;
   hipLaunchKernelGGL(gather_kernel, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n, dperm,
                      d_b, y);
-  for (int bi = 0; bi < nblk; ++bi) {  // forward: L x = y
-    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
-    const int rows = n - s0 - nb;
-    hipLaunchKernelGGL((trsv_block_kernel<T, true>), dim3(std::max(1, (rows + 255) / 256)),
-                       dim3(256), 0, c->stream, n, s0, nb, dLU, lda, DL + bi * blk_elems, y, x);
-  }
-  for (int bi = nblk - 1; bi >= 0; --bi) {  // backward: U y = x
-    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
-    hipLaunchKernelGGL((trsv_block_kernel<T, false>), dim3(std::max(1, (s0 + 255) / 256)),
-                       dim3(256), 0, c->stream, n, s0, nb, dLU, lda, DU + bi * blk_elems, x, y);
-  }
+  hipLaunchKernelGGL((trsv_chain_kernel<T, true>), dim3(nblk), dim3(256), 0, c->stream, n, dLU,
+                     lda, DL, (const double*)y, w, flags, tickets, epoch, abort_flag);
+  hipLaunchKernelGGL((trsv_chain_kernel<T, false>), dim3(nblk), dim3(256), 0, c->stream, n, dLU,
+                     lda, DU, (const double*)w, x, flags + nblk, tickets + 1, epoch, abort_flag);
   BEMHIP_HIP_TRY(c, hipGetLastError());
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(d_b, y, sizeof(double) * n, hipMemcpyDeviceToDevice,
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(d_b, x, sizeof(double) * n, hipMemcpyDeviceToDevice,
                                    c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->h_flag, abort_flag, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                   c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (*c->h_flag)
+    return fail(c, BEMHIP_ERR_HIP, "lu_solve: block-chain wait ran out (solve abandoned)");
   return BEMHIP_OK;
 }
 
