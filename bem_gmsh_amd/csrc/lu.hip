@@ -643,10 +643,13 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, const
 // mapped to tile xcd_chunk(b) so each XCD walks a contiguous range of
 // column-major tiles (shared B columns and neighbouring A rows in its L2).
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int NWM, int NWN>
+// SET = false: C -= A B. SET = true: C = A B, where B may be C itself when
+// one tile row covers all M rows (each workgroup reads its whole column strip
+// of B before it writes the same strip of C).
+template <typename T, int BM, int BN, int NWM, int NWN, bool SET = false>
 __global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
-    int M, int N, int K, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
-    T* __restrict__ C, int ldc, int tiles_m, int ntiles) {
+    int M, int N, int K, const T* A, int lda, const T* B, int ldb, T* C, int ldc, int tiles_m,
+    int ntiles) {
   using acc_t = typename Mfma<T>::acc_t;
   constexpr int NT = NWM * NWN * 64;
   constexpr int BK = 16;
@@ -766,7 +769,19 @@ __global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
   // C -= acc: the product is summed on its own and subtracted with one
   // rounding (as the reference GEMM update does), one MFMA row block of C at a
   // time with the next block's loads in flight.
-  if (interior) {
+  if constexpr (SET) {
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+      const int i = bm + wm + ti * 16 + l16;
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = bn + wn + lane_col + cofs(tj, r);
+          if (interior || (i < M && j < N)) Cl[ti * 16 + (size_t)cofs(tj, r) * ldc] = acc[ti][tj][r];
+        }
+    }
+  } else if (interior) {
     T cv[2][TJ][4];
 #pragma unroll
     for (int tj = 0; tj < TJ; ++tj)
@@ -815,33 +830,71 @@ __global__ void gather_kernel(int n, const int* __restrict__ perm, const double*
 }
 
 // Inverse of diagonal block b of L (unit lower) or U (upper): one workgroup of
-// kSolveNB threads per block; thread c builds column c in LDS. The factor's
-// entries are read at lane-uniform addresses.
+// kSolveNB threads per block; thread c builds column c of X = inverse in LDS,
+// row by row, with FP64 sums split over four accumulators. X is kept
+// transposed (Xt[c][k], rows padded by 16 bytes) so that a thread reads four
+// consecutive k in one 16-byte access; the block of the factor is read at
+// lane-uniform addresses, from an LDS copy for FP32 (row-major, four k per
+// 16-byte broadcast) and from memory for FP64 (Xt alone takes 130 KB).
+constexpr size_t tri_inv_lds(size_t elem) {
+  return (size_t)kSolveNB * (kSolveNB + 16 / elem) * elem +
+         (elem == 4 ? (size_t)kSolveNB * kSolveNB * elem : 0);
+}
+
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(kSolveNB) void tri_inv_kernel(int n, const T* __restrict__ F, int lda,
                                                           T* __restrict__ Dinv) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* X = reinterpret_cast<T*>(smem_raw);  // [kSolveNB][kSolveNB], X[i*NB + c]
+  constexpr int NB = kSolveNB;
+  constexpr int XP = NB + 16 / (int)sizeof(T);
+  constexpr bool kStage = sizeof(T) == 4;
+  T* Xt = reinterpret_cast<T*>(smem_raw);  // Xt[c*XP + k] = X(k, c)
+  T* Fs = Xt + NB * XP;                    // FP32: Fs[i*NB + k] = F(s0 + i, s0 + k)
   const int b = blockIdx.x;
-  const int s0 = b * kSolveNB;
-  const int nb = min(kSolveNB, n - s0);
+  const int s0 = b * NB;
+  const int nb = min(NB, n - s0);
   const int c = threadIdx.x;
   const T* Fb = F + (size_t)s0 + (size_t)s0 * lda;
-  if (LOWER) {
-    for (int i = 0; i < nb; ++i) {
-      double acc = (i == c) ? 1.0 : 0.0;
-      for (int k = 0; k < i; ++k) acc -= (double)Fb[i + (size_t)k * lda] * (double)X[k * kSolveNB + c];
-      X[i * kSolveNB + c] = (T)acc;
-    }
-  } else {
-    for (int i = nb - 1; i >= 0; --i) {
-      double acc = (i == c) ? 1.0 : 0.0;
-      for (int k = i + 1; k < nb; ++k) acc -= (double)Fb[i + (size_t)k * lda] * (double)X[k * kSolveNB + c];
-      X[i * kSolveNB + c] = (T)(acc / (double)Fb[i + (size_t)i * lda]);
-    }
+  if constexpr (kStage) {
+    for (int i = 0; i < NB; ++i) Fs[i * NB + c] = (i < nb && c < nb) ? Fb[i + (size_t)c * lda] : T(0);
+    __syncthreads();
   }
-  T* D = Dinv + (size_t)b * kSolveNB * kSolveNB;
-  for (int i = 0; i < nb; ++i) D[i + (size_t)c * kSolveNB] = (c < nb) ? X[i * kSolveNB + c] : T(0);
+  auto f = [&](int i, int k) -> double {
+    if constexpr (kStage) return (double)Fs[i * NB + k];
+    else return (double)Fb[i + (size_t)k * lda];
+  };
+  T* xc = Xt + c * XP;
+  // sum over k in [k0, k1) of F(i, k) X(k, c), four accumulators
+  auto dot = [&](int i, int k0, int k1) -> double {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = k0;
+    for (; k < k1 && (k & 3); ++k) a0 += f(i, k) * (double)xc[k];
+    for (; k + 4 <= k1; k += 4) {
+      double f0, f1, f2, f3;
+      if constexpr (kStage) {
+        const float4 fv = *reinterpret_cast<const float4*>(&Fs[i * NB + k]);
+        f0 = fv.x; f1 = fv.y; f2 = fv.z; f3 = fv.w;
+      } else {
+        f0 = f(i, k); f1 = f(i, k + 1); f2 = f(i, k + 2); f3 = f(i, k + 3);
+      }
+      a0 += f0 * (double)xc[k];
+      a1 += f1 * (double)xc[k + 1];
+      a2 += f2 * (double)xc[k + 2];
+      a3 += f3 * (double)xc[k + 3];
+    }
+    for (; k < k1; ++k) a0 += f(i, k) * (double)xc[k];
+    return (a0 + a1) + (a2 + a3);
+  };
+  if (LOWER) {
+    for (int i = 0; i < nb; ++i) xc[i] = (T)(((i == c) ? 1.0 : 0.0) - dot(i, 0, i));
+  } else {
+    for (int i = nb - 1; i >= 0; --i)
+      xc[i] = (T)((((i == c) ? 1.0 : 0.0) - dot(i, i + 1, nb)) / f(i, i));
+  }
+  __syncthreads();
+  T* D = Dinv + (size_t)b * NB * NB;
+  // D(i, c) = X(i, c): thread c writes rows i of column c from Xt[c][i]
+  for (int i = 0; i < nb; ++i) D[i + (size_t)c * NB] = (c < nb) ? xc[i] : T(0);
 }
 
 // Block triangular solve in one launch: forward with the unit-lower L
@@ -986,13 +1039,26 @@ __global__ void scatter_kernel(int n, const int* __restrict__ perm, const double
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int NWM, int NWN>
+template <typename T, int BM, int BN, int NWM, int NWN, bool SET = false>
 void launch_tile_gemm(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B,
                       int ldb, T* C, int ldc) {
   const int tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   const int nt = tm * tn;
-  hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, NWM, NWN>), dim3(nt), dim3(NWM * NWN * 64), 0,
-                     c->stream, M, N, K, A, lda, B, ldb, C, ldc, tm, nt);
+  hipLaunchKernelGGL((gemm_tile_kernel<T, BM, BN, NWM, NWN, SET>), dim3(nt),
+                     dim3(NWM * NWN * 64), 0, c->stream, M, N, K, A, lda, B, ldb, C, ldc, tm, nt);
+}
+
+bool gemm_aligned(const void* A, int lda, const void* B, int ldb, int vec) {
+  return ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && (lda % vec == 0) &&
+         (ldb % vec == 0);
+}
+
+// X = Linv X in place (M <= 128 rows of X, Linv M x M with leading dimension
+// kSolveNB), one 128-row tile row. Requires 16-byte aligned operands.
+template <typename T>
+void gemm_set_inplace(bemhip_ctx* c, int M, int N, const T* Linv, T* X, int ldx) {
+  if (M <= 0 || N <= 0) return;
+  launch_tile_gemm<T, 128, 128, 2, 2, true>(c, M, N, M, Linv, kSolveNB, X, ldx, X, ldx);
 }
 
 // C -= A B. 16-byte aligned operands (every LU call on the padded matrices)
@@ -1003,9 +1069,7 @@ int gemm_sub(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B
              int ldc) {
   if (M <= 0 || N <= 0 || K <= 0) return BEMHIP_OK;
   constexpr int VEC = 16 / (int)sizeof(T);
-  const bool aligned = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) &&
-                       (lda % VEC == 0) && (ldb % VEC == 0);
-  if (aligned) {
+  if (gemm_aligned(A, lda, B, ldb, VEC)) {
     const int cus = std::max(1, c->num_cu);
     const long big = (long)((M + 255) / 256) * (long)((N + 255) / 256);
     if constexpr (sizeof(T) == 4) {
@@ -1053,7 +1117,8 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   if (n >= (1 << 24)) return fail(c, BEMHIP_ERR_ARG, "lu: n must be below 2^24");
   // work buffers
   const size_t need = 8 * 2 * (size_t)Gmax + 8 * 2 * (size_t)Gmax * RW + 8 * 2 * (size_t)RW +
-                      sizeof(int) * (4 * kMaxPanel + 16) + 8 * 256;
+                      sizeof(int) * (4 * kMaxPanel + 16) + sizeof(T) * 2 * kSolveNB * kSolveNB +
+                      8 * 256;
   BEMHIP_HIP_TRY(c, c->lu_work.ensure(need));
   char* wp = c->lu_work.ptr;
   auto carve = [&](size_t bytes) {
@@ -1067,6 +1132,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   int* ldst = (int*)carve(sizeof(int) * 2 * kMaxPanel);
   int* lsrc = (int*)carve(sizeof(int) * 2 * kMaxPanel);
   int* llen = (int*)carve(sizeof(int) * 16);
+  T* linv = (T*)carve(sizeof(T) * 2 * kSolveNB * kSolveNB);  // inverses of L11's 128-blocks
   int* info = llen + 4;
   unsigned* abort_flag = (unsigned*)(llen + 8);
   if ((size_t)(wp - c->lu_work.ptr) > c->lu_work.count)
@@ -1078,6 +1144,11 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   if ((int64_t)occ * std::max(1, c->num_cu) < Gmax)
     return fail(c, BEMHIP_ERR_STATE, "lu: panel workgroups cannot all be resident");
   c->dinv_of = nullptr;
+  BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)tri_inv_kernel<T, true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)tri_inv_lds(sizeof(T))));
+  constexpr int VEC = 16 / (int)sizeof(T);
+  const bool aligned = gemm_aligned(linv, kSolveNB, dA, lda, VEC);
 
   // Two-level blocking (the structure of LAPACK getrf over getrf2): outer
   // blocks of OB = 2 KB columns, each factored as KB-wide register-resident
@@ -1085,6 +1156,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   // of the outer block updated right away; the trailing matrix is updated once
   // per outer block, a K = OB product.
   constexpr int OB = 256;
+  static_assert(OB == 2 * kSolveNB, "the outer TRSM works on two 128-blocks");
   enum { kPanel = 0, kSwap = 1, kTrsm = 2, kGemm = 3 };
   std::vector<hipEvent_t> ev;
   std::vector<int> cat;
@@ -1123,13 +1195,28 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
                            dA, lda, 0, kc, kc + kb, n, ldst, lsrc, llen);
       BEMHIP_HIP_TRY(c, hipGetLastError());
       mark(kSwap);
+      // inverse of each completed 128-block of this outer block's L11
+      if (aligned && ((s + kb) % kSolveNB == 0 || s + kb == ob)) {
+        const int jb = (s + kb - 1) / kSolveNB;
+        const int d0 = k0 + jb * kSolveNB;
+        hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(1), dim3(kSolveNB),
+                           tri_inv_lds(sizeof(T)), c->stream, std::min(kSolveNB, ob - jb * kSolveNB),
+                           (const T*)(dA + (size_t)d0 + (size_t)d0 * lda), lda,
+                           linv + (size_t)jb * kSolveNB * kSolveNB);
+        BEMHIP_HIP_TRY(c, hipGetLastError());
+      }
       // the rest of this outer block
       const int rest = k0 + ob - (kc + kb);
       if (rest > 0) {
         T* L11 = dA + (size_t)kc + (size_t)kc * lda;
         T* A12 = dA + (size_t)kc + (size_t)(kc + kb) * lda;
-        int st = trsm_rec(c, kb, rest, (const T*)L11, lda, A12, lda);
-        if (st) return st;
+        if (aligned && KB == kSolveNB) {  // the panel is one 128-block: U12 = L11^-1 A12
+          gemm_set_inplace(c, kb, rest, (const T*)(linv + (size_t)(s / kSolveNB) * kSolveNB *
+                                                              kSolveNB), A12, lda);
+        } else {
+          int st = trsm_rec(c, kb, rest, (const T*)L11, lda, A12, lda);
+          if (st) return st;
+        }
         mark(kTrsm);
         T* L21 = dA + (size_t)(kc + kb) + (size_t)kc * lda;
         T* A22 = dA + (size_t)(kc + kb) + (size_t)(kc + kb) * lda;
@@ -1142,8 +1229,21 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     if (nrest > 0) {
       T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
       T* A12 = dA + (size_t)k0 + (size_t)(k0 + ob) * lda;
-      int st = trsm_rec(c, ob, nrest, (const T*)L11, lda, A12, lda);
-      if (st) return st;
+      if (aligned) {
+        // U12 = L11^-1 A12 by 128-blocks: X1 = Linv1 X1; X2 -= L21 X1; X2 = Linv2 X2
+        const int b1 = std::min(kSolveNB, ob);
+        gemm_set_inplace(c, b1, nrest, (const T*)linv, A12, lda);
+        if (ob > kSolveNB) {
+          gemm_sub(c, ob - kSolveNB, nrest, kSolveNB, (const T*)(L11 + kSolveNB), lda,
+                   (const T*)A12, lda, A12 + kSolveNB, lda);
+          gemm_set_inplace(c, ob - kSolveNB, nrest,
+                           (const T*)(linv + (size_t)kSolveNB * kSolveNB), A12 + kSolveNB, lda);
+        }
+      } else {
+        int st = trsm_rec(c, ob, nrest, (const T*)L11, lda, A12, lda);
+        if (st) return st;
+      }
+      BEMHIP_HIP_TRY(c, hipGetLastError());
       mark(kTrsm);
       T* L21 = dA + (size_t)(k0 + ob) + (size_t)k0 * lda;
       T* A22 = dA + (size_t)(k0 + ob) + (size_t)(k0 + ob) * lda;
@@ -1187,7 +1287,7 @@ int ensure_dinv(bemhip_ctx* c, int n, const T* dLU, int lda) {
   T* DL = (T*)c->dinv.ptr;
   T* DU = DL + nblk * blk_elems;
   if (c->dinv_of != (const void*)dLU || c->dinv_n != n) {
-    const size_t lds = blk_elems * sizeof(T);
+    const size_t lds = tri_inv_lds(sizeof(T));
     BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)tri_inv_kernel<T, true>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)tri_inv_kernel<T, false>,
