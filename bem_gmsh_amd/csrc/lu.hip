@@ -25,6 +25,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <climits>
 #include <cmath>
 
@@ -688,84 +689,96 @@ __global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
 #pragma unroll
     for (int tj = 0; tj < TJ; ++tj) acc[ti][tj] = acc_t{0, 0, 0, 0};
 
-  vec_t ra[LA], rb[LB];
-  auto gload = [&](int kt) {
+  // Main loop, instantiated twice: CHK = false for tiles whose every A/B load is
+  // in bounds (no per-element tests, so nothing serializes the prefetch of the
+  // next K-step behind the current one), CHK = true for edge tiles.
+  auto mainloop = [&](auto chk) {
+    constexpr bool CHK = decltype(chk)::value;
+    vec_t ra[LA], rb[LB];
+    auto gload = [&](int kt) {
 #pragma unroll
-    for (int u = 0; u < LA; ++u) {
-      const int v = tid + u * NT;
-      const int m4 = (v % (BM / VEC)) * VEC, k = v / (BM / VEC);
-      const int r = bm + m4, kk = kt + k;
-      const T* src = A + (size_t)r + (size_t)kk * lda;
-      if (kk < K && r + VEC <= M) {
-        ra[u] = *reinterpret_cast<const vec_t*>(src);
-      } else {
+      for (int u = 0; u < LA; ++u) {
+        const int v = tid + u * NT;
+        const int m4 = (v % (BM / VEC)) * VEC, k = v / (BM / VEC);
+        const int r = bm + m4, kk = kt + k;
+        const T* src = A + (size_t)r + (size_t)kk * lda;
+        if (!CHK || (kk < K && r + VEC <= M)) {
+          ra[u] = *reinterpret_cast<const vec_t*>(src);
+        } else {
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) ra[u][e] = (kk < K && r + e < M) ? src[e] : T(0);
+          for (int e = 0; e < VEC; ++e) ra[u][e] = (kk < K && r + e < M) ? src[e] : T(0);
+        }
       }
-    }
 #pragma unroll
-    for (int u = 0; u < LB; ++u) {
-      const int v = tid + u * NT;
-      const int k4 = (v % (BK / VEC)) * VEC, n = v / (BK / VEC);
-      const int cb = bn + n, kk = kt + k4;
-      const T* src = B + (size_t)kk + (size_t)cb * ldb;
-      if (cb < N && kk + VEC <= K) {
-        rb[u] = *reinterpret_cast<const vec_t*>(src);
-      } else {
+      for (int u = 0; u < LB; ++u) {
+        const int v = tid + u * NT;
+        const int k4 = (v % (BK / VEC)) * VEC, n = v / (BK / VEC);
+        const int cb = bn + n, kk = kt + k4;
+        const T* src = B + (size_t)kk + (size_t)cb * ldb;
+        if (!CHK || (cb < N && kk + VEC <= K)) {
+          rb[u] = *reinterpret_cast<const vec_t*>(src);
+        } else {
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) rb[u][e] = (cb < N && kk + e < K) ? src[e] : T(0);
+          for (int e = 0; e < VEC; ++e) rb[u][e] = (cb < N && kk + e < K) ? src[e] : T(0);
+        }
       }
-    }
-  };
-  auto sstore = [&](int buf) {
+    };
+    auto sstore = [&](int buf) {
 #pragma unroll
-    for (int u = 0; u < LA; ++u) {
-      const int v = tid + u * NT;
-      const int m4 = (v % (BM / VEC)) * VEC, k = v / (BM / VEC);
-      *reinterpret_cast<vec_t*>(&As[buf][k][m4]) = ra[u];
-    }
+      for (int u = 0; u < LA; ++u) {
+        const int v = tid + u * NT;
+        const int m4 = (v % (BM / VEC)) * VEC, k = v / (BM / VEC);
+        *reinterpret_cast<vec_t*>(&As[buf][k][m4]) = ra[u];
+      }
 #pragma unroll
-    for (int u = 0; u < LB; ++u) {
-      const int v = tid + u * NT;
-      const int k4 = (v % (BK / VEC)) * VEC, n = v / (BK / VEC);
-      *reinterpret_cast<vec_t*>(&Bs[buf][n][k4]) = rb[u];
-    }
-  };
+      for (int u = 0; u < LB; ++u) {
+        const int v = tid + u * NT;
+        const int k4 = (v % (BK / VEC)) * VEC, n = v / (BK / VEC);
+        *reinterpret_cast<vec_t*>(&Bs[buf][n][k4]) = rb[u];
+      }
+    };
 
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  int buf = 0;
-  for (int kt = 0; kt < K; kt += BK) {
-    const bool more = kt + BK < K;
-    if (more) gload(kt + BK);
-    T fb[TJ][4];
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int kt = 0; kt < K; kt += BK) {
+      const bool more = kt + BK < K;
+      if (more) gload(kt + BK);
+      T fb[TJ][4];
 #pragma unroll
-    for (int tj = 0; tj < TJ; ++tj) {
-      const T* p = &Bs[buf][wn + tj * 16 + l16][q4 * 4];
+      for (int tj = 0; tj < TJ; ++tj) {
+        const T* p = &Bs[buf][wn + tj * 16 + l16][q4 * 4];
 #pragma unroll
-      for (int h = 0; h < 4; h += VEC) {
-        const vec_t x = *reinterpret_cast<const vec_t*>(p + h);
+        for (int h = 0; h < 4; h += VEC) {
+          const vec_t x = *reinterpret_cast<const vec_t*>(p + h);
 #pragma unroll
-        for (int e = 0; e < VEC; ++e) fb[tj][h + e] = x[e];
+          for (int e = 0; e < VEC; ++e) fb[tj][h + e] = x[e];
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        T fa[TI];
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti) fa[ti] = As[buf][q4 * 4 + s][wm + ti * 16 + l16];
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+          for (int tj = 0; tj < TJ; ++tj)
+            acc[ti][tj] = Mfma<T>::op(fb[tj][s], fa[ti], acc[ti][tj]);
+      }
+      if (more) {
+        sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
       }
     }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      T fa[TI];
-#pragma unroll
-      for (int ti = 0; ti < TI; ++ti) fa[ti] = As[buf][q4 * 4 + s][wm + ti * 16 + l16];
-#pragma unroll
-      for (int ti = 0; ti < TI; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < TJ; ++tj) acc[ti][tj] = Mfma<T>::op(fb[tj][s], fa[ti], acc[ti][tj]);
-    }
-    if (more) {
-      sstore(buf ^ 1);
-      __syncthreads();
-      buf ^= 1;
-    }
-  }
+  };
+  if (interior && (K % BK) == 0)
+    mainloop(std::false_type{});
+  else
+    mainloop(std::true_type{});
+
   // C -= acc: the product is summed on its own and subtracted with one
   // rounding (as the reference GEMM update does), one MFMA row block of C at a
   // time with the next block's loads in flight.
@@ -830,20 +843,23 @@ __global__ void gather_kernel(int n, const int* __restrict__ perm, const double*
 }
 
 // Inverse of diagonal block b of L (unit lower) or U (upper): one workgroup of
-// kSolveNB threads per block; thread c builds column c of X = inverse in LDS,
-// row by row, with FP64 sums split over four accumulators. X is kept
-// transposed (Xt[c][k], rows padded by 16 bytes) so that a thread reads four
-// consecutive k in one 16-byte access; the block of the factor is read at
-// lane-uniform addresses, from an LDS copy for FP32 (row-major, four k per
-// 16-byte broadcast) and from memory for FP64 (Xt alone takes 130 KB).
+// 4 x kSolveNB threads per block. Column c of X = inverse is built row by row
+// by the four lanes of a quad (c = thread / 4): lane p sums the terms k = p
+// mod 4 in FP64 over two accumulators, the quad adds its four partial sums by
+// lane shuffles, and lane 0 stores the entry in LDS (a wavefront's LDS
+// operations complete in order, so the quad's next row sees it). X is kept
+// transposed (Xt[c][k], rows padded by 16 bytes); the block of the factor is
+// read from an LDS copy for FP32 (row-major) and from memory for FP64 (Xt
+// alone takes 130 KB).
+constexpr int kTriInvThreads = 4 * kSolveNB;
 constexpr size_t tri_inv_lds(size_t elem) {
   return (size_t)kSolveNB * (kSolveNB + 16 / elem) * elem +
          (elem == 4 ? (size_t)kSolveNB * kSolveNB * elem : 0);
 }
 
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(kSolveNB) void tri_inv_kernel(int n, const T* __restrict__ F, int lda,
-                                                          T* __restrict__ Dinv) {
+__global__ __launch_bounds__(kTriInvThreads) void tri_inv_kernel(int n, const T* __restrict__ F,
+                                                                int lda, T* __restrict__ Dinv) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int NB = kSolveNB;
   constexpr int XP = NB + 16 / (int)sizeof(T);
@@ -853,10 +869,14 @@ __global__ __launch_bounds__(kSolveNB) void tri_inv_kernel(int n, const T* __res
   const int b = blockIdx.x;
   const int s0 = b * NB;
   const int nb = min(NB, n - s0);
-  const int c = threadIdx.x;
+  const int tid = threadIdx.x;
+  const int c = tid >> 2, part = tid & 3;
   const T* Fb = F + (size_t)s0 + (size_t)s0 * lda;
   if constexpr (kStage) {
-    for (int i = 0; i < NB; ++i) Fs[i * NB + c] = (i < nb && c < nb) ? Fb[i + (size_t)c * lda] : T(0);
+    for (int idx = tid; idx < NB * NB; idx += kTriInvThreads) {
+      const int i = idx & (NB - 1), k = idx / NB;  // coalesced over i of column k
+      Fs[i * NB + k] = (i < nb && k < nb) ? Fb[i + (size_t)k * lda] : T(0);
+    }
     __syncthreads();
   }
   auto f = [&](int i, int k) -> double {
@@ -864,37 +884,38 @@ __global__ __launch_bounds__(kSolveNB) void tri_inv_kernel(int n, const T* __res
     else return (double)Fb[i + (size_t)k * lda];
   };
   T* xc = Xt + c * XP;
-  // sum over k in [k0, k1) of F(i, k) X(k, c), four accumulators
+  // this lane's share of sum_{k0 <= k < k1} F(i, k) X(k, c): k = part mod 4
   auto dot = [&](int i, int k0, int k1) -> double {
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int k = k0;
-    for (; k < k1 && (k & 3); ++k) a0 += f(i, k) * (double)xc[k];
-    for (; k + 4 <= k1; k += 4) {
-      double f0, f1, f2, f3;
-      if constexpr (kStage) {
-        const float4 fv = *reinterpret_cast<const float4*>(&Fs[i * NB + k]);
-        f0 = fv.x; f1 = fv.y; f2 = fv.z; f3 = fv.w;
-      } else {
-        f0 = f(i, k); f1 = f(i, k + 1); f2 = f(i, k + 2); f3 = f(i, k + 3);
-      }
-      a0 += f0 * (double)xc[k];
-      a1 += f1 * (double)xc[k + 1];
-      a2 += f2 * (double)xc[k + 2];
-      a3 += f3 * (double)xc[k + 3];
+    double a0 = 0.0, a1 = 0.0;
+    int k = k0 + ((part - k0) & 3);
+    for (; k + 4 < k1; k += 8) {
+      a0 += f(i, k) * (double)xc[k];
+      a1 += f(i, k + 4) * (double)xc[k + 4];
     }
-    for (; k < k1; ++k) a0 += f(i, k) * (double)xc[k];
-    return (a0 + a1) + (a2 + a3);
+    if (k < k1) a0 += f(i, k) * (double)xc[k];
+    double sum = a0 + a1;
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    return sum;
   };
   if (LOWER) {
-    for (int i = 0; i < nb; ++i) xc[i] = (T)(((i == c) ? 1.0 : 0.0) - dot(i, 0, i));
+    for (int i = 0; i < nb; ++i) {
+      const double x = ((i == c) ? 1.0 : 0.0) - dot(i, 0, i);
+      if (part == 0) xc[i] = (T)x;
+    }
   } else {
-    for (int i = nb - 1; i >= 0; --i)
-      xc[i] = (T)((((i == c) ? 1.0 : 0.0) - dot(i, i + 1, nb)) / f(i, i));
+    for (int i = nb - 1; i >= 0; --i) {
+      const double x = (((i == c) ? 1.0 : 0.0) - dot(i, i + 1, nb)) / f(i, i);
+      if (part == 0) xc[i] = (T)x;
+    }
   }
   __syncthreads();
   T* D = Dinv + (size_t)b * NB * NB;
-  // D(i, c) = X(i, c): thread c writes rows i of column c from Xt[c][i]
-  for (int i = 0; i < nb; ++i) D[i + (size_t)c * NB] = (c < nb) ? xc[i] : T(0);
+  // D(i, c) = X(i, c), coalesced over i
+  for (int idx = tid; idx < NB * NB; idx += kTriInvThreads) {
+    const int i = idx & (NB - 1), cc = idx / NB;
+    if (i < nb) D[i + (size_t)cc * NB] = (cc < nb) ? Xt[cc * XP + i] : T(0);
+  }
 }
 
 // Block triangular solve in one launch: forward with the unit-lower L
@@ -1199,7 +1220,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       if (aligned && ((s + kb) % kSolveNB == 0 || s + kb == ob)) {
         const int jb = (s + kb - 1) / kSolveNB;
         const int d0 = k0 + jb * kSolveNB;
-        hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(1), dim3(kSolveNB),
+        hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(1), dim3(kTriInvThreads),
                            tri_inv_lds(sizeof(T)), c->stream, std::min(kSolveNB, ob - jb * kSolveNB),
                            (const T*)(dA + (size_t)d0 + (size_t)d0 * lda), lda,
                            linv + (size_t)jb * kSolveNB * kSolveNB);
@@ -1292,9 +1313,9 @@ int ensure_dinv(bemhip_ctx* c, int n, const T* dLU, int lda) {
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)tri_inv_kernel<T, false>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(nblk), dim3(kSolveNB), lds, c->stream, n,
+    hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(nblk), dim3(kTriInvThreads), lds, c->stream, n,
                        dLU, lda, DL);
-    hipLaunchKernelGGL((tri_inv_kernel<T, false>), dim3(nblk), dim3(kSolveNB), lds, c->stream, n,
+    hipLaunchKernelGGL((tri_inv_kernel<T, false>), dim3(nblk), dim3(kTriInvThreads), lds, c->stream, n,
                        dLU, lda, DU);
     BEMHIP_HIP_TRY(c, hipGetLastError());
     c->dinv_of = dLU;
