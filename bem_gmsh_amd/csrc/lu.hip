@@ -835,6 +835,8 @@ __global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
 // with the inverses of the diagonal blocks precomputed once per factorization.
 // ---------------------------------------------------------------------------
 constexpr int kSolveNB = 128;
+// Outer block of the LU (K of the trailing update) in 128-column blocks.
+constexpr int kOuterBlocks = 4;
 
 __global__ void gather_kernel(int n, const int* __restrict__ perm, const double* __restrict__ b,
                               double* __restrict__ y) {
@@ -1138,7 +1140,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   if (n >= (1 << 24)) return fail(c, BEMHIP_ERR_ARG, "lu: n must be below 2^24");
   // work buffers
   const size_t need = 8 * 2 * (size_t)Gmax + 8 * 2 * (size_t)Gmax * RW + 8 * 2 * (size_t)RW +
-                      sizeof(int) * (4 * kMaxPanel + 16) + sizeof(T) * 2 * kSolveNB * kSolveNB +
+                      sizeof(int) * (4 * kMaxPanel + 16) + sizeof(T) * kOuterBlocks * kSolveNB * kSolveNB +
                       8 * 256;
   BEMHIP_HIP_TRY(c, c->lu_work.ensure(need));
   char* wp = c->lu_work.ptr;
@@ -1153,7 +1155,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   int* ldst = (int*)carve(sizeof(int) * 2 * kMaxPanel);
   int* lsrc = (int*)carve(sizeof(int) * 2 * kMaxPanel);
   int* llen = (int*)carve(sizeof(int) * 16);
-  T* linv = (T*)carve(sizeof(T) * 2 * kSolveNB * kSolveNB);  // inverses of L11's 128-blocks
+  T* linv = (T*)carve(sizeof(T) * kOuterBlocks * kSolveNB * kSolveNB);  // L11's 128-block inverses
   int* info = llen + 4;
   unsigned* abort_flag = (unsigned*)(llen + 8);
   if ((size_t)(wp - c->lu_work.ptr) > c->lu_work.count)
@@ -1176,8 +1178,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   // panels with their interchanges applied to every other column and the rest
   // of the outer block updated right away; the trailing matrix is updated once
   // per outer block, a K = OB product.
-  constexpr int OB = 256;
-  static_assert(OB == 2 * kSolveNB, "the outer TRSM works on two 128-blocks");
+  constexpr int OB = kOuterBlocks * kSolveNB;
   enum { kPanel = 0, kSwap = 1, kTrsm = 2, kGemm = 3 };
   std::vector<hipEvent_t> ev;
   std::vector<int> cat;
@@ -1251,14 +1252,15 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
       T* A12 = dA + (size_t)k0 + (size_t)(k0 + ob) * lda;
       if (aligned) {
-        // U12 = L11^-1 A12 by 128-blocks: X1 = Linv1 X1; X2 -= L21 X1; X2 = Linv2 X2
-        const int b1 = std::min(kSolveNB, ob);
-        gemm_set_inplace(c, b1, nrest, (const T*)linv, A12, lda);
-        if (ob > kSolveNB) {
-          gemm_sub(c, ob - kSolveNB, nrest, kSolveNB, (const T*)(L11 + kSolveNB), lda,
-                   (const T*)A12, lda, A12 + kSolveNB, lda);
-          gemm_set_inplace(c, ob - kSolveNB, nrest,
-                           (const T*)(linv + (size_t)kSolveNB * kSolveNB), A12 + kSolveNB, lda);
+        // U12 = L11^-1 A12 by 128-blocks b: X_b -= L[b, <b] X_<b, then X_b = Linv_b X_b
+        for (int b0 = 0; b0 < ob; b0 += kSolveNB) {
+          const int bw = std::min(kSolveNB, ob - b0);
+          if (b0 > 0)
+            gemm_sub(c, bw, nrest, b0, (const T*)(L11 + b0), lda, (const T*)A12, lda, A12 + b0,
+                     lda);
+          gemm_set_inplace(c, bw, nrest,
+                           (const T*)(linv + (size_t)(b0 / kSolveNB) * kSolveNB * kSolveNB),
+                           A12 + b0, lda);
         }
       } else {
         int st = trsm_rec(c, ob, nrest, (const T*)L11, lda, A12, lda);
