@@ -72,6 +72,7 @@ namespace {
 // ---------------------------------------------------------------------------
 constexpr int kPW = 8;          // waves per panel workgroup
 constexpr int kPR = kPW * 64;   // rows per panel workgroup
+constexpr int kPU = 8;          // panel columns per unrolled chunk
 constexpr unsigned kSpinLimit = 1u << 20;
 
 template <typename T>
@@ -200,189 +201,207 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
 #pragma unroll
   for (int j = 0; j < W; ++j) v[j] = (valid && j < kb) ? Ap[(size_t)r + (size_t)j * lda] : T(0);
   if (tid == 0) s_abort = 0;
-  int rot = 0;  // column steps done: register k holds column (rot + k) mod W
+  bool stop = false;
+  int rot = 0;  // register k holds column (rot + k) mod W; rot = c0 within a chunk
 
-  for (int c = 0; c < kb; ++c) {
-    const int par = c & 1;
-    const unsigned tag8 = (unsigned)(c + 1);
-    const unsigned tag32 = (a.epoch << 8) | tag8;
-    // 1. wave candidate over rows >= c (column c is register 0)
-    T bv = (valid && r >= c) ? fabs(v[0]) : T(-1);
-    int br = (valid && r >= c) ? r : INT_MAX;
+  // Columns in chunks of kPU: within a chunk starting at c0, column c0 + u is
+  // register u (u known at compile time), and the registers rotate by kPU per
+  // chunk. LDS copies of rows use the rotation of the chunk (index k = column
+  // (c0 + k) mod W). Registers in [W - c0, W) hold finished columns; with c0 a
+  // multiple of kPU they form whole kPU-register groups, skipped as a unit.
+  for (int c0 = 0; c0 < kb && !stop; c0 += kPU) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const T ov = __shfl_xor(bv, off, 64);
-      const int orr = __shfl_xor(br, off, 64);
-      if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
-    }
-    if (lane == 0) { s_wval[w] = bv; s_wrowi[w] = br; }
-    if (br != INT_MAX && r == br) {
+    for (int u = 0; u < kPU; ++u) {
+      const int c = c0 + u;
+      if (c >= kb || stop) break;
+      const int par = c & 1;
+      const unsigned tag8 = (unsigned)(c + 1);
+      const unsigned tag32 = (a.epoch << 8) | tag8;
+      // 1. wave candidate over rows >= c (column c is register u)
+      T bv = (valid && r >= c) ? fabs(v[u]) : T(-1);
+      int br = (valid && r >= c) ? r : INT_MAX;
 #pragma unroll
-      for (int j = 0; j < W; ++j) s_wrow[w][j] = v[j];
-    }
-    if (r == c) {
+      for (int off = 32; off >= 1; off >>= 1) {
+        const T ov = __shfl_xor(bv, off, 64);
+        const int orr = __shfl_xor(br, off, 64);
+        if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+      }
+      if (lane == 0) { s_wval[w] = bv; s_wrowi[w] = br; }
+      if (br != INT_MAX && r == br) {
 #pragma unroll
-      for (int j = 0; j < W; ++j) s_rowc[j] = v[j];
-    }
-    __syncthreads();
-    // 2.-3. wave 0: publish, poll, fetch the pivot row
-    if (w == 0) {
-      T cv = T(-1);
-      int cr = INT_MAX, cw = 0;
-      if (lane < kPW) { cv = s_wval[lane]; cr = s_wrowi[lane]; cw = lane; }
+        for (int j = 0; j < W; ++j) s_wrow[w][j] = v[j];
+      }
+      if (r == c) {
 #pragma unroll
-      for (int off = 4; off >= 1; off >>= 1) {
-        const T ov = __shfl_xor(cv, off, 64);
-        const int orr = __shfl_xor(cr, off, 64);
-        const int ow = __shfl_xor(cw, off, 64);
-        if (better(ov, orr, cv, cr)) { cv = ov; cr = orr; cw = ow; }
+        for (int j = 0; j < W; ++j) s_rowc[j] = v[j];
       }
-      cv = __shfl(cv, 0, 64);
-      cr = __shfl(cr, 0, 64);
-      cw = __shfl(cw, 0, 64);
-      unsigned long long* rec = a.recs + ((size_t)par * a.gstride + blk) * RW;
-      if (cr != INT_MAX) {
-        for (int j = lane; j < kb; j += 64)
-          TW::put(rec + (size_t)j * NWD, s_wrow[cw][(j - c + W) % W], tag32);
-      }
-      if (blk == 0) {
-        unsigned long long* rc = a.rowc + (size_t)par * RW;
-        for (int j = lane; j < kb; j += 64)
-          TW::put(rc + (size_t)j * NWD, s_rowc[(j - c + W) % W], tag32);
-      }
-      if (lane == 0) {
-        const unsigned k32 = (cr != INT_MAX) ? TW::key(cv) : 0u;
-        const unsigned row24 = (cr != INT_MAX) ? (unsigned)cr : 0xffffffu;
-        st_ag(&a.keys[(size_t)par * a.gstride + blk],
-              ((unsigned long long)k32 << 32) | ((unsigned long long)tag8 << 24) | row24);
-      }
-      // poll the keys of all workgroups
-      unsigned long long bk = 0;  // (key << 24) | (0xffffff - row): larger is better
-      int bb = 0;
-      bool aborted = false;
-      for (unsigned spin = 0;; ++spin) {
-        bool ok = true;
-        unsigned long long lk = 0;
-        int lb = 0;
-        for (int q = lane; q < G; q += 64) {
-          const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
-          if (((unsigned)(x >> 24) & 0xffu) != tag8) ok = false;
-          const unsigned long long ck = ((x >> 32) << 24) | (0xffffffull - (x & 0xffffffull));
-          if (ck > lk) { lk = ck; lb = q; }
+      __syncthreads();
+      // 2.-3. wave 0: publish, poll, fetch the pivot row
+      if (w == 0) {
+        T cv = T(-1);
+        int cr = INT_MAX, cw = 0;
+        if (lane < kPW) { cv = s_wval[lane]; cr = s_wrowi[lane]; cw = lane; }
+#pragma unroll
+        for (int off = 4; off >= 1; off >>= 1) {
+          const T ov = __shfl_xor(cv, off, 64);
+          const int orr = __shfl_xor(cr, off, 64);
+          const int ow = __shfl_xor(cw, off, 64);
+          if (better(ov, orr, cv, cr)) { cv = ov; cr = orr; cw = ow; }
         }
-        if (__all(ok)) { bk = lk; bb = lb; break; }
-        if (spin > kSpinLimit || ld_ag32(a.abort_flag)) {
-          if (lane == 0) atomicExch(a.abort_flag, 1u);
-          aborted = true;
-          break;
+        cv = __shfl(cv, 0, 64);
+        cr = __shfl(cr, 0, 64);
+        cw = __shfl(cw, 0, 64);
+        unsigned long long* rec = a.recs + ((size_t)par * a.gstride + blk) * RW;
+        if (cr != INT_MAX) {
+          for (int j = lane; j < kb; j += 64)
+            TW::put(rec + (size_t)j * NWD, s_wrow[cw][(j - c0 + W) % W], tag32);
         }
-      }
-      int p = INT_MAX;
-      bool nz = false;
-      if (!aborted) {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-          const unsigned long long ok_ = __shfl_xor(bk, off, 64);
-          const int ob = __shfl_xor(bb, off, 64);
-          if (ok_ > bk) { bk = ok_; bb = ob; }
+        if (blk == 0) {
+          unsigned long long* rc = a.rowc + (size_t)par * RW;
+          for (int j = lane; j < kb; j += 64)
+            TW::put(rc + (size_t)j * NWD, s_rowc[(j - c0 + W) % W], tag32);
         }
-        const unsigned kmax = (unsigned)(bk >> 24);
-        int pb = bb;
-        p = (int)(0xffffffull - (bk & 0xffffffull));
-        if (sizeof(T) == 8) {
-          // workgroups tied on the FP32-rounded key: decide on the exact values
-          int ntie = 0;
+        if (lane == 0) {
+          const unsigned k32 = (cr != INT_MAX) ? TW::key(cv) : 0u;
+          const unsigned row24 = (cr != INT_MAX) ? (unsigned)cr : 0xffffffu;
+          st_ag(&a.keys[(size_t)par * a.gstride + blk],
+                ((unsigned long long)k32 << 32) | ((unsigned long long)tag8 << 24) | row24);
+        }
+        // poll the keys of all workgroups
+        unsigned long long bk = 0;  // (key << 24) | (0xffffff - row): larger is better
+        int bb = 0;
+        bool aborted = false;
+        for (unsigned spin = 0;; ++spin) {
+          bool ok = true;
+          unsigned long long lk = 0;
+          int lb = 0;
           for (int q = lane; q < G; q += 64) {
             const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
-            if ((unsigned)(x >> 32) == kmax && (x & 0xffffffull) != 0xffffffull) ++ntie;
+            if (((unsigned)(x >> 24) & 0xffu) != tag8) ok = false;
+            const unsigned long long ck = ((x >> 32) << 24) | (0xffffffull - (x & 0xffffffull));
+            if (ck > lk) { lk = ck; lb = q; }
           }
-          for (int off = 32; off >= 1; off >>= 1) ntie += __shfl_xor(ntie, off, 64);
-          if (ntie > 1) {
-            T ev = T(-1);
-            int er = INT_MAX, eb = 0;
-            for (int q = lane; q < G && !aborted; q += 64) {
+          if (__all(ok)) { bk = lk; bb = lb; break; }
+          if (spin > kSpinLimit || ld_ag32(a.abort_flag)) {
+            if (lane == 0) atomicExch(a.abort_flag, 1u);
+            aborted = true;
+            break;
+          }
+        }
+        int p = INT_MAX;
+        bool nz = false;
+        if (!aborted) {
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) {
+            const unsigned long long ok_ = __shfl_xor(bk, off, 64);
+            const int ob = __shfl_xor(bb, off, 64);
+            if (ok_ > bk) { bk = ok_; bb = ob; }
+          }
+          const unsigned kmax = (unsigned)(bk >> 24);
+          int pb = bb;
+          p = (int)(0xffffffull - (bk & 0xffffffull));
+          if (sizeof(T) == 8) {
+            // workgroups tied on the FP32-rounded key: decide on the exact values
+            int ntie = 0;
+            for (int q = lane; q < G; q += 64) {
               const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
-              if ((unsigned)(x >> 32) != kmax || (x & 0xffffffull) == 0xffffffull) continue;
-              const unsigned long long* wq =
-                  a.recs + ((size_t)par * a.gstride + q) * RW + (size_t)c * NWD;
-              T val = T(0);
-              unsigned spin = 0;
-              while (!TW::get(wq, tag32, val)) {
-                if (++spin > kSpinLimit || ld_ag32(a.abort_flag)) { aborted = true; break; }
-              }
-              const int rq = (int)(x & 0xffffffull);
-              if (better(fabs(val), rq, ev, er)) { ev = fabs(val); er = rq; eb = q; }
+              if ((unsigned)(x >> 32) == kmax && (x & 0xffffffull) != 0xffffffull) ++ntie;
             }
-            if (__any(aborted)) {
-              if (lane == 0) atomicExch(a.abort_flag, 1u);
-              aborted = true;
-            } else {
-#pragma unroll
-              for (int off = 32; off >= 1; off >>= 1) {
-                const T ov = __shfl_xor(ev, off, 64);
-                const int orr = __shfl_xor(er, off, 64);
-                const int ob = __shfl_xor(eb, off, 64);
-                if (better(ov, orr, ev, er)) { ev = ov; er = orr; eb = ob; }
+            for (int off = 32; off >= 1; off >>= 1) ntie += __shfl_xor(ntie, off, 64);
+            if (ntie > 1) {
+              T ev = T(-1);
+              int er = INT_MAX, eb = 0;
+              for (int q = lane; q < G && !aborted; q += 64) {
+                const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
+                if ((unsigned)(x >> 32) != kmax || (x & 0xffffffull) == 0xffffffull) continue;
+                const unsigned long long* wq =
+                    a.recs + ((size_t)par * a.gstride + q) * RW + (size_t)c * NWD;
+                T val = T(0);
+                unsigned spin = 0;
+                while (!TW::get(wq, tag32, val)) {
+                  if (++spin > kSpinLimit || ld_ag32(a.abort_flag)) { aborted = true; break; }
+                }
+                const int rq = (int)(x & 0xffffffull);
+                if (better(fabs(val), rq, ev, er)) { ev = fabs(val); er = rq; eb = q; }
               }
-              p = er;
-              pb = eb;
+              if (__any(aborted)) {
+                if (lane == 0) atomicExch(a.abort_flag, 1u);
+                aborted = true;
+              } else {
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                  const T ov = __shfl_xor(ev, off, 64);
+                  const int orr = __shfl_xor(er, off, 64);
+                  const int ob = __shfl_xor(eb, off, 64);
+                  if (better(ov, orr, ev, er)) { ev = ov; er = orr; eb = ob; }
+                }
+                p = er;
+                pb = eb;
+              }
             }
           }
+          if (!aborted) {
+            const unsigned long long* wrec = a.recs + ((size_t)par * a.gstride + pb) * RW;
+            aborted = !read_record<T, W>(wrec, tag32, kb, c0, s_pr, a.abort_flag);
+          }
+          if (!aborted) {
+            nz = s_pr[u] != T(0);
+            if (nz && p != c && p >= rbase && p < rbase + kPR)
+              aborted = !read_record<T, W>(a.rowc + (size_t)par * RW, tag32, kb, c0, s_rc,
+                                           a.abort_flag);
+          }
         }
-        if (!aborted) {
-          const unsigned long long* wrec = a.recs + ((size_t)par * a.gstride + pb) * RW;
-          aborted = !read_record<T, W>(wrec, tag32, kb, c, s_pr, a.abort_flag);
-        }
-        if (!aborted) {
-          nz = s_pr[0] != T(0);
-          if (nz && p != c && p >= rbase && p < rbase + kPR)
-            aborted = !read_record<T, W>(a.rowc + (size_t)par * RW, tag32, kb, c, s_rc,
-                                         a.abort_flag);
+        if (lane == 0) {
+          s_p = p;
+          s_ok = nz ? 1 : 0;
+          s_abort = aborted ? 1 : 0;
+          if (!aborted && blk == 0) {
+            a.ipiv[a.k0 + c] = a.k0 + (nz ? p : c);
+            if (!nz) atomicCAS(a.info, 0, a.k0 + c + 1);
+          }
         }
       }
-      if (lane == 0) {
-        s_p = p;
-        s_ok = nz ? 1 : 0;
-        s_abort = aborted ? 1 : 0;
-        if (!aborted && blk == 0) {
-          a.ipiv[a.k0 + c] = a.k0 + (nz ? p : c);
-          if (!nz) atomicCAS(a.info, 0, a.k0 + c + 1);
+      __syncthreads();
+      if (s_abort) { stop = true; break; }
+      // 4. interchange, multipliers, rank-1 update (zero pivot: nothing, as LAPACK)
+      if (s_ok) {
+        const int p = s_p;
+        if (p != c) {
+          if (r == p) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) v[k] = s_rc[k];
+          } else if (r == c) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) v[k] = s_pr[k];
+          }
+        }
+        const T piv = s_pr[u];
+        const bool below = valid && r > c;
+        const T l = below ? v[u] / piv : T(0);
+        if (below) v[u] = l;
+        // later columns: registers u+1 .. W-c0-1, in groups of kPU
+#pragma unroll
+        for (int g0 = 0; g0 < W; g0 += kPU) {
+          if (g0 + kPU <= u) continue;  // compile time: no later column in the group
+          if (g0 < W - c0) {            // uniform: the group holds later columns
+#pragma unroll
+            for (int k = g0; k < g0 + kPU; ++k)
+              if (k > u) v[k] -= l * s_pr[k];
+          }
         }
       }
     }
-    __syncthreads();
-    if (s_abort) break;
-    // 4. interchange, multipliers, rank-1 update (zero pivot: nothing, as LAPACK)
-    if (s_ok) {
-      const int p = s_p;
-      if (p != c) {
-        if (r == p) {
+    // rotate by kPU: column c0 + kPU into register 0
+    if (!stop) {
+      T t[kPU];
 #pragma unroll
-          for (int k = 0; k < W; ++k) v[k] = s_rc[k];
-        } else if (r == c) {
+      for (int k = 0; k < kPU; ++k) t[k] = v[k];
 #pragma unroll
-          for (int k = 0; k < W; ++k) v[k] = s_pr[k];
-        }
-      }
-      const T piv = s_pr[0];
-      const bool below = valid && r > c;
-      const T l = below ? v[0] / piv : T(0);
-      if (below) v[0] = l;
-      // register k >= 1 holds column c + k while c + k < W (later columns)
+      for (int k = 0; k < W - kPU; ++k) v[k] = v[k + kPU];
 #pragma unroll
-      for (int k = 1; k < W; ++k) {
-        const T lk = (k < W - c) ? l : T(0);
-        v[k] -= lk * s_pr[k];
-      }
+      for (int k = 0; k < kPU; ++k) v[W - kPU + k] = t[k];
+      rot += kPU;
     }
-    // rotate: column c + 1 into register 0
-    {
-      const T t0 = v[0];
-#pragma unroll
-      for (int k = 0; k < W - 1; ++k) v[k] = v[k + 1];
-      v[W - 1] = t0;
-    }
-    ++rot;
   }
   // register k holds column (rot + k) mod W
 #pragma unroll
