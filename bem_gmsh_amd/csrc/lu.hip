@@ -1157,10 +1157,13 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   constexpr int RW = KB * TagWords<T>::kWords;
   const int Gmax = (n + kPR - 1) / kPR;
   if (n >= (1 << 24)) return fail(c, BEMHIP_ERR_ARG, "lu: n must be below 2^24");
+  constexpr int OB = kOuterBlocks * kSolveNB;  // outer block: K of the trailing update
+  constexpr int PPB = OB / KB;                   // panels per outer block
+  constexpr int LSLOT = 4 * kMaxPanel + 16;      // ints per interchange-list slot
   // work buffers
   const size_t need = 8 * 2 * (size_t)Gmax + 8 * 2 * (size_t)Gmax * RW + 8 * 2 * (size_t)RW +
-                      sizeof(int) * (4 * kMaxPanel + 16) + sizeof(T) * kOuterBlocks * kSolveNB * kSolveNB +
-                      8 * 256;
+                      sizeof(int) * ((size_t)PPB * LSLOT + 16) +
+                      sizeof(T) * kOuterBlocks * kSolveNB * kSolveNB + 8 * 256;
   BEMHIP_HIP_TRY(c, c->lu_work.ensure(need));
   char* wp = c->lu_work.ptr;
   auto carve = [&](size_t bytes) {
@@ -1171,15 +1174,18 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   unsigned long long* keys = (unsigned long long*)carve(8 * 2 * (size_t)Gmax);
   unsigned long long* recs = (unsigned long long*)carve(8 * 2 * (size_t)Gmax * RW);
   unsigned long long* rowc = (unsigned long long*)carve(8 * 2 * (size_t)RW);
-  int* ldst = (int*)carve(sizeof(int) * 2 * kMaxPanel);
-  int* lsrc = (int*)carve(sizeof(int) * 2 * kMaxPanel);
-  int* llen = (int*)carve(sizeof(int) * 16);
+  int* lists = (int*)carve(sizeof(int) * (size_t)PPB * LSLOT);  // one per panel of a block
+  int* flags = (int*)carve(sizeof(int) * 16);
   T* linv = (T*)carve(sizeof(T) * kOuterBlocks * kSolveNB * kSolveNB);  // L11's 128-block inverses
-  int* info = llen + 4;
-  unsigned* abort_flag = (unsigned*)(llen + 8);
+  int* info = flags + 4;
+  unsigned* abort_flag = (unsigned*)(flags + 8);
+  auto l_dst = [&](int q) { return lists + (size_t)q * LSLOT; };
+  auto l_src = [&](int q) { return lists + (size_t)q * LSLOT + 2 * kMaxPanel; };
+  auto l_len = [&](int q) { return lists + (size_t)q * LSLOT + 4 * kMaxPanel; };
   if ((size_t)(wp - c->lu_work.ptr) > c->lu_work.count)
     return fail(c, BEMHIP_ERR_STATE, "lu: work buffer carve overflow");
-  BEMHIP_HIP_TRY(c, hipMemsetAsync(llen, 0, sizeof(int) * 16, c->stream));
+  hipStream_t sA = c->stream, sB = c->stream2;
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(flags, 0, sizeof(int) * 16, sA));
   int occ = 0;
   BEMHIP_HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                         &occ, (const void*)panel_kernel<T, KB>, kPR, 0));
@@ -1192,32 +1198,46 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   constexpr int VEC = 16 / (int)sizeof(T);
   const bool aligned = gemm_aligned(linv, kSolveNB, dA, lda, VEC);
 
-  // Two-level blocking (the structure of LAPACK getrf over getrf2): outer
-  // blocks of OB = 2 KB columns, each factored as KB-wide register-resident
-  // panels with their interchanges applied to every other column and the rest
-  // of the outer block updated right away; the trailing matrix is updated once
-  // per outer block, a K = OB product.
-  constexpr int OB = kOuterBlocks * kSolveNB;
+  // Timing: event pairs per phase on the stream that runs it (the phases of
+  // the two streams overlap, so their sums exceed the factorization's time).
   enum { kPanel = 0, kSwap = 1, kTrsm = 2, kGemm = 3 };
-  std::vector<hipEvent_t> ev;
-  std::vector<int> cat;
-  ev.reserve((size_t)(n / KB + 2) * 8);
-  auto mark = [&](int category) {
+  struct Seg {
+    hipEvent_t a, b;
+    int cat;
+  };
+  std::vector<Seg> segs;
+  std::vector<hipEvent_t> joins;
+  auto ev_now = [&](hipStream_t st) {
     hipEvent_t e;
     hipEventCreate(&e);
-    hipEventRecord(e, c->stream);
-    ev.push_back(e);
-    cat.push_back(category);  // category of the segment that ends at this event
+    hipEventRecord(e, st);
+    return e;
   };
-  mark(-1);
-  for (int k0 = 0; k0 < n; k0 += OB) {
-    const int ob = std::min(OB, n - k0);
-    for (int s = 0; s < ob; s += KB) {
+  auto seg_end = [&](hipStream_t st, hipEvent_t a0, int category) {
+    segs.push_back({a0, ev_now(st), category});
+  };
+  // the helpers (gemm_sub, trsm_rec, ...) enqueue on c->stream
+  struct StreamScope {
+    bemhip_ctx* c;
+    hipStream_t old;
+    StreamScope(bemhip_ctx* cc, hipStream_t st) : c(cc), old(cc->stream) { c->stream = st; }
+    ~StreamScope() { c->stream = old; }
+  };
+
+  // Factor outer block [k0, k0 + ob) on stream st: register-resident panels,
+  // each panel's interchanges applied to the other columns of the block (the
+  // list kept in slot q for the columns outside the block), the inverses of
+  // the block's 128-blocks of L11, and the rest of the block updated after
+  // each panel.
+  auto factor_block = [&](hipStream_t st, int k0, int ob) -> int {
+    StreamScope scope(c, st);
+    for (int s = 0, q = 0; s < ob; s += KB, ++q) {
       const int kc = k0 + s;
       const int kb = std::min(KB, ob - s);
       const int m = n - kc;
       const int G = (m + kPR - 1) / kPR;
-      BEMHIP_HIP_TRY(c, hipMemsetAsync(keys, 0, 8 * 2 * (size_t)Gmax, c->stream));
+      hipEvent_t e0 = ev_now(st);
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(keys, 0, 8 * 2 * (size_t)Gmax, st));
       PanelArgs<T> pa;
       pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = kc; pa.G = G; pa.gstride = Gmax;
       pa.epoch = (++c->panel_epoch) & 0xffffffu;
@@ -1225,23 +1245,25 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       pa.abort_flag = abort_flag;
       void* args[] = {&pa};
       BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, KB>, dim3(G),
-                                                   dim3(kPR), args, 0, c->stream));
-      mark(kPanel);
-      // interchanges outside the panel
-      hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, c->stream, dipiv, kc, kb, ldst, lsrc,
-                         llen);
-      const int ncols_out = n - kb;
-      if (ncols_out > 0)
-        hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_out + 3) / 4), dim3(256), 0, c->stream,
-                           dA, lda, 0, kc, kc + kb, n, ldst, lsrc, llen);
+                                                   dim3(kPR), args, 0, st));
+      seg_end(st, e0, kPanel);
+      e0 = ev_now(st);
+      hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, st, dipiv, kc, kb, l_dst(q), l_src(q),
+                         l_len(q));
+      const int ncols_blk = ob - kb;
+      if (ncols_blk > 0)
+        hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_blk + 3) / 4), dim3(256), 0, st, dA, lda,
+                           k0, kc, kc + kb, k0 + ob, (const int*)l_dst(q), (const int*)l_src(q),
+                           (const int*)l_len(q));
       BEMHIP_HIP_TRY(c, hipGetLastError());
-      mark(kSwap);
+      seg_end(st, e0, kSwap);
+      e0 = ev_now(st);
       // inverse of each completed 128-block of this outer block's L11
       if (aligned && ((s + kb) % kSolveNB == 0 || s + kb == ob)) {
         const int jb = (s + kb - 1) / kSolveNB;
         const int d0 = k0 + jb * kSolveNB;
         hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(1), dim3(kTriInvThreads),
-                           tri_inv_lds(sizeof(T)), c->stream, std::min(kSolveNB, ob - jb * kSolveNB),
+                           tri_inv_lds(sizeof(T)), st, std::min(kSolveNB, ob - jb * kSolveNB),
                            (const T*)(dA + (size_t)d0 + (size_t)d0 * lda), lda,
                            linv + (size_t)jb * kSolveNB * kSolveNB);
         BEMHIP_HIP_TRY(c, hipGetLastError());
@@ -1252,56 +1274,113 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
         T* L11 = dA + (size_t)kc + (size_t)kc * lda;
         T* A12 = dA + (size_t)kc + (size_t)(kc + kb) * lda;
         if (aligned && KB == kSolveNB) {  // the panel is one 128-block: U12 = L11^-1 A12
-          gemm_set_inplace(c, kb, rest, (const T*)(linv + (size_t)(s / kSolveNB) * kSolveNB *
-                                                              kSolveNB), A12, lda);
+          gemm_set_inplace(c, kb, rest,
+                           (const T*)(linv + (size_t)(s / kSolveNB) * kSolveNB * kSolveNB), A12,
+                           lda);
         } else {
-          int st = trsm_rec(c, kb, rest, (const T*)L11, lda, A12, lda);
-          if (st) return st;
+          int stt = trsm_rec(c, kb, rest, (const T*)L11, lda, A12, lda);
+          if (stt) return stt;
         }
-        mark(kTrsm);
+        seg_end(st, e0, kTrsm);
+        e0 = ev_now(st);
         T* L21 = dA + (size_t)(kc + kb) + (size_t)kc * lda;
         T* A22 = dA + (size_t)(kc + kb) + (size_t)(kc + kb) * lda;
         gemm_sub(c, n - kc - kb, rest, kb, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
         BEMHIP_HIP_TRY(c, hipGetLastError());
-        mark(kGemm);
-      }
-    }
-    const int nrest = n - k0 - ob;
-    if (nrest > 0) {
-      T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
-      T* A12 = dA + (size_t)k0 + (size_t)(k0 + ob) * lda;
-      if (aligned) {
-        // U12 = L11^-1 A12 by 128-blocks b: X_b -= L[b, <b] X_<b, then X_b = Linv_b X_b
-        for (int b0 = 0; b0 < ob; b0 += kSolveNB) {
-          const int bw = std::min(kSolveNB, ob - b0);
-          if (b0 > 0)
-            gemm_sub(c, bw, nrest, b0, (const T*)(L11 + b0), lda, (const T*)A12, lda, A12 + b0,
-                     lda);
-          gemm_set_inplace(c, bw, nrest,
-                           (const T*)(linv + (size_t)(b0 / kSolveNB) * kSolveNB * kSolveNB),
-                           A12 + b0, lda);
-        }
+        seg_end(st, e0, kGemm);
       } else {
-        int st = trsm_rec(c, ob, nrest, (const T*)L11, lda, A12, lda);
-        if (st) return st;
+        seg_end(st, e0, kTrsm);
       }
-      BEMHIP_HIP_TRY(c, hipGetLastError());
-      mark(kTrsm);
-      T* L21 = dA + (size_t)(k0 + ob) + (size_t)k0 * lda;
-      T* A22 = dA + (size_t)(k0 + ob) + (size_t)(k0 + ob) * lda;
-      gemm_sub(c, nrest, nrest, ob, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
-      BEMHIP_HIP_TRY(c, hipGetLastError());
-      mark(kGemm);
     }
+    return BEMHIP_OK;
+  };
+  // The block's interchanges on the columns outside it (stream A).
+  auto outer_perms = [&](int k0, int ob) -> int {
+    hipEvent_t e0 = ev_now(sA);
+    const int ncols = n - ob;
+    for (int s = 0, q = 0; s < ob && ncols > 0; s += KB, ++q)
+      hipLaunchKernelGGL((perm_apply<T>), dim3((ncols + 3) / 4), dim3(256), 0, sA, dA, lda, 0, k0,
+                         k0 + ob, n, (const int*)l_dst(q), (const int*)l_src(q),
+                         (const int*)l_len(q));
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    seg_end(sA, e0, kSwap);
+    return BEMHIP_OK;
+  };
+  // U12 = L11^-1 A12 for the columns [c0, n) of block k0 (stream A).
+  auto outer_trsm = [&](int k0, int ob, int c0) -> int {
+    const int nc = n - c0;
+    if (nc <= 0) return BEMHIP_OK;
+    hipEvent_t e0 = ev_now(sA);
+    T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
+    T* X = dA + (size_t)k0 + (size_t)c0 * lda;
+    if (aligned) {
+      // by 128-blocks b: X_b -= L[b, <b] X_<b, then X_b = Linv_b X_b
+      for (int b0 = 0; b0 < ob; b0 += kSolveNB) {
+        const int bw = std::min(kSolveNB, ob - b0);
+        if (b0 > 0)
+          gemm_sub(c, bw, nc, b0, (const T*)(L11 + b0), lda, (const T*)X, lda, X + b0, lda);
+        gemm_set_inplace(c, bw, nc,
+                         (const T*)(linv + (size_t)(b0 / kSolveNB) * kSolveNB * kSolveNB),
+                         X + b0, lda);
+      }
+    } else {
+      int st = trsm_rec(c, ob, nc, (const T*)L11, lda, X, lda);
+      if (st) return st;
+    }
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    seg_end(sA, e0, kTrsm);
+    return BEMHIP_OK;
+  };
+  // A[k0+ob.., c0..c1) -= L21 U12[:, c0..c1) (stream A).
+  auto outer_gemm = [&](int k0, int ob, int c0, int c1) -> int {
+    if (c1 <= c0 || n - k0 - ob <= 0) return BEMHIP_OK;
+    hipEvent_t e0 = ev_now(sA);
+    gemm_sub(c, n - k0 - ob, c1 - c0, ob, (const T*)(dA + (size_t)(k0 + ob) + (size_t)k0 * lda),
+             lda, (const T*)(dA + (size_t)k0 + (size_t)c0 * lda), lda,
+             dA + (size_t)(k0 + ob) + (size_t)c0 * lda, lda);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    seg_end(sA, e0, kGemm);
+    return BEMHIP_OK;
+  };
+
+  // Blocked right-looking LU with look-ahead (the structure of LAPACK getrf
+  // over getrf2, plus one block of look-ahead): after block j's U12, block
+  // j+1's columns are updated first; block j+1 is then factored on stream B
+  // while stream A updates the remaining trailing columns with block j. Block
+  // j+1's interchanges reach the columns outside it only after that update:
+  // P (A22 - L21 U12) = P A22 - (P L21) U12, and during the overlap the two
+  // streams touch disjoint columns.
+  int st = factor_block(sA, 0, std::min(OB, n));
+  if (st) return st;
+  if ((st = outer_perms(0, std::min(OB, n)))) return st;
+  for (int k0 = 0; k0 < n; k0 += OB) {
+    const int ob = std::min(OB, n - k0);
+    const int k1 = k0 + ob;
+    if (k1 >= n) break;
+    const int ob1 = std::min(OB, n - k1);
+    const int k2 = k1 + ob1;
+    if ((st = outer_trsm(k0, ob, k1))) return st;
+    if ((st = outer_gemm(k0, ob, k1, k2))) return st;
+    hipEvent_t next_ready = ev_now(sA);
+    joins.push_back(next_ready);
+    BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sB, next_ready, 0));
+    if ((st = factor_block(sB, k1, ob1))) return st;
+    hipEvent_t factored = ev_now(sB);
+    joins.push_back(factored);
+    if ((st = outer_gemm(k0, ob, k2, n))) return st;
+    BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sA, factored, 0));
+    if ((st = outer_perms(k1, ob1))) return st;
   }
-  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(sA));
   double tcat[4] = {0, 0, 0, 0};
-  for (size_t i = 1; i < ev.size(); ++i) {
+  for (auto& sg : segs) {
     float t = 0;
-    hipEventElapsedTime(&t, ev[i - 1], ev[i]);
-    tcat[cat[i]] += t;
+    hipEventElapsedTime(&t, sg.a, sg.b);
+    tcat[sg.cat] += t;
+    hipEventDestroy(sg.a);
+    hipEventDestroy(sg.b);
   }
-  for (auto& x : ev) hipEventDestroy(x);
+  for (auto& e : joins) hipEventDestroy(e);
   const double t_panel = tcat[kPanel], t_swap = tcat[kSwap], t_trsm = tcat[kTrsm],
                t_gemm = tcat[kGemm];
   const char* pfx = (sizeof(T) == 8) ? "lu64_" : "lu32_";
@@ -1312,12 +1391,12 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   c->stats[std::string(pfx) + "kb"] = KB;
   c->stats[std::string(pfx) + "ob"] = OB;
   c->stats[std::string(pfx) + "panel_wgs"] = Gmax;
-  int flags[16];
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(flags, llen, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
-  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (flags[8] != 0)
+  int hflags[16];
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, sA));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(sA));
+  if (hflags[8] != 0)
     return fail(c, BEMHIP_ERR_HIP, "lu: panel exchange wait ran out (factorization abandoned)");
-  if (info_host) *info_host = flags[4];
+  if (info_host) *info_host = hflags[4];
   return BEMHIP_OK;
 }
 
