@@ -60,6 +60,12 @@ struct DevBuf {
   }
 };
 
+// One timed GEMM launch (event pair on the launching stream, flops).
+struct GemmTiming {
+  hipEvent_t a, b;
+  double flops;
+};
+
 }  // namespace bemhip
 
 struct bemhip_ctx {
@@ -121,6 +127,7 @@ struct bemhip_ctx {
   // scratch for LU and solves
   bemhip::DevBuf<char> lu_work;
   unsigned panel_epoch = 0;           // tag base of the LU panel exchange words
+  std::vector<bemhip::GemmTiming>* gemm_log = nullptr;  // set during an LU: large-tile GEMM timings
   bemhip::DevBuf<char> dinv;          // inverses of the diagonal blocks of L and U
   const void* dinv_of = nullptr;      // factor the inverses belong to
   int dinv_n = 0;

@@ -87,7 +87,8 @@ struct PanelArgs {
   unsigned epoch; // per-launch tag base of the data words
   int* ipiv;      // ipiv[k0 + c] = global pivot row of column k0 + c
   int* info;      // first zero pivot: global column + 1 (0 if none)
-  unsigned long long* keys;   // [2][gstride] key words
+  unsigned long long* keys;   // [2][gstride] key words (zero at launch)
+  unsigned long long* keys_next;  // the next launch's key buffer, cleared by this one
   unsigned long long* recs;   // [2][gstride][RW] candidate rows (tagged words)
   unsigned long long* rowc;   // [2][RW] row c before the interchange (tagged words)
   unsigned* abort_flag;       // raised by a workgroup whose wait ran out
@@ -201,6 +202,10 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
 #pragma unroll
   for (int j = 0; j < W; ++j) v[j] = (valid && j < kb) ? Ap[(size_t)r + (size_t)j * lda] : T(0);
   if (tid == 0) s_abort = 0;
+  // Clear this workgroup's key slots of the next launch's buffer: the previous
+  // launch (the buffer's last user) has finished, and the next launch of the
+  // factorization has no more workgroups than this one.
+  if (tid < 2) a.keys_next[(size_t)tid * a.gstride + blk] = 0ull;
   bool stop = false;
   int rot = 0;  // register k holds column (rot + k) mod W; rot = c0 within a chunk
 
@@ -423,13 +428,18 @@ __global__ __launch_bounds__(64) void perm_build(const int* __restrict__ ipiv, i
                                                  int* __restrict__ list_len) {
   __shared__ int pos[2 * kMaxPanel];
   __shared__ int content[2 * kMaxPanel];
+  __shared__ int piv[kMaxPanel];
   __shared__ int s_nout;
   const int lane = threadIdx.x;
-  for (int t = lane; t < kb; t += 64) { pos[t] = k0 + t; content[t] = k0 + t; }
+  for (int t = lane; t < kb; t += 64) {
+    pos[t] = k0 + t;
+    content[t] = k0 + t;
+    piv[t] = ipiv[k0 + t];
+  }
   if (lane == 0) s_nout = 0;
   __syncthreads();
   for (int s = 0; s < kb; ++s) {
-    const int r = ipiv[k0 + s];
+    const int r = piv[s];
     int idx;
     if (r < k0 + kb) {
       idx = r - k0;
@@ -1104,8 +1114,9 @@ void gemm_set_inplace(bemhip_ctx* c, int M, int N, const T* Linv, T* X, int ldx)
 }
 
 // C -= A B. 16-byte aligned operands (every LU call on the padded matrices)
-// take the tile kernel: large products on 256-wide tiles, small ones on 128x128
-// tiles for more workgroups; anything else takes the generic kernel.
+// take the tile kernel: products with at least one 256-wide tile per CU on
+// 256-wide tiles, smaller ones on 128x128 tiles for more workgroups; anything
+// else takes the generic kernel.
 template <typename T>
 int gemm_sub(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B, int ldb, T* C,
              int ldc) {
@@ -1114,16 +1125,24 @@ int gemm_sub(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B
   if (gemm_aligned(A, lda, B, ldb, VEC)) {
     const int cus = std::max(1, c->num_cu);
     const long big = (long)((M + 255) / 256) * (long)((N + 255) / 256);
-    if constexpr (sizeof(T) == 4) {
-      if (big >= 2L * cus)
+    if (big >= cus) {
+      // the large-tile launches (the LU's trailing updates) are timed one by one
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (c->gemm_log) {
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, c->stream);
+      }
+      if constexpr (sizeof(T) == 4)
         launch_tile_gemm<T, 256, 256, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
       else
-        launch_tile_gemm<T, 128, 128, 2, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
-    } else {
-      if (big >= 2L * cus)
         launch_tile_gemm<T, 256, 128, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
-      else
-        launch_tile_gemm<T, 128, 128, 2, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
+      if (c->gemm_log) {
+        hipEventRecord(e1, c->stream);
+        c->gemm_log->push_back({e0, e1, 2.0 * M * (double)N * K});
+      }
+    } else {
+      launch_tile_gemm<T, 128, 128, 2, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
     }
     return BEMHIP_OK;
   }
@@ -1161,9 +1180,9 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   constexpr int PPB = OB / KB;                   // panels per outer block
   constexpr int LSLOT = 4 * kMaxPanel + 16;      // ints per interchange-list slot
   // work buffers
-  const size_t need = 8 * 2 * (size_t)Gmax + 8 * 2 * (size_t)Gmax * RW + 8 * 2 * (size_t)RW +
+  const size_t need = 8 * 4 * (size_t)Gmax + 8 * 2 * (size_t)Gmax * RW + 8 * 2 * (size_t)RW +
                       sizeof(int) * ((size_t)PPB * LSLOT + 16) +
-                      sizeof(T) * kOuterBlocks * kSolveNB * kSolveNB + 8 * 256;
+                      sizeof(T) * 2 * kOuterBlocks * kSolveNB * kSolveNB + 8 * 256;
   BEMHIP_HIP_TRY(c, c->lu_work.ensure(need));
   char* wp = c->lu_work.ptr;
   auto carve = [&](size_t bytes) {
@@ -1171,12 +1190,17 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     wp += (bytes + 255) & ~size_t(255);
     return p;
   };
-  unsigned long long* keys = (unsigned long long*)carve(8 * 2 * (size_t)Gmax);
+  unsigned long long* keys = (unsigned long long*)carve(8 * 4 * (size_t)Gmax);  // 2 buffers
   unsigned long long* recs = (unsigned long long*)carve(8 * 2 * (size_t)Gmax * RW);
   unsigned long long* rowc = (unsigned long long*)carve(8 * 2 * (size_t)RW);
   int* lists = (int*)carve(sizeof(int) * (size_t)PPB * LSLOT);  // one per panel of a block
   int* flags = (int*)carve(sizeof(int) * 16);
-  T* linv = (T*)carve(sizeof(T) * kOuterBlocks * kSolveNB * kSolveNB);  // L11's 128-block inverses
+  // inverses of the outer block's 128-blocks of L11, by block parity: block j+1's
+  // are built on stream B while block j's still serve its TRSM on stream A
+  T* linv2 = (T*)carve(sizeof(T) * 2 * kOuterBlocks * kSolveNB * kSolveNB);
+  auto linv_of = [&](int k0) {
+    return linv2 + (size_t)((k0 / OB) & 1) * kOuterBlocks * kSolveNB * kSolveNB;
+  };
   int* info = flags + 4;
   unsigned* abort_flag = (unsigned*)(flags + 8);
   auto l_dst = [&](int q) { return lists + (size_t)q * LSLOT; };
@@ -1186,6 +1210,14 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     return fail(c, BEMHIP_ERR_STATE, "lu: work buffer carve overflow");
   hipStream_t sA = c->stream, sB = c->stream2;
   BEMHIP_HIP_TRY(c, hipMemsetAsync(flags, 0, sizeof(int) * 16, sA));
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(keys, 0, 8 * 4 * (size_t)Gmax, sA));
+  int key_buf = 0;  // key buffer of the next panel launch
+  std::vector<GemmTiming> big_gemms;
+  c->gemm_log = &big_gemms;
+  struct LogOff {
+    bemhip_ctx* c;
+    ~LogOff() { c->gemm_log = nullptr; }
+  } log_off{c};
   int occ = 0;
   BEMHIP_HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                         &occ, (const void*)panel_kernel<T, KB>, kPR, 0));
@@ -1196,7 +1228,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)tri_inv_lds(sizeof(T))));
   constexpr int VEC = 16 / (int)sizeof(T);
-  const bool aligned = gemm_aligned(linv, kSolveNB, dA, lda, VEC);
+  const bool aligned = gemm_aligned(linv2, kSolveNB, dA, lda, VEC);
 
   // Timing: event pairs per phase on the stream that runs it (the phases of
   // the two streams overlap, so their sums exceed the factorization's time).
@@ -1231,17 +1263,20 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   // each panel.
   auto factor_block = [&](hipStream_t st, int k0, int ob) -> int {
     StreamScope scope(c, st);
+    T* linv = linv_of(k0);
     for (int s = 0, q = 0; s < ob; s += KB, ++q) {
       const int kc = k0 + s;
       const int kb = std::min(KB, ob - s);
       const int m = n - kc;
       const int G = (m + kPR - 1) / kPR;
       hipEvent_t e0 = ev_now(st);
-      BEMHIP_HIP_TRY(c, hipMemsetAsync(keys, 0, 8 * 2 * (size_t)Gmax, st));
       PanelArgs<T> pa;
       pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = kc; pa.G = G; pa.gstride = Gmax;
       pa.epoch = (++c->panel_epoch) & 0xffffffu;
-      pa.ipiv = dipiv; pa.info = info; pa.keys = keys; pa.recs = recs; pa.rowc = rowc;
+      pa.ipiv = dipiv; pa.info = info; pa.recs = recs; pa.rowc = rowc;
+      pa.keys = keys + (size_t)key_buf * 2 * Gmax;
+      pa.keys_next = keys + (size_t)(key_buf ^ 1) * 2 * Gmax;
+      key_buf ^= 1;
       pa.abort_flag = abort_flag;
       void* args[] = {&pa};
       BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, KB>, dim3(G),
@@ -1306,10 +1341,11 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     seg_end(sA, e0, kSwap);
     return BEMHIP_OK;
   };
-  // U12 = L11^-1 A12 for the columns [c0, n) of block k0 (stream A).
-  auto outer_trsm = [&](int k0, int ob, int c0) -> int {
-    const int nc = n - c0;
+  // U12 = L11^-1 A12 for the columns [c0, c1) of block k0 (stream A).
+  auto outer_trsm = [&](int k0, int ob, int c0, int c1) -> int {
+    const int nc = c1 - c0;
     if (nc <= 0) return BEMHIP_OK;
+    const T* linv = linv_of(k0);
     hipEvent_t e0 = ev_now(sA);
     T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
     T* X = dA + (size_t)k0 + (size_t)c0 * lda;
@@ -1359,7 +1395,8 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     if (k1 >= n) break;
     const int ob1 = std::min(OB, n - k1);
     const int k2 = k1 + ob1;
-    if ((st = outer_trsm(k0, ob, k1))) return st;
+    // the next block's columns first: U12 and the update, then its factorization
+    if ((st = outer_trsm(k0, ob, k1, k2))) return st;
     if ((st = outer_gemm(k0, ob, k1, k2))) return st;
     hipEvent_t next_ready = ev_now(sA);
     joins.push_back(next_ready);
@@ -1367,6 +1404,8 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     if ((st = factor_block(sB, k1, ob1))) return st;
     hipEvent_t factored = ev_now(sB);
     joins.push_back(factored);
+    // meanwhile the remaining trailing columns
+    if ((st = outer_trsm(k0, ob, k2, n))) return st;
     if ((st = outer_gemm(k0, ob, k2, n))) return st;
     BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sA, factored, 0));
     if ((st = outer_perms(k1, ob1))) return st;
@@ -1381,6 +1420,15 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     hipEventDestroy(sg.b);
   }
   for (auto& e : joins) hipEventDestroy(e);
+  double big_ms = 0.0, big_flops = 0.0;
+  for (auto& g : big_gemms) {
+    float t = 0;
+    hipEventElapsedTime(&t, g.a, g.b);
+    big_ms += t;
+    big_flops += g.flops;
+    hipEventDestroy(g.a);
+    hipEventDestroy(g.b);
+  }
   const double t_panel = tcat[kPanel], t_swap = tcat[kSwap], t_trsm = tcat[kTrsm],
                t_gemm = tcat[kGemm];
   const char* pfx = (sizeof(T) == 8) ? "lu64_" : "lu32_";
@@ -1391,6 +1439,10 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   c->stats[std::string(pfx) + "kb"] = KB;
   c->stats[std::string(pfx) + "ob"] = OB;
   c->stats[std::string(pfx) + "panel_wgs"] = Gmax;
+  // large-tile trailing-update launches: count, flops, summed duration
+  c->ms[std::string(pfx) + "big_gemm"] = big_ms;
+  c->stats[std::string(pfx) + "big_gemm_launches"] = (double)big_gemms.size();
+  c->stats[std::string(pfx) + "big_gemm_flops"] = big_flops;
   int hflags[16];
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, sA));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(sA));

@@ -102,28 +102,6 @@ class Ctx:
         self.L.bemhip_destroy(self.h)
 
 
-def gemm_flops(n, kb, ob):
-    """FP32 flops and launches of the LU's trailing-update GEMMs: per outer block
-    of ob columns, the kb-wide panels' updates of the rest of the block, then
-    the K = ob update of the trailing matrix."""
-    tot = 0.0
-    launches = 0
-    for k0 in range(0, n, ob):
-        blk = min(ob, n - k0)
-        for s in range(0, blk, kb):
-            kc = k0 + s
-            w = min(kb, blk - s)
-            rest = k0 + blk - (kc + w)
-            if rest > 0 and n - kc - w > 0:
-                tot += 2.0 * (n - kc - w) * rest * w
-                launches += 1
-        nrest = n - k0 - blk
-        if nrest > 0:
-            tot += 2.0 * nrest * nrest * blk
-            launches += 1
-    return tot, launches
-
-
 def cpu_baseline(dad, npg, rows, lu_n, threads):
     """Reference-algorithm CPU restatement (the FP64 oracle, OpenMP over rows)
     on a bounded sample: `rows` assembly rows of this workload, and an LAPACK LU
@@ -239,7 +217,8 @@ def main():
     for w in range(args.warmup):
         ctx.pipeline(dad, args.npg, 1, 1)
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
-    asm_ms, lu_ms, ref_ms, iters, far_ms, gemm_ms = [], [], [], [], [], []
+    asm_ms, lu_ms, ref_ms, iters, far_ms = [], [], [], [], []
+    big_ms_l, big_n_l, big_flops_l = [], [], []
     t_start = time.perf_counter()
     x_mixed = None
     for s in range(args.steps):
@@ -248,7 +227,10 @@ def main():
         asm_ms.append(cnt["assemble"])
         far_ms.append(cnt["assemble_far"])
         lu_ms.append(cnt["lu32"])
-        gemm_ms.append(cnt["lu32_gemm"])
+        st = ctx.counters()["stats"]
+        big_ms_l.append(cnt["lu32_big_gemm"])
+        big_n_l.append(st["lu32_big_gemm_launches"])
+        big_flops_l.append(st["lu32_big_gemm_flops"])
         ref_ms.append(cnt.get("refine", 0.0))
         iters.append(it)
         log(f"[bench] step {s + 1}/{args.steps}: assemble {cnt['assemble']:.2f} ms, "
@@ -297,18 +279,20 @@ def main():
     if not args.no_adjoint:
         detail["adjoint"] = adjoint_leg()
 
-    # roofline of the dominant kernel (FP32 MFMA trailing update of the LU)
-    kb = int(counters["stats"].get("lu32_kb", 128))
-    ob = int(counters["stats"].get("lu32_ob", kb))
-    gflops, glaunch = gemm_flops(nt, kb, ob)
-    gemm_s = float(np.mean(gemm_ms)) * 1e-3
-    ach = gflops / gemm_s
+    # roofline of the dominant kernel: the LU's FP32 trailing updates on
+    # 256x256 tiles, each launch timed with HIP events on its stream
+    big_ms = float(np.mean(big_ms_l))
+    big_n = int(big_n_l[-1])
+    big_flops = float(np.mean(big_flops_l))
+    ach = big_flops / (big_ms * 1e-3)
     traffic = load_traffic(nt)
-    roofline = {"kernel": "gemm_kernel<float> (LU trailing update)", "bound": "mfma",
+    big_name = "gemm_tile_kernel<float, 256, 256, 4, 2, false>"
+    roofline = {"kernel": big_name + " (LU trailing update)", "bound": "mfma",
                 "achieved": ach / 1e12, "peak": PEAK_MFMA_F32 / 1e12, "unit": "TFLOP/s",
                 "frac": ach / PEAK_MFMA_F32,
-                "traffic": traffic.get("gemm_kernel<float>") if traffic else None,
-                "launches": glaunch, "flops_total": gflops}
+                "traffic": traffic.get(big_name) if traffic else None,
+                "launches": big_n, "flops_per_launch": big_flops / max(big_n, 1),
+                "avg_launch_us": 1e3 * big_ms / max(big_n, 1)}
     far_bytes = entries * 4.0
     far_s = float(np.mean(far_ms)) * 1e-3
     roofline_far = {"kernel": "far_kernel<float,3> (H,G far field)", "bound": "hbm",
