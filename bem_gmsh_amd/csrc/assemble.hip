@@ -398,10 +398,20 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       const int t = el * NN + k;
-      const double U = -log(d2[k]) * cs.cU;
-      const double T = (rx[k] * s_nx[t] + ry[k] * s_ny[t]) * (cs.cT / d2[k]);
+      // A column with zero H (G) weights needs no T (U): the term would be an
+      // exact zero. The test is uniform over the workgroup (weights per column).
+      const bool needH = s_w[0][t] != 0.0 || (nvec == 2 && s_w[2][t] != 0.0);
+      const bool needG = s_w[1][t] != 0.0 || (nvec == 2 && s_w[3][t] != 0.0);
       const double w = s_jw[t];
-      const double h = T * w, gg = U * w;
+      double h = 0.0, gg = 0.0;
+      if (needG) {
+        const double U = -log(d2[k]) * cs.cU;
+        gg = U * w;
+      }
+      if (needH) {
+        const double T = (rx[k] * s_nx[t] + ry[k] * s_ny[t]) * (cs.cT / d2[k]);
+        h = T * w;
+      }
       acc1 += h * s_w[0][t] + gg * s_w[1][t];
       if (nvec == 2) acc2 += h * s_w[2][t] + gg * s_w[3][t];
     }
