@@ -435,6 +435,7 @@ int bemhip_synchronize(bemhip_ctx* c) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "synchronize: NULL context");
   hipSetDevice(c->device);
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream2));
   return BEMHIP_OK;
 }
 

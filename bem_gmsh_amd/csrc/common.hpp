@@ -64,6 +64,7 @@ struct DevBuf {
 struct GemmTiming {
   hipEvent_t a, b;
   double flops;
+  double bytes;  // algorithmic: A and B read once, C read and written once
 };
 
 }  // namespace bemhip

@@ -1139,7 +1139,8 @@ int gemm_sub(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B
         launch_tile_gemm<T, 256, 128, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
       if (c->gemm_log) {
         hipEventRecord(e1, c->stream);
-        c->gemm_log->push_back({e0, e1, 2.0 * M * (double)N * K});
+        c->gemm_log->push_back({e0, e1, 2.0 * M * (double)N * K,
+                                sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N)});
       }
     } else {
       launch_tile_gemm<T, 128, 128, 2, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
@@ -1420,12 +1421,13 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     hipEventDestroy(sg.b);
   }
   for (auto& e : joins) hipEventDestroy(e);
-  double big_ms = 0.0, big_flops = 0.0;
+  double big_ms = 0.0, big_flops = 0.0, big_bytes = 0.0;
   for (auto& g : big_gemms) {
     float t = 0;
     hipEventElapsedTime(&t, g.a, g.b);
     big_ms += t;
     big_flops += g.flops;
+    big_bytes += g.bytes;
     hipEventDestroy(g.a);
     hipEventDestroy(g.b);
   }
@@ -1443,6 +1445,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   c->ms[std::string(pfx) + "big_gemm"] = big_ms;
   c->stats[std::string(pfx) + "big_gemm_launches"] = (double)big_gemms.size();
   c->stats[std::string(pfx) + "big_gemm_flops"] = big_flops;
+  c->stats[std::string(pfx) + "big_gemm_bytes"] = big_bytes;
   int hflags[16];
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, sA));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(sA));

@@ -176,6 +176,27 @@ def adjoint_leg(ne=1365, seed=1, npg=12):
     return out
 
 
+def write_ceiling(ne):
+    """Practical HBM write bandwidth: a write-only fill (torch's fill kernel) of
+    the bytes the far-field kernel writes, 2 nt n FP32 entries."""
+    import torch
+    nt = 3 * ne
+    nbytes = 2 * nt * nt * 4
+    buf = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = None
+    for r in range(5):
+        a.record()
+        buf.fill_(1.0)
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b)
+        if r and (best is None or ms < best):
+            best = ms
+    print(json.dumps({"write_ceiling": {"bytes": nbytes, "ms": best,
+                                        "GB_per_s": nbytes / (best * 1e-3) / 1e9}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,12 +209,17 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--phase", choices=["all", "assemble"], default="all",
                     help="'assemble': only assembly + BC passes (for counter runs)")
+    ap.add_argument("--write-ceiling", action="store_true",
+                    help="only time a write-only fill of the FP32 H+G size (HBM write ceiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the path is single-GPU; "
               "see the sharding notes); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.write_ceiling:
+        write_ceiling(args.ne)
+        return
     t0 = time.perf_counter()
     dad = build_problem(args.ne)
     nt, n = dad.nt, dad.n
@@ -218,7 +244,7 @@ def main():
         ctx.pipeline(dad, args.npg, 1, 1)
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
     asm_ms, lu_ms, ref_ms, iters, far_ms = [], [], [], [], []
-    big_ms_l, big_n_l, big_flops_l = [], [], []
+    big_ms_l, big_n_l, big_flops_l, big_bytes_l = [], [], [], []
     t_start = time.perf_counter()
     x_mixed = None
     for s in range(args.steps):
@@ -231,6 +257,7 @@ def main():
         big_ms_l.append(cnt["lu32_big_gemm"])
         big_n_l.append(st["lu32_big_gemm_launches"])
         big_flops_l.append(st["lu32_big_gemm_flops"])
+        big_bytes_l.append(st["lu32_big_gemm_bytes"])
         ref_ms.append(cnt.get("refine", 0.0))
         iters.append(it)
         log(f"[bench] step {s + 1}/{args.steps}: assemble {cnt['assemble']:.2f} ms, "
@@ -292,6 +319,7 @@ def main():
                 "frac": ach / PEAK_MFMA_F32,
                 "traffic": traffic.get(big_name) if traffic else None,
                 "launches": big_n, "flops_per_launch": big_flops / max(big_n, 1),
+                "algorithmic_bytes_per_launch": float(np.mean(big_bytes_l)) / max(big_n, 1),
                 "avg_launch_us": 1e3 * big_ms / max(big_n, 1)}
     far_bytes = entries * 4.0
     far_s = float(np.mean(far_ms)) * 1e-3

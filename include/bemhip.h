@@ -10,11 +10,15 @@
  *
  * Memory: the caller owns host buffers; the context owns device buffers
  * (geometry, H, G, A, LU factors, pivots), which stay resident in HBM between
- * calls. All matrices are column-major with leading dimension nt, exactly the
- * Julia layout of dad.H (nt x nt) and dad.G (nt x n).
+ * calls. All host matrices are column-major with leading dimension nt, exactly
+ * the Julia layout of dad.H (nt x nt) and dad.G (nt x n); on the device the
+ * library pads the leading dimension.
  *
- * Threading: a context is bound to one device and one HIP stream; calls on a
- * context are serialized and not re-entrant. Separate contexts are independent.
+ * Threading: a context is bound to one device and two HIP streams (default
+ * priority; the second runs the far field during assembly and the panels of the
+ * LU's look-ahead). Each call orders its work on both and returns when it is
+ * done; calls on a context are serialized and not re-entrant. Separate contexts
+ * are independent.
  *
  * Reference interfaces (paths relative to the BEM_gmsh repository root) are
  * cited on each declaration.
@@ -166,11 +170,11 @@ int bemhip_lu_solve(bemhip_ctx* ctx, int n, int dtype, const void* LU, const int
 int bemhip_adjoint(bemhip_ctx* ctx, const double* q_target, const double* w, double* dJ_dgeo,
                    double* J);
 
-/* Timings (ms, HIP events on the context stream) and sizes of the last calls
+/* Timings (ms, HIP events on the context's streams) and sizes of the last calls
  * as a JSON object. */
 int bemhip_counters(bemhip_ctx* ctx, char* json, size_t cap);
 
-/* Wait for all work queued on the context stream. */
+/* Wait for all work queued on the context's streams. */
 int bemhip_synchronize(bemhip_ctx* ctx);
 
 #ifdef __cplusplus
