@@ -17,6 +17,8 @@ from .analytical import (AnalyticalSolution, ana_exp_cos, ana_laplace_linear,
 from .api import (ArgumentError, HipContext, adjoint, apply_bc, apply_operator, assemble,
                   get_HG, lu_factor, lu_solve, near_pairs, solve, split_sol,
                   unique_node_gradient)
+from .interior import (gera_p_in, interior_laplace, internal_grid, internal_grid_, internal_layer,
+                       internal_layer_, point_in_domain, set_internal_nodes)
 
 __all__ = [
     "BEMData", "Element", "Laplace", "Mesh2D", "SplitMix64", "ellipse_mesh", "format2d",
@@ -26,5 +28,7 @@ __all__ = [
     "analytical", "analytical_flux", "apply_analytical_bc", "apply_bc_from_physical",
     "attach_analytical", "rel_error", "rel_error_flux", "ArgumentError", "HipContext",
     "adjoint", "apply_bc", "apply_operator", "assemble", "get_HG", "lu_factor", "lu_solve",
-    "near_pairs", "solve", "split_sol", "unique_node_gradient",
+    "near_pairs", "solve", "split_sol", "unique_node_gradient", "gera_p_in",
+    "interior_laplace", "internal_grid", "internal_grid_", "internal_layer", "internal_layer_",
+    "point_in_domain", "set_internal_nodes",
 ]

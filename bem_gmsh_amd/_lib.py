@@ -47,6 +47,9 @@ SIGNATURES = [
     ("bemhip_lu_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p,
                                   c_double_p, C.c_int]),
     ("bemhip_adjoint", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p]),
+    ("bemhip_interior_laplace", C.c_int,
+     [C.c_void_p, C.c_int, c_double_p, c_double_p, c_double_p, C.c_int, C.c_double,
+      c_double_p]),
     ("bemhip_counters", C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     ("bemhip_synchronize", C.c_int, [C.c_void_p]),
 ]

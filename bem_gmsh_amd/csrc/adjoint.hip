@@ -52,6 +52,7 @@ Geo make_geo_adj(const bemhip_ctx* c) {
   g.p = c->p; g.nn = c->nn; g.ne = c->ne; g.n = c->n; g.ni = c->ni; g.nt = c->nt;
   g.x = c->x.ptr; g.y = c->y.ptr; g.nx = c->nx.ptr; g.ny = c->ny.ptr; g.jw = c->jw.ptr;
   g.elen = c->elen.ptr; g.gx = c->gx.ptr; g.gy = c->gy.ptr;
+  g.rx = c->x.ptr; g.ry = c->y.ptr; g.row0 = 0;
   g.ld = (c->precision == BEMHIP_FP64) ? c->ld64 : c->ld32;
   return g;
 }

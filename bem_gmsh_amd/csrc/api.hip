@@ -82,11 +82,11 @@ static void diff_matrix(int n, const double* x, const double* bw, double* D) {
   }
 }
 
-static void build_quadconst(bemhip_ctx* c, QuadConst* q) {
+static void build_quadconst(bemhip_ctx* c, QuadConst* q, int npg, double near_factor) {
   std::memset(q, 0, sizeof(*q));
-  q->npg = c->npg;
+  q->npg = npg;
   q->p = c->p;
-  gauss_legendre_host(c->npg, q->gl_x, q->gl_w);
+  gauss_legendre_host(npg, q->gl_x, q->gl_w);
   gauss_legendre_host(kNinterp, q->g20_x, q->g20_w);
   for (int i = 0; i < kNinterp; ++i) {
     double wi = 1.0;
@@ -113,7 +113,7 @@ static void build_quadconst(bemhip_ctx* c, QuadConst* q) {
   diff_matrix(nn, q->eq_x, q->eq_bw, q->eq_D);
   for (int j = 0; j < nn && j < (int)c->h_elem_w.size(); ++j) q->ew[j] = c->h_elem_w[j];
   q->k = c->k;
-  q->near_factor = c->near_factor;
+  q->near_factor = near_factor;
 }
 
 // Device matrix (rows x cols, leading dimension ld) -> dense host column-major
@@ -303,7 +303,7 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   c->near_factor = near_factor;
   c->precision = precision;
   QuadConst q;
-  build_quadconst(c, &q);
+  build_quadconst(c, &q, npg, near_factor);
   BEMHIP_HIP_TRY(c, c->qc.ensure(1));
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->qc.ptr, &q, sizeof(q), hipMemcpyHostToDevice, c->stream));
   hipEventRecord(c->ev0, c->stream);
@@ -323,6 +323,64 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   c->ms["assemble"] = ms;
   c->stats["npairs"] = (double)c->npairs;
   c->assembled = true;
+  return BEMHIP_OK;
+}
+
+int bemhip_interior_laplace(bemhip_ctx* c, int npts, const double* pts_xy, const double* T,
+                            const double* q, int npg, double near_factor, double* u) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "interior_laplace: NULL context");
+  if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "interior_laplace: set_geometry first");
+  if (npts < 0) return fail(c, BEMHIP_ERR_ARG, "interior_laplace: npts must be >= 0");
+  if (npg < 1 || npg > kMaxNpg)
+    return fail(c, BEMHIP_ERR_ARG, "interior_laplace: npg must be in 1..64");
+  if (!(near_factor >= 0.0))
+    return fail(c, BEMHIP_ERR_ARG, "interior_laplace: near_factor must be >= 0");
+  if (npts == 0) return BEMHIP_OK;
+  if (!pts_xy || !T || !q || !u) return fail(c, BEMHIP_ERR_ARG, "interior_laplace: NULL buffer");
+  const int n = c->n;
+  std::vector<double> hp(2 * (size_t)npts), hw(4 * (size_t)n);
+  for (int i = 0; i < npts; ++i) {
+    const double x = pts_xy[2 * (size_t)i], y = pts_xy[2 * (size_t)i + 1];
+    if (!std::isfinite(x) || !std::isfinite(y))
+      return fail(c, BEMHIP_ERR_ARG, "interior_laplace: non-finite point coordinate");
+    hp[i] = x;
+    hp[(size_t)npts + i] = y;
+  }
+  for (int j = 0; j < n; ++j) {
+    hw[j] = -T[j];
+    hw[(size_t)n + j] = q[j];
+    hw[2 * (size_t)n + j] = 1.0;
+    hw[3 * (size_t)n + j] = 0.0;
+  }
+  hipSetDevice(c->device);
+  QuadConst qh;
+  build_quadconst(c, &qh, npg, near_factor);
+  BEMHIP_HIP_TRY(c, c->ev_qc.ensure(1));
+  BEMHIP_HIP_TRY(c, c->ev_pts.ensure(hp.size()));
+  BEMHIP_HIP_TRY(c, c->ev_w.ensure(hw.size()));
+  BEMHIP_HIP_TRY(c, c->ev_out.ensure((size_t)npts));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->ev_qc.ptr, &qh, sizeof(qh), hipMemcpyHostToDevice,
+                                   c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->ev_pts.ptr, hp.data(), hp.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->ev_w.ptr, hw.data(), hw.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, c->stream));
+  hipEvent_t t0, t1;
+  BEMHIP_HIP_TRY(c, hipEventCreate(&t0));
+  BEMHIP_HIP_TRY(c, hipEventCreate(&t1));
+  hipEventRecord(t0, c->stream);
+  int st = interior_impl(c, npts, c->ev_pts.ptr, c->ev_pts.ptr + npts, c->ev_w.ptr,
+                         c->ev_qc.ptr, near_factor, c->ev_out.ptr);
+  hipEventRecord(t1, c->stream);
+  hipEventSynchronize(t1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, t0, t1);
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  if (st) return st;
+  BEMHIP_HIP_TRY(c, hipMemcpy(u, c->ev_out.ptr, sizeof(double) * npts, hipMemcpyDeviceToHost));
+  c->ms["interior"] = ms;
+  c->stats["interior_points"] = (double)npts;
   return BEMHIP_OK;
 }
 

@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kFarThreads) void near_classify(Geo g, Consts cs, i
   __shared__ double s_red[4][kFarThreads / 64];
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   const bool ok = i < g.nt;
-  const double px = ok ? g.x[i] : 0.0, py = ok ? g.y[i] : 0.0;
+  const double px = ok ? g.rx[i] : 0.0, py = ok ? g.ry[i] : 0.0;
   // bounding box of this block's rows
   double rlox = ok ? px : INFINITY, rhix = ok ? px : -INFINITY;
   double rloy = ok ? py : INFINITY, rhiy = ok ? py : -INFINITY;
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kFarThreads) void near_classify(Geo g, Consts cs, i
           double rx[NN], ry[NN];
           float rxf[NN], ryf[NN], d2f[NN];
           const int col0 = (e0 + el) * NN;
-          const bool onel = (i >= col0) && (i < col0 + NN);
+          const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
           if (classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el], s_lim2f[el],
                                 cs.nf, onel, rx, ry, rxf, ryf, d2f))
             bits |= 1u << el;
@@ -288,14 +288,14 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;
-  const double px = g.x[i], py = g.y[i];
+  const double px = g.rx[i], py = g.ry[i];
   const size_t ld = g.ld;
   double rowsum = 0.0;
   for (int el = 0; el < ecount; ++el) {
     double rx[NN], ry[NN];
     float rxf[NN], ryf[NN], d2f[NN];
     const int col0 = (e0 + el) * NN;
-    const bool onel = (i >= col0) && (i < col0 + NN);
+    const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
     const bool nr = classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el],
                                       s_lim2f[el], cs.nf, onel, rx, ry, rxf, ryf, d2f);
     TS* Hc = H + (size_t)col0 * ld + i;
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;
-  const double px = g.x[i], py = g.y[i];
+  const double px = g.rx[i], py = g.ry[i];
   double acc1 = 0.0, acc2 = 0.0;
   for (int el = 0; el < ecount; ++el) {
     const double* cx = s_cx + el * NN;
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
       d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
     }
     const int col0 = (e0 + el) * NN;
-    const bool onel = (i >= col0) && (i < col0 + NN);
+    const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
     if (onel) continue;
     if (!clearly_far<NN>(rxf, ryf, d2f, s_lim2f[el]) &&
         is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf))
@@ -858,7 +858,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
   const int gshift = threadIdx.x & kGroup;  // 0 or 32: bit offset of this group in a ballot
   const int i = prow[pair];
   const int e = pelem[pair];
-  const double px = g.x[i], py = g.y[i];
+  const double px = g.rx[i], py = g.ry[i];
   double cx[NN], cy[NN], gxx[NN], gyy[NN];
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
   const double cU = 0.5 * (1.0 / (2.0 * kPi)) / qc->k;
   const double cT = 1.0 / (2.0 * kPi);
   double hs[NN], gs[NN];
-  const bool singular = (i >= e * NN) && (i < (e + 1) * NN);
+  const bool singular = (i + g.row0 >= e * NN) && (i + g.row0 < (e + 1) * NN);
 
   if (!singular) {
     double zeta, eta;
@@ -1122,6 +1122,7 @@ Geo make_geo(bemhip_ctx* c) {
   g.p = c->p; g.nn = c->nn; g.ne = c->ne; g.n = c->n; g.ni = c->ni; g.nt = c->nt;
   g.x = c->x.ptr; g.y = c->y.ptr; g.nx = c->nx.ptr; g.ny = c->ny.ptr; g.jw = c->jw.ptr;
   g.elen = c->elen.ptr; g.gx = c->gx.ptr; g.gy = c->gy.ptr;
+  g.rx = c->x.ptr; g.ry = c->y.ptr; g.row0 = 0;
   g.ld = (c->precision == BEMHIP_FP64) ? c->ld64 : c->ld32;
   return g;
 }
@@ -1367,6 +1368,160 @@ int apply_bc_impl(bemhip_ctx* c) {
     }
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
+  return BEMHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Potential at points inside the domain (src/Laplace/Lubrication.jl:399-417;
+// the same formula in the reference's tests, test/laplace.jl:36-53):
+//   u(p) = (sum_j g_j q_j - sum_j h_j T_j) / c,   c = -sum_j h_j,
+// h_j, g_j the integrals of T*, U* times the shape function of boundary node
+// j over its element, seen from p. Row sums per point: s1 = sum_j h_j w[0][j]
+// + g_j w[1][j] (w[0] = -T, w[1] = q) and s2 = sum_j h_j; u = s1 / (-s2).
+// part: [2][ncb][npts] far partial sums (null when every pair is near).
+// ---------------------------------------------------------------------------
+__global__ void interior_finalize(int npts, int n, int nn, int ncb,
+                                  const double* __restrict__ part,
+                                  const int* __restrict__ row_ptr,
+                                  const int* __restrict__ pair_elem,
+                                  const double* __restrict__ ph,
+                                  const double* __restrict__ pg,
+                                  const double* __restrict__ w, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  double s1 = 0.0, s2 = 0.0;
+  if (part) {
+    const size_t plane = (size_t)ncb * npts;
+    for (int cb = 0; cb < ncb; ++cb) {
+      s1 += part[(size_t)cb * npts + i];
+      s2 += part[plane + (size_t)cb * npts + i];
+    }
+  }
+  const double* wT = w;
+  const double* wq = w + n;
+  for (int pr = row_ptr[i]; pr < row_ptr[i + 1]; ++pr) {
+    const int c0 = pair_elem[pr] * nn;
+    for (int k = 0; k < nn; ++k) {
+      const double h = ph[(size_t)pr * nn + k];
+      s1 += h * wT[c0 + k] + pg[(size_t)pr * nn + k] * wq[c0 + k];
+      s2 += h;
+    }
+  }
+  out[i] = s1 / (-s2);
+}
+
+template <int NN>
+static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const QuadConst* d_qc,
+                          const double* d_w, bool all_near, double* d_out, hipEvent_t start,
+                          hipEvent_t far_done, int64_t* npairs_out) {
+  const int npts = g.nt;
+  const int ncb = (g.ne + kEB - 1) / kEB;
+  const int nrb = (npts + kFarThreads - 1) / kFarThreads;
+  hipStream_t s1 = c->stream, s2 = c->stream2;
+  BEMHIP_HIP_TRY(c, hipEventRecord(start, s1));
+  // far field: FP64 contraction of the far pairs with (-T, q) and (1, 0), stream 2
+  if (!all_near) {
+    BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, start, 0));
+    double* part = c->ev_part.ptr;
+    dim3 grid((unsigned)nrb, (unsigned)ncb);
+    const int n = g.n;
+    hipLaunchKernelGGL(op64_kernel<NN>, grid, dim3(kFarThreads), 0, s2, g, cs, 2, d_w, d_w + n,
+                       d_w + 2 * (size_t)n, d_w + 3 * (size_t)n, part,
+                       part + (size_t)ncb * npts);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    BEMHIP_HIP_TRY(c, hipEventRecord(far_done, s2));
+  }
+  // near pairs: classification, CSR, near rule (stream 1)
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(c->ev_cnt.ptr, 0, sizeof(int) * npts, s1));
+  const int cpb = 32;
+  dim3 cgrid((unsigned)nrb, (unsigned)((ncb + cpb - 1) / cpb));
+  hipLaunchKernelGGL((near_classify<NN>), cgrid, dim3(kFarThreads), 0, s1, g, cs, ncb, cpb,
+                     c->ev_box.ptr, c->ev_bits.ptr, c->ev_cnt.ptr, c->ev_tiles.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, s1, npts, c->ev_cnt.ptr, c->ev_ptr.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->h_count, c->ev_ptr.ptr + npts, sizeof(int),
+                                   hipMemcpyDeviceToHost, s1));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(s1));
+  const int total = *c->h_count;
+  const size_t cap = (size_t)std::max(total, 1);
+  BEMHIP_HIP_TRY(c, c->ev_prow.ensure(cap));
+  BEMHIP_HIP_TRY(c, c->ev_pelem.ensure(cap));
+  BEMHIP_HIP_TRY(c, c->ev_ph.ensure(cap * NN));
+  BEMHIP_HIP_TRY(c, c->ev_pg.ensure(cap * NN));
+  hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, s1, npts, ncb, c->ev_bits.ptr,
+                     c->ev_tiles.ptr, c->ev_ptr.ptr, c->ev_prow.ptr, c->ev_pelem.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  if (total > 0) {
+    const int pairs_per_block = 2 * (256 / kWave);
+    const int64_t blocks = ((int64_t)total + pairs_per_block - 1) / pairs_per_block;
+    hipLaunchKernelGGL((near_kernel<double, NN>), dim3((unsigned)blocks), dim3(256), 0, s1, g,
+                       d_qc, (int64_t)total, c->ev_prow.ptr, c->ev_pelem.ptr, c->ev_ph.ptr,
+                       c->ev_pg.ptr, (double*)nullptr, (double*)nullptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
+  if (!all_near) BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s1, far_done, 0));
+  const int tb = 256;
+  hipLaunchKernelGGL(interior_finalize, dim3((npts + tb - 1) / tb), dim3(tb), 0, s1, npts, g.n,
+                     NN, ncb, all_near ? (const double*)nullptr : c->ev_part.ptr, c->ev_ptr.ptr,
+                     c->ev_pelem.ptr, c->ev_ph.ptr, c->ev_pg.ptr, d_w, d_out);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  *npairs_out += total;
+  return BEMHIP_OK;
+}
+
+// Points in chunks: the pair counts are 32-bit (at most npts x ne pairs per
+// chunk) and an all-near chunk keeps its pair list below 2^26 pairs.
+int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
+                  const double* d_w, const QuadConst* d_qc, double near_factor, double* d_out) {
+  if (npts == 0) return BEMHIP_OK;
+  Geo g = make_geo(c);
+  g.row0 = c->nt;  // no point row is a node of an element
+  Consts cs = make_consts(c);
+  cs.nf = near_factor;
+  const bool all_near = near_factor > 1e20;
+  const int ne = c->ne;
+  const int ncb = (ne + kEB - 1) / kEB;
+  int64_t chunk = std::min<int64_t>(npts, 65536);
+  chunk = std::min<int64_t>(chunk, std::max<int64_t>(1, (int64_t)INT32_MAX / ne - 1));
+  if (all_near) chunk = std::min<int64_t>(chunk, std::max<int64_t>(1, ((int64_t)1 << 26) / ne));
+  const int nrb = (int)((chunk + kFarThreads - 1) / kFarThreads);
+  BEMHIP_HIP_TRY(c, c->ev_box.ensure((size_t)ncb * 5));
+  BEMHIP_HIP_TRY(c, c->ev_bits.ensure((size_t)ncb * chunk));
+  BEMHIP_HIP_TRY(c, c->ev_cnt.ensure((size_t)chunk));
+  BEMHIP_HIP_TRY(c, c->ev_ptr.ensure((size_t)chunk + 1));
+  BEMHIP_HIP_TRY(c, c->ev_tiles.ensure((size_t)nrb * ncb));
+  if (!all_near) BEMHIP_HIP_TRY(c, c->ev_part.ensure(2 * (size_t)ncb * chunk));
+  if (!c->h_count) BEMHIP_HIP_TRY(c, hipHostMalloc((void**)&c->h_count, sizeof(int)));
+  if (c->nn == 2)
+    hipLaunchKernelGGL((elem_block_boxes<2>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+                       cs, ncb, c->ev_box.ptr);
+  else
+    hipLaunchKernelGGL((elem_block_boxes<3>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+                       cs, ncb, c->ev_box.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  hipEvent_t ev[2];
+  for (auto& e : ev) BEMHIP_HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  struct Destroy {
+    hipEvent_t* e;
+    ~Destroy() { hipEventDestroy(e[0]); hipEventDestroy(e[1]); }
+  } destroy{ev};
+  int64_t npairs = 0;
+  for (int64_t p0 = 0; p0 < npts; p0 += chunk) {
+    const int m = (int)std::min<int64_t>(chunk, npts - p0);
+    Geo gc = g;
+    gc.nt = m;
+    gc.rx = d_px + p0;
+    gc.ry = d_py + p0;
+    const int st = (c->nn == 2)
+                       ? interior_chunk<2>(c, gc, cs, d_qc, d_w, all_near, d_out + p0, ev[0],
+                                           ev[1], &npairs)
+                       : interior_chunk<3>(c, gc, cs, d_qc, d_w, all_near, d_out + p0, ev[0],
+                                           ev[1], &npairs);
+    if (st) return st;
+  }
+  c->stats["interior_pairs"] = (double)npairs;
+  c->stats["interior_chunk"] = (double)chunk;
   return BEMHIP_OK;
 }
 

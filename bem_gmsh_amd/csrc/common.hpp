@@ -26,6 +26,13 @@ struct Geo {
   const double* elen;  // ne element lengths
   const double* gx;    // ne*nn geometric nodes (equispaced order)
   const double* gy;
+  // Rows: points p_i = (rx[i], ry[i]), i < nt. The assembly's rows are the
+  // collocation points (rx = x, row0 = 0); an evaluation at other points sets
+  // rx, ry to them and row0 >= n, so that no row lies on an element. Row i is
+  // the source of column j's element when row0 + i is one of its nodes.
+  const double* rx;
+  const double* ry;
+  int row0;
 };
 
 // Leading dimension of the device-resident matrices: the smallest multiple of
@@ -125,6 +132,18 @@ struct bemhip_ctx {
   bemhip::DevBuf<double> Jn;        // n |dx/dxi| at each boundary node
   bemhip::DevBuf<double> adj_work;  // adjoint scratch (carved per call)
 
+  // interior evaluation (points, weights, its own near-pair list and rule)
+  bemhip::DevBuf<bemhip::QuadConst> ev_qc;
+  bemhip::DevBuf<double> ev_pts;    // [2][npts] x then y
+  bemhip::DevBuf<double> ev_w;      // [4][n] -T, q, 1, 0
+  bemhip::DevBuf<double> ev_out;    // [npts]
+  bemhip::DevBuf<double> ev_box;    // [ncb][5]
+  bemhip::DevBuf<uint32_t> ev_bits; // [ncb][chunk]
+  bemhip::DevBuf<int> ev_cnt, ev_ptr, ev_prow, ev_pelem;
+  bemhip::DevBuf<uint8_t> ev_tiles;
+  bemhip::DevBuf<double> ev_ph, ev_pg;  // [pairs][nn]
+  bemhip::DevBuf<double> ev_part;       // [2][ncb][chunk]
+
   // scratch for LU and solves
   bemhip::DevBuf<char> lu_work;
   unsigned panel_epoch = 0;           // tag base of the LU panel exchange words
@@ -184,6 +203,8 @@ int lu_solve_trans_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int 
                           const int* dperm, double* d_b);
 int build_perm(bemhip_ctx* c, int n, const int* dipiv, int* dperm);
 int operator_apply_trans(bemhip_ctx* c, const double* d_v, double* d_y);
+int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
+                  const double* d_w, const QuadConst* d_qc, double near_factor, double* d_out);
 
 // Small host helpers
 void gauss_legendre_host(int n, double* x, double* w);

@@ -98,6 +98,15 @@ class Ctx:
         self.chk(self.L.bemhip_synchronize(self.h))
         return x, it.value, rel.value
 
+    def interior(self, pts, T, q, npg, near_factor):
+        P = np.ascontiguousarray(pts, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        u = np.empty(P.shape[0])
+        self.chk(self.L.bemhip_interior_laplace(self.h, P.shape[0], dptr(P), dptr(T), dptr(q),
+                                                npg, near_factor, dptr(u)))
+        return u
+
     def close(self):
         self.L.bemhip_destroy(self.h)
 
@@ -176,6 +185,32 @@ def adjoint_leg(ne=1365, seed=1, npg=12):
     return out
 
 
+def interior_leg(ctx, dad, npg, T, q, m_far=65536, m_near=2048, seed=3):
+    """Potential at random points inside the star from the boundary solution:
+    m_far points with the assembly's rules (far lumping beyond 1.5 L) and m_near
+    points with every (point, element) pair through the near rule. Times from
+    HIP events around the library call; accuracy against the exact field."""
+    rng = np.random.default_rng(seed)
+    ana = B.ana_exp_cos()
+    out = {}
+    for name, m, nf in (("assembly_rules", m_far, 1.5), ("all_near", m_near, 1e300)):
+        th = rng.uniform(0, 2 * np.pi, m)
+        r = 0.85 * np.sqrt(rng.uniform(0, 1, m))
+        pts = np.stack([r * np.cos(th), r * np.sin(th)], axis=1)
+        ctx.interior(pts[:64], T, q, npg, nf)                    # warm-up
+        u = ctx.interior(pts, T, q, npg, nf)
+        c = ctx.counters()
+        ms = c["ms"]["interior"]
+        pairs = float(m) * dad.ne
+        out[name] = {"points": m, "ms": ms, "points_per_s": m / (ms * 1e-3),
+                     "pair_integrals_per_s": pairs / (ms * 1e-3),
+                     "near_pairs": c["stats"]["interior_pairs"],
+                     "max_abs_err_vs_exact": float(np.abs(u - ana.u(pts)).max())}
+        log(f"[bench] interior {name}: {m} points in {ms:.2f} ms, "
+            f"err vs exact {out[name]['max_abs_err_vs_exact']:.1e}")
+    return out
+
+
 def write_ceiling(ne):
     """Practical HBM write bandwidth: a write-only fill (torch's fill kernel) of
     the bytes the far-field kernel writes, 2 nt n FP32 entries."""
@@ -207,6 +242,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-interior", action="store_true")
     ap.add_argument("--phase", choices=["all", "assemble"], default="all",
                     help="'assemble': only assembly + BC passes (for counter runs)")
     ap.add_argument("--write-ceiling", action="store_true",
@@ -301,6 +337,9 @@ def main():
                        "lu64_gemm_ms": c64.get("lu64_gemm")})
         log(f"[bench] fp64 pipeline: assemble {c64['assemble']:.2f} ms, lu64 {c64['lu64']:.1f} ms;"
             f" mixed vs fp64: T {eT:.2e} q {eq:.2e}")
+    if not args.no_interior:
+        Tm, qm = B.split_sol(dad, x_mixed)
+        detail["interior"] = interior_leg(ctx, dad, args.npg, Tm[:n], qm)
     ctx.close()
 
     if not args.no_adjoint:

@@ -170,6 +170,28 @@ int bemhip_lu_solve(bemhip_ctx* ctx, int n, int dtype, const void* LU, const int
 int bemhip_adjoint(bemhip_ctx* ctx, const double* q_target, const double* w, double* dJ_dgeo,
                    double* J);
 
+/* Potential at points inside the domain from a boundary solution. Replaces
+ * interior_laplace (src/Laplace/Lubrication.jl:399-417), whose formula the
+ * reference's tests use as _interior_T (test/laplace.jl:36-53):
+ *   u(p) = (sum_j g_j(p) q_j - sum_j h_j(p) T_j) / c(p),   c(p) = -sum_j h_j(p)
+ * over the n boundary nodes, h_j, g_j the integrals of T*, U* times the shape
+ * function of node j over its element, seen from p.
+ *   npts, pts_xy  points (npts x 2, interleaved like coll_xy); a point must not
+ *                 lie on an element (there the reference switches to its
+ *                 singular rule; this call does not)
+ *   T, q          (n) boundary potential and flux (dad.T[1:n], dad.q[1:n])
+ *   npg           Gauss points of the near rule (1..64, as bemhip_assemble)
+ *   near_factor   > 1e20: every (point, element) pair through the tan-p3c near
+ *                 rule, as interior_laplace does (integrate_element,
+ *                 src/Core/Assembly_full.jl:262-313); otherwise the assembly's
+ *                 rules with that cutoff (far nodal lumping), i.e. the value
+ *                 of an internal row of H, G (src/Core/Assembly_full.jl:61-85)
+ *   u             (out, npts)
+ * Needs the geometry only; leaves the assembled system, factors and solution
+ * of the context untouched. */
+int bemhip_interior_laplace(bemhip_ctx* ctx, int npts, const double* pts_xy, const double* T,
+                            const double* q, int npg, double near_factor, double* u);
+
 /* Timings (ms, HIP events on the context's streams) and sizes of the last calls
  * as a JSON object. */
 int bemhip_counters(bemhip_ctx* ctx, char* json, size_t cap);

@@ -177,6 +177,31 @@ def assemble(dad, npg=20, near_factor=1.5, nthreads=0, rows=None):
     return H, G
 
 
+def interior_laplace(dad, pts, T, q, npg=20, near_factor=1e300):
+    """u(p) = (sum_j g_j q_j - sum_j h_j T_j) / c, c = -sum_j h_j, at points p
+    (src/Laplace/Lubrication.jl:399-417; test/laplace.jl:36-53): the rows of
+    an assembly with the points as internal nodes (far lumping beyond
+    near_factor * L, every pair through the near rule when near_factor > 1e20)."""
+    coll, nrm, J, ew, geo, L = _geom_args(dad)
+    pts = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(-1, 2))
+    n, m = dad.n, pts.shape[0]
+    coll = np.ascontiguousarray(np.concatenate([coll[:n], pts], axis=0))
+    nt = n + m
+    H = np.zeros((nt, nt), order="F")
+    G = np.zeros((nt, n), order="F")
+    st = lib().oracle_assemble(dad.p, dad.ne, n, m, _p(coll), _p(nrm), _p(J), _p(ew), _p(geo),
+                               _p(L), float(dad.properties.k), int(npg), float(near_factor), 0,
+                               n, nt, H.ctypes.data_as(dp), G.ctypes.data_as(dp))
+    if st != 0:
+        raise ValueError("oracle_assemble: bad arguments")
+    h = H[n:, :n]
+    g = G[n:, :]
+    c = -h.sum(axis=1)
+    T = np.asarray(T, dtype=np.float64)[:n]
+    q = np.asarray(q, dtype=np.float64)[:n]
+    return (g @ q - h @ T) / c
+
+
 def pair(dad, i, e, npg=20):
     coll, nrm, J, ew, geo, L = _geom_args(dad)
     nn = dad.p + 1
