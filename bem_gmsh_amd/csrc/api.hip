@@ -370,7 +370,7 @@ int bemhip_interior_laplace(bemhip_ctx* c, int npts, const double* pts_xy, const
   BEMHIP_HIP_TRY(c, hipEventCreate(&t1));
   hipEventRecord(t0, c->stream);
   int st = interior_impl(c, npts, c->ev_pts.ptr, c->ev_pts.ptr + npts, c->ev_w.ptr,
-                         c->ev_qc.ptr, near_factor, c->ev_out.ptr);
+                         c->ev_qc.ptr, npg, near_factor, c->ev_out.ptr);
   hipEventRecord(t1, c->stream);
   hipEventSynchronize(t1);
   float ms = 0.f;

@@ -9,11 +9,13 @@
 //                   and the far nodal lumping (:801-810) written column-major
 //                   with lanes over rows (coalesced), FP64 row sums.
 //   near_scan / near_fill   counts and near bits -> row-sorted CSR of pairs.
-//   near_kernel     one wavefront per near pair: tan-p3c rule for off-element
-//                   pairs (:649-695, src/Core/Integration.jl:68-93, :1117-1365),
-//                   interpolated Guiggiani for on-element pairs (:397-424,
+//   near_reg_kernel / near_sing_kernel   one lane group per near pair:
+//                   tan-p3c rule for off-element pairs (:649-695,
+//                   src/Core/Integration.jl:68-93, :1117-1365), interpolated
+//                   Guiggiani for on-element pairs (:397-424,
 //                   src/Core/Integration.jl:1597-1845); lanes over quadrature
-//                   points, wave reductions.
+//                   points, group reductions; the two kinds in separate
+//                   launches.
 //   op64_finalize   rigid-body diagonal H_ii = -sum_{j!=i} H_ij (:384-390) and
 //                   the near / on-element entries into H and G.
 //   op64_kernel / op64_finalize   FP64 operator passes that recompute the far
@@ -502,24 +504,40 @@ __global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict_
 }
 
 // One thread per row, in the row blocks of near_classify: only the column
-// blocks that took the pair tests can hold near bits.
-__global__ __launch_bounds__(kFarThreads) void near_fill(int nt, int nwords,
-                                                         const uint32_t* __restrict__ bits,
+// blocks that took the pair tests can hold near bits. The pairs are also
+// listed by kind: sing[i] = CSR index of the on-element pair of row i (the
+// rows whose global index row0 + i is a boundary node, one such pair each,
+// i < nsing) and reg[] = CSR indices of the other pairs in order (null: not
+// written).
+__global__ __launch_bounds__(kFarThreads) void near_fill(int nt, int nwords, int nn, int row0,
+                                                         int n, const uint32_t* __restrict__ bits,
                                                          const uint8_t* __restrict__ tiles,
                                                          const int* __restrict__ ptr,
                                                          int* __restrict__ prow,
-                                                         int* __restrict__ pelem) {
+                                                         int* __restrict__ pelem,
+                                                         int* __restrict__ sing,
+                                                         int* __restrict__ reg) {
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   const uint8_t* tl = tiles + (size_t)blockIdx.x * nwords;
+  const int nsing = max(0, min(nt, n - row0));
+  const int own = (i < nsing) ? (i + row0) / nn : -1;  // element holding row i's node
   int off = (i < nt) ? ptr[i] : 0;
+  int before = min(i, nsing);  // on-element pairs ahead of this row's next pair
   for (int w = 0; w < nwords; ++w) {
     if (!tl[w]) continue;  // uniform over the block
     uint32_t b = (i < nt) ? bits[(size_t)w * nt + i] : 0u;
     while (b) {
       const int t = __ffs(b) - 1;
       b &= b - 1u;
+      const int e = w * 32 + t;
       prow[off] = i;
-      pelem[off] = w * 32 + t;
+      pelem[off] = e;
+      if (e == own) {
+        sing[i] = off;
+        ++before;
+      } else if (reg) {
+        reg[off - before] = off;
+      }
       ++off;
     }
   }
@@ -830,32 +848,39 @@ __device__ void tanp3c_node(const TanPlan& P, int q, int npg, const double* u, c
 }
 
 // ---------------------------------------------------------------------------
-// near_kernel: one wavefront per near pair.
+// Near-pair integrals, one group of lanes per pair. near_fill lists the two
+// kinds of pairs apart, so that no wavefront runs both rules:
+//   near_reg_kernel<NN, G>  off-element pairs: tan-p3c rule over npg points
+//                           (2 npg when split), G lanes per pair (G = 16, 32
+//                           or 64, at least npg), 64 / G pairs per wavefront;
+//   near_sing_kernel<NN>    on-element pairs: interpolated Guiggiani over 20
+//                           nodes, 32 lanes per pair, two pairs per wavefront.
+// Lanes run over quadrature points; the values go to the pair list at the
+// pair's CSR index. list == nullptr: the k-th pair is CSR index k.
 // ---------------------------------------------------------------------------
-// Sum over the 32 lanes of a half-wavefront group (xor offsets < 32 stay in
-// the group).
-constexpr int kGroup = 32;
+// Sum over the G lanes of a group (xor offsets < G stay in the group).
+template <int G>
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
-  for (int off = kGroup / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
   return v;
 }
 
-template <typename TS, int NN>
-__global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __restrict__ qc,
-                                                   int64_t npairs, const int* __restrict__ prow,
-                                                   const int* __restrict__ pelem,
-                                                   double* __restrict__ ph,
-                                                   double* __restrict__ pg, TS* __restrict__ H,
-                                                   TS* __restrict__ G) {
-  // two pairs per wavefront: lanes 0-31 and 32-63 (kGroup lanes each)
-  const int64_t pair_raw =
-      ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * 2 +
-      ((threadIdx.x & (kWave - 1)) >> 5);
-  const bool active = pair_raw < npairs;
-  const int64_t pair = active ? pair_raw : npairs - 1;  // idle groups redo the last pair
-  const int lane = threadIdx.x & (kGroup - 1);
-  const int gshift = threadIdx.x & kGroup;  // 0 or 32: bit offset of this group in a ballot
+template <int NN, int G>
+__global__ __launch_bounds__(256) void near_reg_kernel(Geo g, const QuadConst* __restrict__ qc,
+                                                       int64_t count,
+                                                       const int* __restrict__ list,
+                                                       const int* __restrict__ prow,
+                                                       const int* __restrict__ pelem,
+                                                       double* __restrict__ ph,
+                                                       double* __restrict__ pg) {
+  const int64_t k_raw =
+      ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * (kWave / G) +
+      (threadIdx.x & (kWave - 1)) / G;
+  const bool active = k_raw < count;
+  const int64_t kk = active ? k_raw : count - 1;  // idle groups redo the last pair
+  const int64_t pair = list ? (int64_t)list[kk] : kk;
+  const int lane = threadIdx.x & (G - 1);
   const int i = prow[pair];
   const int e = pelem[pair];
   const double px = g.rx[i], py = g.ry[i];
@@ -875,9 +900,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
   const double cU = 0.5 * (1.0 / (2.0 * kPi)) / qc->k;
   const double cT = 1.0 / (2.0 * kPi);
   double hs[NN], gs[NN];
-  const bool singular = (i + g.row0 >= e * NN) && (i + g.row0 < (e + 1) * NN);
-
-  if (!singular) {
+  {
     double zeta, eta;
     complex_pole<NN>(lx, cx, cy, px, py, zeta, eta);
     TanPlan P;
@@ -901,7 +924,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     double ha[NN], ga[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) { ha[k] = 0.0; ga[k] = 0.0; }
-    for (int q0 = 0; q0 < nq; q0 += kGroup) {
+    for (int q0 = 0; q0 < nq; q0 += G) {
       const int q = q0 + lane;
       if (q < nq) {
         double xq, wq;
@@ -932,8 +955,55 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
       }
     }
 #pragma unroll
-    for (int k = 0; k < NN; ++k) { hs[k] = group_sum(ha[k]); gs[k] = group_sum(ga[k]); }
-  } else {
+    for (int k = 0; k < NN; ++k) { hs[k] = group_sum<G>(ha[k]); gs[k] = group_sum<G>(ga[k]); }
+  }
+  if (active && lane < NN) {
+    double hv = hs[0], gv = gs[0];
+#pragma unroll
+    for (int k = 1; k < NN; ++k)
+      if (lane == k) { hv = hs[k]; gv = gs[k]; }
+    ph[(size_t)pair * NN + lane] = hv;
+    pg[(size_t)pair * NN + lane] = gv;
+  }
+}
+
+template <int NN>
+__global__ __launch_bounds__(256) void near_sing_kernel(Geo g, const QuadConst* __restrict__ qc,
+                                                        int64_t count,
+                                                        const int* __restrict__ list,
+                                                        const int* __restrict__ prow,
+                                                        const int* __restrict__ pelem,
+                                                        double* __restrict__ ph,
+                                                        double* __restrict__ pg) {
+  constexpr int G = 32;
+  const int64_t k_raw =
+      ((int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * 2 +
+      ((threadIdx.x & (kWave - 1)) >> 5);
+  const bool active = k_raw < count;
+  const int64_t kk = active ? k_raw : count - 1;  // idle groups redo the last pair
+  const int64_t pair = list ? (int64_t)list[kk] : kk;
+  const int lane = threadIdx.x & (G - 1);
+  const int gshift = threadIdx.x & G;  // 0 or 32: bit offset of this group in a ballot
+  const int i = prow[pair];
+  const int e = pelem[pair];
+  const double px = g.rx[i], py = g.ry[i];
+  double cx[NN], cy[NN], gxx[NN], gyy[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    cx[k] = g.x[e * NN + k]; cy[k] = g.y[e * NN + k];
+    gxx[k] = g.gx[e * NN + k]; gyy[k] = g.gy[e * NN + k];
+  }
+  const double nrefx = g.nx[e * NN], nrefy = g.ny[e * NN];
+  const double* lx = qc->lp_x;
+  const double* lbw = qc->lp_bw;
+  const double* lD = qc->lp_D;
+  const double* ex = qc->eq_x;
+  const double* ebw = qc->eq_bw;
+  const double* eD = qc->eq_D;
+  const double cU = 0.5 * (1.0 / (2.0 * kPi)) / qc->k;
+  const double cT = 1.0 / (2.0 * kPi);
+  double hs[NN], gs[NN];
+  {
     // closest point on the collocation interpolant (src/Core/Integration.jl:323-349)
     double a;
     {
@@ -1006,7 +1076,7 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     }
     const double dm = a - xm;
     const double bnum = keep ? bw / dm : 0.0;
-    const double msum = group_sum(bnum);
+    const double msum = group_sum<G>(bnum);
     const double Ni = keep ? bnum * (1.0 / msum) : 0.0;
     const double delta = xm - a;
     // kernel samples F = (U, T) * N_k * J  (src/Core/Assembly_full.jl:629-643)
@@ -1042,8 +1112,8 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     // log-coefficient fit for G (order 0) and CPV coefficient for H (order -1)
     const bool used = keep && !(fabs(delta) < 1e-14);
     const double lg = used ? log(fabs(delta)) : 0.0;
-    const double sL = group_sum(lg);
-    const double sL2 = group_sum(lg * lg);
+    const double sL = group_sum<G>(lg);
+    const double sL2 = group_sum<G>(lg * lg);
     const int nused = __popc((uint32_t)(__ballot(used) >> gshift));
     const double detA = sL2 * nused - sL * sL;
     const double wq = keep ? qc->g20_w[lane] : 0.0;
@@ -1052,17 +1122,17 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
     const double lgd = keep ? log(fabs(delta)) : 0.0;
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
-      const double Fm1g = group_sum(keep ? Ni * (Fg[k] * delta) : 0.0);
-      const double F1h = group_sum(keep ? Ni * (Fh[k] * delta) : 0.0);
+      const double Fm1g = group_sum<G>(keep ? Ni * (Fg[k] * delta) : 0.0);
+      const double F1h = group_sum<G>(keep ? Ni * (Fh[k] * delta) : 0.0);
       const double wi = keep ? Fg[k] - Fm1g / delta : 0.0;
-      const double sW = group_sum(used ? wi : 0.0);
-      const double sWL = group_sum(used ? wi * lg : 0.0);
+      const double sW = group_sum<G>(used ? wi : 0.0);
+      const double sWL = group_sum<G>(used ? wi * lg : 0.0);
       double F0 = 0.0;
       if (nused >= 2 && !(fabs(detA) < 1e-30)) F0 = (nused * sWL - sL * sW) / detA;
       const double regg = keep ? wq * ((Fg[k] - Fm1g / delta) - F0 * lgd) : 0.0;
       const double regh = keep ? wq * (Fh[k] - F1h / delta) : 0.0;
-      gs[k] = group_sum(regg) + (Fm1g * hfp1 + F0 * logp);
-      hs[k] = group_sum(regh) + F1h * hfp1;
+      gs[k] = group_sum<G>(regg) + (Fm1g * hfp1 + F0 * logp);
+      hs[k] = group_sum<G>(regh) + F1h * hfp1;
     }
   }
   if (active && lane < NN) {
@@ -1072,11 +1142,6 @@ __global__ __launch_bounds__(256) void near_kernel(Geo g, const QuadConst* __res
       if (lane == k) { hv = hs[k]; gv = gs[k]; }
     ph[(size_t)pair * NN + lane] = hv;
     pg[(size_t)pair * NN + lane] = gv;
-    if (H) {  // null: the values stay in the pair list (written to H, G later)
-      const size_t col = (size_t)e * NN + lane;
-      H[i + col * g.ld] = TS(hv);
-      G[i + col * g.ld] = TS(gv);
-    }
   }
 }
 
@@ -1103,14 +1168,33 @@ int launch_far(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Consts& cs
   return BEMHIP_OK;
 }
 
-template <typename TS, int NN>
-int launch_near(bemhip_ctx* c, hipStream_t stream, const Geo& g, TS* H, TS* G) {
-  if (c->npairs == 0) return BEMHIP_OK;
-  const int pairs_per_block = 2 * (256 / kWave);  // two pairs per wavefront
-  const int64_t blocks = (c->npairs + pairs_per_block - 1) / pairs_per_block;
-  hipLaunchKernelGGL((near_kernel<TS, NN>), dim3((unsigned)blocks), dim3(256), 0, stream, g,
-                     c->qc.ptr, c->npairs, c->pair_row.ptr, c->pair_elem.ptr, c->pair_h.ptr,
-                     c->pair_g.ptr, H, G);
+// Near-pair integrals of a CSR pair list: the on-element pairs (sing, nsing)
+// and the others (reg, nreg; null = CSR order) in launches of their own.
+template <int NN>
+int launch_near_pairs(bemhip_ctx* c, hipStream_t stream, const Geo& g, const QuadConst* qc,
+                      int npg, int64_t nsing, const int* sing, int64_t nreg, const int* reg,
+                      const int* prow, const int* pelem, double* ph, double* pg) {
+  constexpr int waves = 256 / kWave;
+  if (nsing > 0) {
+    const int64_t blocks = (nsing + 2 * waves - 1) / (2 * waves);
+    hipLaunchKernelGGL((near_sing_kernel<NN>), dim3((unsigned)blocks), dim3(256), 0, stream, g,
+                       qc, nsing, sing, prow, pelem, ph, pg);
+  }
+  if (nreg > 0) {
+    if (npg <= 16) {
+      const int64_t blocks = (nreg + 4 * waves - 1) / (4 * waves);
+      hipLaunchKernelGGL((near_reg_kernel<NN, 16>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                         g, qc, nreg, reg, prow, pelem, ph, pg);
+    } else if (npg <= 32) {
+      const int64_t blocks = (nreg + 2 * waves - 1) / (2 * waves);
+      hipLaunchKernelGGL((near_reg_kernel<NN, 32>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                         g, qc, nreg, reg, prow, pelem, ph, pg);
+    } else {
+      const int64_t blocks = (nreg + waves - 1) / waves;
+      hipLaunchKernelGGL((near_reg_kernel<NN, 64>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                         g, qc, nreg, reg, prow, pelem, ph, pg);
+    }
+  }
   BEMHIP_HIP_TRY(c, hipGetLastError());
   return BEMHIP_OK;
 }
@@ -1137,12 +1221,12 @@ static Consts make_consts(const bemhip_ctx* c) {
 }
 
 // Full assembly on two streams:
-//   stream 1: near_classify -> near-pair CSR -> near_kernel --+-> finalize
-//   stream 2:               \-> far_kernel -------------------/
+//   stream 1: near_classify -> near-pair CSR -> near_sing / near_reg --+-> finalize
+//   stream 2:               \-> far_kernel ----------------------------/
 // The far field (HBM-write bound) and the near/singular integrals (FP64 VALU
-// bound) run concurrently. far_kernel writes 0 into the near entries, so
-// near_kernel keeps its values in the pair list and the finalize pass, after
-// both, writes them into H and G together with the rigid-body diagonal.
+// bound) run concurrently. far_kernel writes 0 into the near entries, so the
+// near kernels keep their values in the pair list and the finalize pass,
+// after both, writes them into H and G together with the rigid-body diagonal.
 int assemble_impl(bemhip_ctx* c) {
   const Geo g = make_geo(c);
   const Consts cs = make_consts(c);
@@ -1220,21 +1304,25 @@ int assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, c->pair_elem.ensure((size_t)std::max(total, 1)));
   BEMHIP_HIP_TRY(c, c->pair_h.ensure((size_t)std::max(total, 1) * c->nn));
   BEMHIP_HIP_TRY(c, c->pair_g.ensure((size_t)std::max(total, 1) * c->nn));
-  hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, c->stream, (int)nt, ncb,
-                     c->nearbits.ptr, c->tiles.ptr, c->row_ptr.ptr, c->pair_row.ptr,
-                     c->pair_elem.ptr);
+  const int64_t nsing = std::min<int64_t>(nt, c->n);  // one on-element pair per boundary row
+  const int64_t nreg = (int64_t)total - nsing;
+  BEMHIP_HIP_TRY(c, c->sing_idx.ensure((size_t)std::max<int64_t>(nsing, 1)));
+  BEMHIP_HIP_TRY(c, c->reg_idx.ensure((size_t)std::max<int64_t>(nreg, 1)));
+  hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, c->stream, (int)nt, ncb, c->nn,
+                     0, c->n, c->nearbits.ptr, c->tiles.ptr, c->row_ptr.ptr, c->pair_row.ptr,
+                     c->pair_elem.ptr, c->sing_idx.ptr, c->reg_idx.ptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[4], c->stream));
-  // Near entries: far_kernel writes 0 into the near entries of H and G, so
-  // near_kernel keeps its values in the pair list and the finalize pass
-  // writes them into H and G after both.
-  int st;
-  if (fp64)
-    st = (c->nn == 2) ? launch_near<double, 2>(c, c->stream, g, (double*)nullptr, (double*)nullptr)
-                      : launch_near<double, 3>(c, c->stream, g, (double*)nullptr, (double*)nullptr);
-  else
-    st = (c->nn == 2) ? launch_near<float, 2>(c, c->stream, g, (float*)nullptr, (float*)nullptr)
-                      : launch_near<float, 3>(c, c->stream, g, (float*)nullptr, (float*)nullptr);
+  // Near entries: far_kernel writes 0 into the near entries of H and G, so the
+  // near values stay in the pair list and the finalize pass writes them into
+  // H and G after both.
+  int st = (c->nn == 2)
+               ? launch_near_pairs<2>(c, c->stream, g, c->qc.ptr, c->npg, nsing, c->sing_idx.ptr,
+                                      nreg, c->reg_idx.ptr, c->pair_row.ptr, c->pair_elem.ptr,
+                                      c->pair_h.ptr, c->pair_g.ptr)
+               : launch_near_pairs<3>(c, c->stream, g, c->qc.ptr, c->npg, nsing, c->sing_idx.ptr,
+                                      nreg, c->reg_idx.ptr, c->pair_row.ptr, c->pair_elem.ptr,
+                                      c->pair_h.ptr, c->pair_g.ptr);
   if (st) return st;
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[5], c->stream));
   // stream 2: the far field (needs only the geometry; starts after the classify
@@ -1412,7 +1500,8 @@ __global__ void interior_finalize(int npts, int n, int nn, int ncb,
 
 template <int NN>
 static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const QuadConst* d_qc,
-                          const double* d_w, bool all_near, double* d_out, hipEvent_t start,
+                          int npg, const double* d_w, bool all_near, double* d_out,
+                          hipEvent_t start,
                           hipEvent_t far_done, int64_t* npairs_out) {
   const int npts = g.nt;
   const int ncb = (g.ne + kEB - 1) / kEB;
@@ -1449,17 +1538,15 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
   BEMHIP_HIP_TRY(c, c->ev_pelem.ensure(cap));
   BEMHIP_HIP_TRY(c, c->ev_ph.ensure(cap * NN));
   BEMHIP_HIP_TRY(c, c->ev_pg.ensure(cap * NN));
-  hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, s1, npts, ncb, c->ev_bits.ptr,
-                     c->ev_tiles.ptr, c->ev_ptr.ptr, c->ev_prow.ptr, c->ev_pelem.ptr);
+  hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, s1, npts, ncb, NN, g.row0, g.n,
+                     c->ev_bits.ptr, c->ev_tiles.ptr, c->ev_ptr.ptr, c->ev_prow.ptr,
+                     c->ev_pelem.ptr, (int*)nullptr, (int*)nullptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
-  if (total > 0) {
-    const int pairs_per_block = 2 * (256 / kWave);
-    const int64_t blocks = ((int64_t)total + pairs_per_block - 1) / pairs_per_block;
-    hipLaunchKernelGGL((near_kernel<double, NN>), dim3((unsigned)blocks), dim3(256), 0, s1, g,
-                       d_qc, (int64_t)total, c->ev_prow.ptr, c->ev_pelem.ptr, c->ev_ph.ptr,
-                       c->ev_pg.ptr, (double*)nullptr, (double*)nullptr);
-    BEMHIP_HIP_TRY(c, hipGetLastError());
-  }
+  // no point is an element's node: every pair takes the tan-p3c rule, in CSR order
+  const int st = launch_near_pairs<NN>(c, s1, g, d_qc, npg, 0, nullptr, total, nullptr,
+                                       c->ev_prow.ptr, c->ev_pelem.ptr, c->ev_ph.ptr,
+                                       c->ev_pg.ptr);
+  if (st) return st;
   if (!all_near) BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s1, far_done, 0));
   const int tb = 256;
   hipLaunchKernelGGL(interior_finalize, dim3((npts + tb - 1) / tb), dim3(tb), 0, s1, npts, g.n,
@@ -1473,7 +1560,8 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
 // Points in chunks: the pair counts are 32-bit (at most npts x ne pairs per
 // chunk) and an all-near chunk keeps its pair list below 2^26 pairs.
 int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
-                  const double* d_w, const QuadConst* d_qc, double near_factor, double* d_out) {
+                  const double* d_w, const QuadConst* d_qc, int npg, double near_factor,
+                  double* d_out) {
   if (npts == 0) return BEMHIP_OK;
   Geo g = make_geo(c);
   g.row0 = c->nt;  // no point row is a node of an element
@@ -1514,10 +1602,10 @@ int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_p
     gc.rx = d_px + p0;
     gc.ry = d_py + p0;
     const int st = (c->nn == 2)
-                       ? interior_chunk<2>(c, gc, cs, d_qc, d_w, all_near, d_out + p0, ev[0],
-                                           ev[1], &npairs)
-                       : interior_chunk<3>(c, gc, cs, d_qc, d_w, all_near, d_out + p0, ev[0],
-                                           ev[1], &npairs);
+                       ? interior_chunk<2>(c, gc, cs, d_qc, npg, d_w, all_near, d_out + p0,
+                                           ev[0], ev[1], &npairs)
+                       : interior_chunk<3>(c, gc, cs, d_qc, npg, d_w, all_near, d_out + p0,
+                                           ev[0], ev[1], &npairs);
     if (st) return st;
   }
   c->stats["interior_pairs"] = (double)npairs;

@@ -102,6 +102,7 @@ struct bemhip_ctx {
   bemhip::DevBuf<uint32_t> nearbits;   // [ceil(ne/32)][nt]
   bemhip::DevBuf<int> row_cnt, row_ptr, pair_row, pair_elem;
   bemhip::DevBuf<double> pair_h, pair_g;  // [npairs][nn]
+  bemhip::DevBuf<int> sing_idx, reg_idx;  // CSR indices of the on-element / other pairs
   int64_t npairs = 0;
   bemhip::DevBuf<double> hpart;   // [ncb][nt] far row sums of H
   bemhip::DevBuf<double> blkbox;  // [ncb][5] node bounding box and cutoff of each column block
@@ -204,7 +205,8 @@ int lu_solve_trans_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int 
 int build_perm(bemhip_ctx* c, int n, const int* dipiv, int* dperm);
 int operator_apply_trans(bemhip_ctx* c, const double* d_v, double* d_y);
 int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
-                  const double* d_w, const QuadConst* d_qc, double near_factor, double* d_out);
+                  const double* d_w, const QuadConst* d_qc, int npg, double near_factor,
+                  double* d_out);
 
 // Small host helpers
 void gauss_legendre_host(int n, double* x, double* w);
