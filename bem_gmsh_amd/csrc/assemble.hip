@@ -268,6 +268,8 @@ __global__ __launch_bounds__(kFarThreads) void near_classify(Geo g, Consts cs, i
 // FP64 storage: the entries with the FP64 operation order of the reference.
 // Mixed storage: FP32 entries (FP32 reciprocal and logarithm) and the FP64
 // row sum of the far H entries (Newton reciprocal) for the diagonal.
+// H and G are written with non-temporal stores: nothing reads them back
+// before the kernel ends.
 // ---------------------------------------------------------------------------
 template <typename TS, int NN>
 __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* __restrict__ H,
@@ -305,8 +307,8 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
     if (nr) {
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
-        Hc[k * ld] = TS(0);
-        Gc[k * ld] = TS(0);
+        __builtin_nontemporal_store(TS(0), Hc + k * ld);
+        __builtin_nontemporal_store(TS(0), Gc + k * ld);
       }
     } else if constexpr (sizeof(TS) == 8) {
 #pragma unroll
@@ -317,8 +319,8 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
                          (cs.cT / d2);
         const double w = s_jw[el * NN + k];
         const double h = T * w;
-        Hc[k * ld] = h;
-        Gc[k * ld] = U * w;
+        __builtin_nontemporal_store(h, Hc + k * ld);
+        __builtin_nontemporal_store(U * w, Gc + k * ld);
         rowsum += h;
       }
     } else {
@@ -332,8 +334,8 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
         const float rinv = __builtin_amdgcn_rcpf(d2f[k]);
         const float U = -__logf(d2f[k]) * cUf;
         const float T = (rxf[k] * s_nxf[t] + ryf[k] * s_nyf[t]) * (cTf * rinv);
-        Hc[k * ld] = T * s_jwf[t];
-        Gc[k * ld] = U * s_jwf[t];
+        __builtin_nontemporal_store(T * s_jwf[t], Hc + k * ld);
+        __builtin_nontemporal_store(U * s_jwf[t], Gc + k * ld);
       }
     }
   }
@@ -423,24 +425,39 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
   if (nvec == 2) part2[(size_t)cb * ld + i] = acc2;
 }
 
-// Finalize an operator pass for row i:
+// Finalize an operator pass for row i (64 rows per 256-thread block):
 //   mode 0 (row sums):  diag_i = -(far + near H entries except column i)
 //                       written to diag[] (and to H's diagonal when given);
 //   mode 1 (apply):     y_i = far + near (self column: G_ii wG_i + diag_i wH_i)
 //                       + diag_i wH_i for internal rows.
-template <typename TS>
-__global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode, size_t ldh,
-                              const double* __restrict__ part, const int* __restrict__ row_ptr,
-                              const int* __restrict__ pair_elem,
-                              const double* __restrict__ pair_h,
-                              const double* __restrict__ pair_g, const double* __restrict__ wH,
-                              const double* __restrict__ wG, double* __restrict__ diag,
-                              double* __restrict__ y, TS* __restrict__ Hm,
-                              TS* __restrict__ Gm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nt) return;
+// Row sums of column-block partials part[cb * nt + i] for kFinRows rows per
+// 256-thread block: wave w adds blocks w, w + 4, w + 8, ... in order, then the
+// four wave sums are added in order (a fixed order for every launch).
+constexpr int kFinRows = 64;
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int ncb, int nt,
+                                               int i, double (*s_red)[kFinRows]) {
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   double s = 0.0;
-  for (int cb = 0; cb < ncb; ++cb) s += part[(size_t)cb * nt + i];
+  if (i < nt) {
+#pragma unroll 8
+    for (int cb = wv; cb < ncb; cb += 4) s += part[(size_t)cb * nt + i];
+  }
+  s_red[wv][lane] = s;
+  __syncthreads();
+  return ((s_red[0][lane] + s_red[1][lane]) + s_red[2][lane]) + s_red[3][lane];
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void op64_finalize(
+    int nt, int n, int nn, int ncb, int mode, size_t ldh, const double* __restrict__ part,
+    const int* __restrict__ row_ptr, const int* __restrict__ pair_elem,
+    const double* __restrict__ pair_h, const double* __restrict__ pair_g,
+    const double* __restrict__ wH, const double* __restrict__ wG, double* __restrict__ diag,
+    double* __restrict__ y, TS* __restrict__ Hm, TS* __restrict__ Gm) {
+  __shared__ double s_red[4][kFinRows];
+  const int i = blockIdx.x * kFinRows + (threadIdx.x & (kWave - 1));
+  double s = sum_partials(part, ncb, nt, i, s_red);
+  if (threadIdx.x >= kWave || i >= nt) return;  // wave 0 finishes its 64 rows
   const int p0 = row_ptr[i], p1 = row_ptr[i + 1];
   if (mode == 0) {
     for (int pr = p0; pr < p1; ++pr) {
@@ -478,29 +495,44 @@ __global__ void op64_finalize(int nt, int n, int nn, int ncb, int mode, size_t l
 // ---------------------------------------------------------------------------
 // Near-pair CSR from the bit matrix [word][row].
 // ---------------------------------------------------------------------------
-// Exclusive scan of cnt[0..nt) into ptr[0..nt] with one workgroup of 1024.
+// Exclusive scan of cnt[0..nt) into ptr[0..nt] with one workgroup of 16
+// wavefronts: each wave owns a contiguous segment, reads it 64 rows at a time
+// (coalesced) for its total, then scans it with lane shuffles from the sum of
+// the segments before it. The total also goes to host_total (pinned host
+// memory, may be null), so the host reads it without a copy on the stream.
 __global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict__ cnt,
-                                                  int* __restrict__ ptr) {
-  __shared__ int s[1024];
-  const int tid = threadIdx.x;
-  const int per = (nt + 1023) / 1024;
-  const int lo = min(nt, tid * per), hi = min(nt, lo + per);
-  int local = 0;
-  for (int r = lo; r < hi; ++r) local += cnt[r];
-  s[tid] = local;
+                                                  int* __restrict__ ptr,
+                                                  int* __restrict__ host_total) {
+  __shared__ int s_tot[16];
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int seg = (nt + 15) / 16;
+  const int lo = min(nt, wv * seg), hi = min(nt, lo + seg);
+  int tot = 0;
+  for (int r = lo + lane; r < hi; r += kWave) tot += cnt[r];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, kWave);
+  if (lane == 0) s_tot[wv] = tot;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = (tid >= off) ? s[tid - off] : 0;
-    __syncthreads();
-    s[tid] += v;
-    __syncthreads();
+  int run = 0;
+  for (int w = 0; w < wv; ++w) run += s_tot[w];
+  for (int r0 = lo; r0 < hi; r0 += kWave) {
+    const int r = r0 + lane;
+    const int v = (r < hi) ? cnt[r] : 0;
+    int x = v;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(x, off, kWave);
+      if (lane >= off) x += y;
+    }
+    if (r < hi) ptr[r] = run + x - v;
+    run += __shfl(x, kWave - 1, kWave);
   }
-  int run = s[tid] - local;
-  for (int r = lo; r < hi; ++r) {
-    ptr[r] = run;
-    run += cnt[r];
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < 16; ++w) t += s_tot[w];
+    ptr[nt] = t;
+    if (host_total) __hip_atomic_store(host_total, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  if (tid == 1023) ptr[nt] = s[1023];
 }
 
 // One thread per row, in the row blocks of near_classify: only the column
@@ -1201,6 +1233,17 @@ int launch_near_pairs(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Qua
 
 }  // namespace
 
+// Pinned host word for the near-pair count, written by near_scan through its
+// device address.
+static hipError_t ensure_count_word(bemhip_ctx* c) {
+  if (c->h_count && c->d_count) return hipSuccess;
+  if (!c->h_count) {
+    hipError_t e = hipHostMalloc((void**)&c->h_count, sizeof(int));
+    if (e != hipSuccess) return e;
+  }
+  return hipHostGetDevicePointer((void**)&c->d_count, c->h_count, 0);
+}
+
 Geo make_geo(bemhip_ctx* c) {
   Geo g;
   g.p = c->p; g.nn = c->nn; g.ne = c->ne; g.n = c->n; g.ni = c->ni; g.nt = c->nt;
@@ -1254,24 +1297,23 @@ int assemble_impl(bemhip_ctx* c) {
                                        sizeof(float) * c->ld32 * c->ni, c->stream));
   }
   // events: 0 start, 1 classified, 2 far done, 3 pairs start, 4 pairs done,
-  // 5 near done, 6 diagonal done, 7 far start
-  hipEvent_t ev[8];
+  // 5 near done, 6 diagonal done, 7 far start, 8 finalize start
+  hipEvent_t ev[9];
   for (auto& e : ev) BEMHIP_HIP_TRY(c, hipEventCreate(&e));
   struct Destroy {
     hipEvent_t* e;
-    ~Destroy() { for (int k = 0; k < 8; ++k) hipEventDestroy(e[k]); }
+    ~Destroy() { for (int k = 0; k < 9; ++k) hipEventDestroy(e[k]); }
   } destroy{ev};
-  const int tb = 256;
-  const int nb = (int)((nt + tb - 1) / tb);
+  const int nfb = (int)((nt + kFinRows - 1) / kFinRows);
   hipStream_t s2 = c->stream2;
 
-  // stream 1: classification with the per-row counts, near-pair CSR, near
-  // integrals; the far field goes to stream 2 once the near work is queued
-  // (the near chain is the longer one, so it is dispatched first).
+  // stream 1: classification with the per-row counts; then the far field on
+  // stream 2 (the longer chain: HBM-write bound) while stream 1 builds the
+  // near-pair CSR and runs the near integrals.
   const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);  // row blocks
   BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
   BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
-  if (!c->h_count) BEMHIP_HIP_TRY(c, hipHostMalloc((void**)&c->h_count, sizeof(int)));
+  BEMHIP_HIP_TRY(c, ensure_count_word(c));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
   BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
   {
@@ -1292,13 +1334,24 @@ int assemble_impl(bemhip_ctx* c) {
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[3], c->stream));
+  // stream 2: the far field (needs only the geometry; starts after the classify
+  // so that the phases can be timed apart)
+  int st;
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
+  if (fp64)
+    st = (c->nn == 2) ? launch_far<double, 2>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
+                      : launch_far<double, 3>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+  else
+    st = (c->nn == 2) ? launch_far<float, 2>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr)
+                      : launch_far<float, 3>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
+  if (st) return st;
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], s2));
   hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
-                     c->row_ptr.ptr);
+                     c->row_ptr.ptr, c->d_count);
   BEMHIP_HIP_TRY(c, hipGetLastError());
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->h_count, c->row_ptr.ptr + nt, sizeof(int),
-                                   hipMemcpyDeviceToHost, c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  const int total = *c->h_count;
+  const int total = *(volatile int*)c->h_count;
   c->npairs = total;
   BEMHIP_HIP_TRY(c, c->pair_row.ensure((size_t)std::max(total, 1)));
   BEMHIP_HIP_TRY(c, c->pair_elem.ensure((size_t)std::max(total, 1)));
@@ -1316,7 +1369,7 @@ int assemble_impl(bemhip_ctx* c) {
   // Near entries: far_kernel writes 0 into the near entries of H and G, so the
   // near values stay in the pair list and the finalize pass writes them into
   // H and G after both.
-  int st = (c->nn == 2)
+  st = (c->nn == 2)
                ? launch_near_pairs<2>(c, c->stream, g, c->qc.ptr, c->npg, nsing, c->sing_idx.ptr,
                                       nreg, c->reg_idx.ptr, c->pair_row.ptr, c->pair_elem.ptr,
                                       c->pair_h.ptr, c->pair_g.ptr)
@@ -1325,28 +1378,17 @@ int assemble_impl(bemhip_ctx* c) {
                                       c->pair_h.ptr, c->pair_g.ptr);
   if (st) return st;
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[5], c->stream));
-  // stream 2: the far field (needs only the geometry; starts after the classify
-  // so that the phases can be timed apart)
-  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
-  BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
-  if (fp64)
-    st = (c->nn == 2) ? launch_far<double, 2>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
-                      : launch_far<double, 3>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
-  else
-    st = (c->nn == 2) ? launch_far<float, 2>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr)
-                      : launch_far<float, 3>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
-  if (st) return st;
-  BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], s2));
   BEMHIP_HIP_TRY(c, hipStreamWaitEvent(c->stream, ev[2], 0));
-  // join (above): near values into H, G and the rigid-body diagonal
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[8], c->stream));
+  // join: near values into H, G and the rigid-body diagonal
   if (fp64)
-    hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
-                       c->nn, ncb, 0, c->ld64, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+    hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt,
+                       c->n, c->nn, ncb, 0, c->ld64, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                        c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr, nullptr,
                        c->H64.ptr, c->G64.ptr);
   else
-    hipLaunchKernelGGL((op64_finalize<float>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
-                       c->nn, ncb, 0, c->ld32, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+    hipLaunchKernelGGL((op64_finalize<float>), dim3(nfb), dim3(256), 0, c->stream, (int)nt,
+                       c->n, c->nn, ncb, 0, c->ld32, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                        c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr, nullptr,
                        c->H32.ptr, c->G32.ptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
@@ -1361,7 +1403,7 @@ int assemble_impl(bemhip_ctx* c) {
   c->ms["assemble_far"] = el(7, 2);
   c->ms["assemble_pairs"] = el(3, 4);
   c->ms["assemble_near"] = el(4, 5);
-  c->ms["assemble_diag"] = el(5, 6);
+  c->ms["assemble_diag"] = el(8, 6);
   return BEMHIP_OK;
 }
 
@@ -1402,9 +1444,8 @@ static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, doub
   else
     hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1, wH, wG,
                        nullptr, nullptr, c->part1.ptr, nullptr);
-  const int tb = 256;
-  const int nb = (int)((nt + tb - 1) / tb);
-  hipLaunchKernelGGL((op64_finalize<double>), dim3(nb), dim3(tb), 0, c->stream, (int)nt, c->n,
+  const int nfb = (int)((nt + kFinRows - 1) / kFinRows);
+  hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt, c->n,
                      c->nn, ncb, 1, (size_t)0, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                      c->pair_h.ptr, c->pair_g.ptr, wH, wG, c->diag64.ptr, d_y,
                      (double*)nullptr, (double*)nullptr);
@@ -1468,23 +1509,19 @@ int apply_bc_impl(bemhip_ctx* c) {
 // + g_j w[1][j] (w[0] = -T, w[1] = q) and s2 = sum_j h_j; u = s1 / (-s2).
 // part: [2][ncb][npts] far partial sums (null when every pair is near).
 // ---------------------------------------------------------------------------
-__global__ void interior_finalize(int npts, int n, int nn, int ncb,
-                                  const double* __restrict__ part,
-                                  const int* __restrict__ row_ptr,
-                                  const int* __restrict__ pair_elem,
-                                  const double* __restrict__ ph,
-                                  const double* __restrict__ pg,
-                                  const double* __restrict__ w, double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npts) return;
+__global__ __launch_bounds__(256) void interior_finalize(
+    int npts, int n, int nn, int ncb, const double* __restrict__ part,
+    const int* __restrict__ row_ptr, const int* __restrict__ pair_elem,
+    const double* __restrict__ ph, const double* __restrict__ pg, const double* __restrict__ w,
+    double* __restrict__ out) {
+  __shared__ double s_red1[4][kFinRows], s_red2[4][kFinRows];
+  const int i = blockIdx.x * kFinRows + (threadIdx.x & (kWave - 1));
   double s1 = 0.0, s2 = 0.0;
-  if (part) {
-    const size_t plane = (size_t)ncb * npts;
-    for (int cb = 0; cb < ncb; ++cb) {
-      s1 += part[(size_t)cb * npts + i];
-      s2 += part[plane + (size_t)cb * npts + i];
-    }
+  if (part) {  // uniform over the launch
+    s1 = sum_partials(part, ncb, npts, i, s_red1);
+    s2 = sum_partials(part + (size_t)ncb * npts, ncb, npts, i, s_red2);
   }
+  if (threadIdx.x >= kWave || i >= npts) return;  // wave 0 finishes its 64 points
   const double* wT = w;
   const double* wq = w + n;
   for (int pr = row_ptr[i]; pr < row_ptr[i + 1]; ++pr) {
@@ -1527,12 +1564,11 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
   hipLaunchKernelGGL((near_classify<NN>), cgrid, dim3(kFarThreads), 0, s1, g, cs, ncb, cpb,
                      c->ev_box.ptr, c->ev_bits.ptr, c->ev_cnt.ptr, c->ev_tiles.ptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
-  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, s1, npts, c->ev_cnt.ptr, c->ev_ptr.ptr);
+  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, s1, npts, c->ev_cnt.ptr, c->ev_ptr.ptr,
+                     c->d_count);
   BEMHIP_HIP_TRY(c, hipGetLastError());
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->h_count, c->ev_ptr.ptr + npts, sizeof(int),
-                                   hipMemcpyDeviceToHost, s1));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(s1));
-  const int total = *c->h_count;
+  const int total = *(volatile int*)c->h_count;
   const size_t cap = (size_t)std::max(total, 1);
   BEMHIP_HIP_TRY(c, c->ev_prow.ensure(cap));
   BEMHIP_HIP_TRY(c, c->ev_pelem.ensure(cap));
@@ -1548,8 +1584,8 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
                                        c->ev_pg.ptr);
   if (st) return st;
   if (!all_near) BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s1, far_done, 0));
-  const int tb = 256;
-  hipLaunchKernelGGL(interior_finalize, dim3((npts + tb - 1) / tb), dim3(tb), 0, s1, npts, g.n,
+  hipLaunchKernelGGL(interior_finalize, dim3((npts + kFinRows - 1) / kFinRows), dim3(256), 0, s1,
+                     npts, g.n,
                      NN, ncb, all_near ? (const double*)nullptr : c->ev_part.ptr, c->ev_ptr.ptr,
                      c->ev_pelem.ptr, c->ev_ph.ptr, c->ev_pg.ptr, d_w, d_out);
   BEMHIP_HIP_TRY(c, hipGetLastError());
@@ -1580,7 +1616,7 @@ int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_p
   BEMHIP_HIP_TRY(c, c->ev_ptr.ensure((size_t)chunk + 1));
   BEMHIP_HIP_TRY(c, c->ev_tiles.ensure((size_t)nrb * ncb));
   if (!all_near) BEMHIP_HIP_TRY(c, c->ev_part.ensure(2 * (size_t)ncb * chunk));
-  if (!c->h_count) BEMHIP_HIP_TRY(c, hipHostMalloc((void**)&c->h_count, sizeof(int)));
+  BEMHIP_HIP_TRY(c, ensure_count_word(c));
   if (c->nn == 2)
     hipLaunchKernelGGL((elem_block_boxes<2>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
                        cs, ncb, c->ev_box.ptr);

@@ -108,6 +108,7 @@ struct bemhip_ctx {
   bemhip::DevBuf<double> blkbox;  // [ncb][5] node bounding box and cutoff of each column block
   bemhip::DevBuf<uint8_t> tiles;  // [row block][ncb] 1 where the pair tests ran
   int* h_count = nullptr;          // pinned host word: near-pair count
+  int* d_count = nullptr;          // its device address
   bemhip::DevBuf<double> diag64;  // [nt] rigid-body diagonal
   bemhip::DevBuf<char> scan_tmp;
 
