@@ -1515,24 +1515,39 @@ __global__ __launch_bounds__(256) void interior_finalize(
     const double* __restrict__ ph, const double* __restrict__ pg, const double* __restrict__ w,
     double* __restrict__ out) {
   __shared__ double s_red1[4][kFinRows], s_red2[4][kFinRows];
-  const int i = blockIdx.x * kFinRows + (threadIdx.x & (kWave - 1));
-  double s1 = 0.0, s2 = 0.0;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int i0 = blockIdx.x * kFinRows;
   if (part) {  // uniform over the launch
-    s1 = sum_partials(part, ncb, npts, i, s_red1);
-    s2 = sum_partials(part + (size_t)ncb * npts, ncb, npts, i, s_red2);
+    const int i = i0 + lane;
+    const double f1 = sum_partials(part, ncb, npts, i, s_red1);
+    const double f2 = sum_partials(part + (size_t)ncb * npts, ncb, npts, i, s_red2);
+    __syncthreads();  // every wave has read the sums of its lane's point
+    if (wv == 0) { s_red1[0][lane] = f1; s_red2[0][lane] = f2; }
+  } else if (wv == 0) {
+    s_red1[0][lane] = 0.0;
+    s_red2[0][lane] = 0.0;
   }
-  if (threadIdx.x >= kWave || i >= npts) return;  // wave 0 finishes its 64 points
+  __syncthreads();
   const double* wT = w;
   const double* wq = w + n;
-  for (int pr = row_ptr[i]; pr < row_ptr[i + 1]; ++pr) {
-    const int c0 = pair_elem[pr] * nn;
-    for (int k = 0; k < nn; ++k) {
-      const double h = ph[(size_t)pr * nn + k];
-      s1 += h * wT[c0 + k] + pg[(size_t)pr * nn + k] * wq[c0 + k];
-      s2 += h;
+  // near pairs: wave wv takes points i0 + wv, i0 + wv + 4, ...; its lanes take
+  // the point's pairs in strides of 64 and the wave sums them (fixed order)
+  for (int t = wv; t < kFinRows; t += 4) {
+    const int i = i0 + t;
+    if (i >= npts) break;  // uniform over the wave
+    double s1 = 0.0, s2 = 0.0;
+    for (int pr = row_ptr[i] + lane; pr < row_ptr[i + 1]; pr += kWave) {
+      const int c0 = pair_elem[pr] * nn;
+      for (int k = 0; k < nn; ++k) {
+        const double h = ph[(size_t)pr * nn + k];
+        s1 += h * wT[c0 + k] + pg[(size_t)pr * nn + k] * wq[c0 + k];
+        s2 += h;
+      }
     }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) out[i] = (s_red1[0][t] + s1) / (-(s_red2[0][t] + s2));
   }
-  out[i] = s1 / (-s2);
 }
 
 template <int NN>
