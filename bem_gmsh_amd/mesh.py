@@ -436,13 +436,39 @@ _CELL_CORNERS = {2: 3, 3: 4, 9: 3, 10: 4, 16: 4, 21: 3, 23: 3}
 
 
 def read_msh(path: str, name: Optional[str] = None) -> Mesh2D:
-    """Read a Gmsh ASCII mesh (format 2.2 or 4.1) into a Mesh2D.
+    """Read a Gmsh mesh (format 2.2 or 4.1, ASCII or binary) into a Mesh2D.
 
     Line elements carry the physical-group name of their entity (first group);
     2-D elements become domain cells (corners only, as src/Core/Input.jl:307-322).
     """
-    with open(path, "r") as f:
-        tok = f.read().split("\n")
+    with open(path, "rb") as f:
+        data = f.read()
+    version, binary, dsize = _msh_format(data)
+    if binary:
+        parsed = _parse_msh_binary(data, version, dsize)
+    else:
+        parsed = _parse_msh_ascii(data.decode("utf-8", errors="replace").split("\n"))
+    names, nodes, lines_raw, cells_raw = parsed
+    nm = name or path.rsplit("/", 1)[-1].rsplit(".", 1)[0]
+    return _mesh_from_raw(nm, names, nodes, lines_raw, cells_raw)
+
+
+def _msh_format(data: bytes):
+    """(version, binary, data size) from the $MeshFormat header."""
+    k = data.find(b"$MeshFormat")
+    if k < 0:
+        raise ValueError("read_msh: no $MeshFormat section")
+    line = data[k:].split(b"\n", 2)[1].split()
+    version, ftype, dsize = float(line[0]), int(line[1]), int(line[2])
+    if ftype not in (0, 1):
+        raise ValueError(f"read_msh: unknown file type {ftype}")
+    if ftype == 1 and dsize != 8:
+        raise ValueError(f"read_msh: binary data size {dsize} (only 8 is defined)")
+    return version, ftype == 1, dsize
+
+
+def _parse_msh_ascii(tok):
+    """Sections of an ASCII .msh: (names, nodes, lines, cells)."""
     i = 0
     version = 2.2
     names = {}
@@ -526,6 +552,133 @@ def read_msh(path: str, name: Optional[str] = None) -> Mesh2D:
                 i += 2 + cnt
         else:
             i += 1
+    return names, nodes, lines_raw, cells_raw
+
+
+# Nodes per element of the Gmsh element types a binary file may hold (the
+# binary element records have no length field).
+_MSH_NODES = {1: 2, 2: 3, 3: 4, 4: 4, 5: 8, 6: 6, 7: 5, 8: 3, 9: 6, 10: 9, 11: 10, 12: 27,
+              13: 18, 14: 14, 15: 1, 16: 8, 17: 20, 18: 15, 19: 13, 20: 9, 21: 10, 22: 12,
+              23: 15, 24: 15, 25: 21, 26: 4, 27: 5, 28: 6, 29: 20, 30: 35, 31: 56, 36: 16,
+              37: 25, 38: 36}
+
+
+class _Cursor:
+    """Byte cursor over a binary .msh: text lines and little/big-endian arrays."""
+
+    def __init__(self, data: bytes, pos: int, endian: str):
+        self.d, self.p, self.e = data, pos, endian
+
+    def line(self) -> str:
+        k = self.d.index(b"\n", self.p)
+        s = self.d[self.p:k].decode("utf-8", errors="replace").strip()
+        self.p = k + 1
+        return s
+
+    def arr(self, code: str, count: int) -> np.ndarray:
+        dt = np.dtype(self.e + code)
+        a = np.frombuffer(self.d, dtype=dt, count=count, offset=self.p)
+        self.p += dt.itemsize * count
+        return a
+
+    def one(self, code: str):
+        return self.arr(code, 1)[0].item()
+
+
+def _parse_msh_binary(data: bytes, version: float, dsize: int):
+    """Sections of a binary .msh (Gmsh file format 2.2 and 4.1, data size 8):
+    ints are 4 bytes, size_t 8 bytes, reals 8-byte IEEE; the endianness comes
+    from the integer 1 written after the format line."""
+    k = data.find(b"$MeshFormat")
+    k = data.index(b"\n", data.index(b"\n", k) + 1) + 1     # after the format line
+    one = data[k:k + 4]
+    endian = "<" if np.frombuffer(one, "<i4")[0] == 1 else ">"
+    if np.frombuffer(one, endian + "i4")[0] != 1:
+        raise ValueError("read_msh: bad endianness marker")
+    cur = _Cursor(data, k + 4, endian)
+    names, nodes = {}, {}
+    lines_raw, cells_raw = [], []
+    ent_phys = {}
+    v4 = version >= 4
+    st = "u8"                                               # size_t
+    while cur.p < len(data):
+        t = cur.line()
+        if t == "$PhysicalNames":                           # ASCII in both versions
+            for _ in range(int(cur.line())):
+                parts = cur.line().split(maxsplit=2)
+                names[(int(parts[0]), int(parts[1]))] = parts[2].strip().strip('"')
+        elif t == "$Entities" and v4:
+            npt, ncu, nsu, nvo = (int(x) for x in cur.arr(st, 4))
+            for _ in range(npt):
+                cur.arr("i4", 1); cur.arr("f8", 3)
+                cur.arr("i4", cur.one(st))
+            for dim, count in ((1, ncu), (2, nsu), (3, nvo)):
+                for _ in range(count):
+                    tag = cur.one("i4")
+                    cur.arr("f8", 6)
+                    ph = cur.arr("i4", cur.one(st))
+                    if ph.size:
+                        ent_phys[(dim, tag)] = abs(int(ph[0]))
+                    cur.arr("i4", cur.one(st))
+        elif t == "$Nodes":
+            if v4:
+                nblk, _, _, _ = (int(x) for x in cur.arr(st, 4))
+                for _ in range(nblk):
+                    cur.arr("i4", 2)
+                    par = cur.one("i4")
+                    cnt = cur.one(st)
+                    tags = cur.arr(st, cnt)
+                    xyz = cur.arr("f8", cnt * (3 + 3 * par)).reshape(cnt, -1)
+                    for q in range(cnt):
+                        nodes[int(tags[q])] = (float(xyz[q, 0]), float(xyz[q, 1]))
+            else:
+                cnt = int(cur.line())
+                rec = np.dtype([("tag", endian + "i4"), ("xyz", endian + "f8", 3)])
+                a = np.frombuffer(data, dtype=rec, count=cnt, offset=cur.p)
+                cur.p += rec.itemsize * cnt
+                for r in a:
+                    nodes[int(r["tag"])] = (float(r["xyz"][0]), float(r["xyz"][1]))
+        elif t == "$Elements":
+            if v4:
+                nblk, _, _, _ = (int(x) for x in cur.arr(st, 4))
+                for _ in range(nblk):
+                    dim, etag, etype = (int(x) for x in cur.arr("i4", 3))
+                    cnt = cur.one(st)
+                    if etype not in _MSH_NODES:
+                        raise ValueError(f"read_msh: unknown element type {etype}")
+                    recs = cur.arr(st, cnt * (1 + _MSH_NODES[etype])).reshape(cnt, -1)
+                    for q in range(cnt):
+                        conn = [int(x) for x in recs[q, 1:]]
+                        if dim == 1 and etype in _LINE_TYPES:
+                            lines_raw.append((etype, conn, ent_phys.get((1, etag), -1), etag))
+                        elif dim == 2:
+                            cells_raw.append((etype, conn))
+            else:
+                total = int(cur.line())
+                seen = 0
+                while seen < total:
+                    etype, follow, ntag = (int(x) for x in cur.arr("i4", 3))
+                    if etype not in _MSH_NODES:
+                        raise ValueError(f"read_msh: unknown element type {etype}")
+                    width = 1 + ntag + _MSH_NODES[etype]
+                    recs = cur.arr("i4", follow * width).reshape(follow, width)
+                    for q in range(follow):
+                        tags = [int(x) for x in recs[q, 1:1 + ntag]]
+                        conn = [int(x) for x in recs[q, 1 + ntag:]]
+                        phys = tags[0] if ntag > 0 else -1
+                        ent = tags[1] if ntag > 1 else -1
+                        if etype in _LINE_TYPES:
+                            lines_raw.append((etype, conn, phys, ent))
+                        elif etype in _CELL_CORNERS:
+                            cells_raw.append((etype, conn))
+                    seen += follow
+        elif t.startswith("$") and not t.startswith("$End"):
+            # a section this reader does not use: skip to its end marker
+            cur.p = data.index(b"$End" + t[1:].encode(), cur.p)
+    return names, nodes, lines_raw, cells_raw
+
+
+def _mesh_from_raw(nm, names, nodes, lines_raw, cells_raw) -> Mesh2D:
     if not lines_raw:
         raise ValueError("read_msh: no 1D mesh elements")
     order = _LINE_TYPES[lines_raw[0][0]]
@@ -538,12 +691,18 @@ def read_msh(path: str, name: Optional[str] = None) -> Mesh2D:
     for etype, conn in cells_raw:
         nc = _CELL_CORNERS.get(etype, min(len(conn), 4))
         cells.append(np.array([nodes[t] for t in conn[:nc]]))
-    nm = name or path.rsplit("/", 1)[-1].rsplit(".", 1)[0]
     return Mesh2D(nm, order, lines, phys, region, cells)
 
 
-def write_msh(mesh: Mesh2D, path: str) -> None:
-    """Write a Mesh2D as Gmsh ASCII 2.2 (lines + quad/tri cells, physical names)."""
+
+
+def write_msh(mesh: Mesh2D, path: str, fmt: str = "2.2", binary: bool = False) -> None:
+    """Write a Mesh2D as a Gmsh mesh: lines with their physical names, quad /
+    triangle domain cells. fmt "2.2" (ASCII) or "4.1" (ASCII or binary)."""
+    if fmt not in ("2.2", "4.1"):
+        raise ValueError(f"write_msh: fmt must be '2.2' or '4.1', got {fmt!r}")
+    if binary and fmt != "4.1":
+        raise ValueError("write_msh: binary output is written in format 4.1")
     names = {}
     for nm in mesh.physical:
         if nm and nm not in names:
@@ -558,28 +717,125 @@ def write_msh(mesh: Mesh2D, path: str) -> None:
             pts.append(key)
         return index[key]
 
-    elines = []
     etype = {1: 1, 2: 8, 3: 26, 4: 27, 5: 28}[mesh.order]
-    for e in range(mesh.lines.shape[0]):
-        conn = [tag(x) for x in mesh.lines[e]]
-        ph = names.get(mesh.physical[e], 0)
-        elines.append(f"{etype} 2 {ph} {int(mesh.region[e])} " + " ".join(map(str, conn)))
+    lconn = [[tag(x) for x in mesh.lines[e]] for e in range(mesh.lines.shape[0])]
+    cconn = [[tag(x) for x in c] for c in mesh.cells]
     dom = len(names) + 1
-    for c in mesh.cells:
-        conn = [tag(x) for x in c]
-        ct = 3 if len(conn) == 4 else 2
-        elines.append(f"{ct} 2 {dom} 1 " + " ".join(map(str, conn)))
-    with open(path, "w") as f:
-        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
-        f.write(f"$PhysicalNames\n{len(names) + 1}\n")
+    if fmt == "2.2":
+        elines = []
+        for e, conn in enumerate(lconn):
+            ph = names.get(mesh.physical[e], 0)
+            elines.append(f"{etype} 2 {ph} {int(mesh.region[e])} " + " ".join(map(str, conn)))
+        for conn in cconn:
+            ct = 3 if len(conn) == 4 else 2
+            elines.append(f"{ct} 2 {dom} 1 " + " ".join(map(str, conn)))
+        with open(path, "w") as f:
+            f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
+            f.write(f"$PhysicalNames\n{len(names) + 1}\n")
+            for nm, t in names.items():
+                f.write(f'1 {t} "{nm}"\n')
+            f.write(f'2 {dom} "Domain"\n$EndPhysicalNames\n')
+            f.write(f"$Nodes\n{len(pts)}\n")
+            for q, (x, y) in enumerate(pts):
+                f.write(f"{q + 1} {x!r} {y!r} 0\n")
+            f.write("$EndNodes\n")
+            f.write(f"$Elements\n{len(elines)}\n")
+            for q, s_ in enumerate(elines):
+                f.write(f"{q + 1} {s_}\n")
+            f.write("$EndElements\n")
+        return
+    _write_msh41(path, binary, names, dom, pts, etype, lconn, cconn, mesh)
+
+
+def _write_msh41(path, binary, names, dom, pts, etype, lconn, cconn, mesh):
+    """Format 4.1: one curve entity per (region, physical name), one surface for
+    the cells, all nodes in one block on the surface."""
+    curves = {}                                  # (region, name) -> entity tag
+    used = set()
+    for e in range(len(lconn)):
+        key = (int(mesh.region[e]), mesh.physical[e])
+        if key not in curves:
+            t = key[0] if key[0] > 0 and key[0] not in used else max(used | {0}) + 1
+            used.add(t)
+            curves[key] = t
+    blocks = {}                                  # entity tag -> element indices
+    for e in range(len(lconn)):
+        blocks.setdefault(curves[(int(mesh.region[e]), mesh.physical[e])], []).append(e)
+    cell_blocks = {}
+    for q, conn in enumerate(cconn):
+        cell_blocks.setdefault(3 if len(conn) == 4 else 2, []).append(q)
+    nel = len(lconn) + len(cconn)
+    xy = np.asarray(pts, dtype=np.float64).reshape(-1, 2)
+    with open(path, "wb") as f:
+        w = f.write
+        w(b"$MeshFormat\n")
+        w(b"4.1 1 8\n" if binary else b"4.1 0 8\n")
+        if binary:
+            w(np.array([1], dtype="<i4").tobytes() + b"\n")
+        w(b"$EndMeshFormat\n")
+        w(f"$PhysicalNames\n{len(names) + 1}\n".encode())
         for nm, t in names.items():
-            f.write(f'1 {t} "{nm}"\n')
-        f.write(f'2 {dom} "Domain"\n$EndPhysicalNames\n')
-        f.write(f"$Nodes\n{len(pts)}\n")
-        for q, (x, y) in enumerate(pts):
-            f.write(f"{q + 1} {x!r} {y!r} 0\n")
-        f.write("$EndNodes\n")
-        f.write(f"$Elements\n{len(elines)}\n")
-        for q, s in enumerate(elines):
-            f.write(f"{q + 1} {s}\n")
-        f.write("$EndElements\n")
+            w(f'1 {t} "{nm}"\n'.encode())
+        w(f'2 {dom} "Domain"\n$EndPhysicalNames\n'.encode())
+        # entities: curves with their physical group, one surface
+        w(b"$Entities\n")
+        lo, hi = xy.min(axis=0) if len(xy) else (0, 0), xy.max(axis=0) if len(xy) else (0, 0)
+        box = [float(lo[0]), float(lo[1]), 0.0, float(hi[0]), float(hi[1]), 0.0]
+        if binary:
+            w(np.array([0, len(curves), 1, 0], dtype="<u8").tobytes())
+            for (reg, nm), t in curves.items():
+                ph = [names[nm]] if nm in names else []
+                w(np.array([t], "<i4").tobytes() + np.array(box, "<f8").tobytes())
+                w(np.array([len(ph)], "<u8").tobytes() + np.array(ph, "<i4").tobytes())
+                w(np.array([0], "<u8").tobytes())
+            w(np.array([1], "<i4").tobytes() + np.array(box, "<f8").tobytes())
+            w(np.array([1], "<u8").tobytes() + np.array([dom], "<i4").tobytes())
+            w(np.array([0], "<u8").tobytes())
+            w(b"\n")
+        else:
+            w(f"0 {len(curves)} 1 0\n".encode())
+            bs = " ".join(repr(v) for v in box)
+            for (reg, nm), t in curves.items():
+                ph = f"1 {names[nm]}" if nm in names else "0"
+                w(f"{t} {bs} {ph} 0\n".encode())
+            w(f"1 {bs} 1 {dom} 0\n".encode())
+        w(b"$EndEntities\n")
+        # nodes: one block on the surface
+        w(b"$Nodes\n")
+        n = len(pts)
+        if binary:
+            w(np.array([1, n, 1, n], "<u8").tobytes())
+            w(np.array([2, 1, 0], "<i4").tobytes() + np.array([n], "<u8").tobytes())
+            w(np.arange(1, n + 1, dtype="<u8").tobytes())
+            xyz = np.zeros((n, 3), "<f8")
+            xyz[:, :2] = xy
+            w(xyz.tobytes())
+            w(b"\n")
+        else:
+            w(f"1 {n} 1 {n}\n2 1 0 {n}\n".encode())
+            w("".join(f"{q + 1}\n" for q in range(n)).encode())
+            w("".join(f"{x!r} {y!r} 0\n" for x, y in pts).encode())
+        w(b"$EndNodes\n")
+        # elements: one block per curve entity, one per cell type
+        w(b"$Elements\n")
+        nblk = len(blocks) + len(cell_blocks)
+        tagc = 1
+        if binary:
+            w(np.array([nblk, nel, 1, nel], "<u8").tobytes())
+        else:
+            w(f"{nblk} {nel} 1 {nel}\n".encode())
+        groups = [(1, t, etype, [lconn[e] for e in es]) for t, es in blocks.items()]
+        groups += [(2, 1, ct, [cconn[q] for q in qs]) for ct, qs in cell_blocks.items()]
+        for dim, ent, et, conns in groups:
+            if binary:
+                w(np.array([dim, ent, et], "<i4").tobytes() + np.array([len(conns)], "<u8").tobytes())
+                recs = np.array([[tagc + q] + c for q, c in enumerate(conns)], dtype="<u8")
+                w(recs.tobytes())
+            else:
+                w(f"{dim} {ent} {et} {len(conns)}\n".encode())
+                w("".join(f"{tagc + q} " + " ".join(map(str, c)) + "\n"
+                          for q, c in enumerate(conns)).encode())
+            tagc += len(conns)
+        if binary:
+            w(b"\n")
+        w(b"$EndElements\n")
