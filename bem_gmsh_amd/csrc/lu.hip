@@ -873,18 +873,22 @@ __global__ void gather_kernel(int n, const int* __restrict__ perm, const double*
   if (i < n) y[i] = b[perm[i]];
 }
 
-// Inverse of diagonal block b of L (unit lower) or U (upper): one workgroup of
-// 4 x kSolveNB threads per block. Column c of X = inverse is built row by row
-// by the four lanes of a quad (c = thread / 4): lane p sums the terms k = p
-// mod 4 in FP64 over two accumulators, the quad adds its four partial sums by
-// lane shuffles, and lane 0 stores the entry in LDS (a wavefront's LDS
-// operations complete in order, so the quad's next row sees it). X is kept
-// transposed (Xt[c][k], rows padded by 16 bytes); the block of the factor is
-// read from an LDS copy for FP32 (row-major) and from memory for FP64 (Xt
-// alone takes 130 KB).
-constexpr int kTriInvThreads = 4 * kSolveNB;
+// Inverse of diagonal block b of L (unit lower) or U (upper): kTriInvGroups
+// workgroups per block, each building kTriInvCols columns of X = inverse.
+// Column c is built row by row by a group of kTriInvLanes lanes (one wave
+// holds four columns): lane p sums the terms k = p mod kTriInvLanes of the row's
+// dot product in FP64 over two accumulators, the group adds its partial sums
+// by lane shuffles, and lane 0 stores the entry in LDS (a wavefront's LDS
+// operations complete in order, so the group's next row sees it). A column's
+// chain starts at its diagonal: X(i, c) = 0 for i < c (L) or i > c (U). X is
+// kept transposed (Xt[c][k], rows padded by 16 bytes); the block of the factor
+// is read from an LDS copy for FP32 (row-major) and from memory for FP64.
+constexpr int kTriInvThreads = 512;
+constexpr int kTriInvLanes = 16;
+constexpr int kTriInvCols = kTriInvThreads / kTriInvLanes;  // columns per workgroup
+constexpr int kTriInvGroups = kSolveNB / kTriInvCols;       // workgroups per block
 constexpr size_t tri_inv_lds(size_t elem) {
-  return (size_t)kSolveNB * (kSolveNB + 16 / elem) * elem +
+  return (size_t)kTriInvCols * (kSolveNB + 16 / elem) * elem +
          (elem == 4 ? (size_t)kSolveNB * kSolveNB * elem : 0);
 }
 
@@ -893,15 +897,18 @@ __global__ __launch_bounds__(kTriInvThreads) void tri_inv_kernel(int n, const T*
                                                                 int lda, T* __restrict__ Dinv) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int NB = kSolveNB;
+  constexpr int LP = kTriInvLanes;
   constexpr int XP = NB + 16 / (int)sizeof(T);
   constexpr bool kStage = sizeof(T) == 4;
-  T* Xt = reinterpret_cast<T*>(smem_raw);  // Xt[c*XP + k] = X(k, c)
-  T* Fs = Xt + NB * XP;                    // FP32: Fs[i*NB + k] = F(s0 + i, s0 + k)
-  const int b = blockIdx.x;
+  T* Xt = reinterpret_cast<T*>(smem_raw);  // Xt[cl*XP + k] = X(k, c0 + cl)
+  T* Fs = Xt + kTriInvCols * XP;           // FP32: Fs[i*NB + k] = F(s0 + i, s0 + k)
+  const int b = blockIdx.x / kTriInvGroups;
+  const int cg0 = (blockIdx.x % kTriInvGroups) * kTriInvCols;  // first column of this workgroup
   const int s0 = b * NB;
   const int nb = min(NB, n - s0);
   const int tid = threadIdx.x;
-  const int c = tid >> 2, part = tid & 3;
+  const int cl = tid / LP, part = tid % LP;
+  const int c = cg0 + cl;
   const T* Fb = F + (size_t)s0 + (size_t)s0 * lda;
   if constexpr (kStage) {
     for (int idx = tid; idx < NB * NB; idx += kTriInvThreads) {
@@ -914,52 +921,44 @@ __global__ __launch_bounds__(kTriInvThreads) void tri_inv_kernel(int n, const T*
     if constexpr (kStage) return (double)Fs[i * NB + k];
     else return (double)Fb[i + (size_t)k * lda];
   };
-  T* xc = Xt + c * XP;
-  // this lane's share of sum_{k0 <= k < k1} F(i, k) X(k, c): k = part mod 4
+  T* xc = Xt + cl * XP;
+  // this lane's share of sum_{k0 <= k < k1} F(i, k) X(k, c): k = part mod LP
   auto dot = [&](int i, int k0, int k1) -> double {
     double a0 = 0.0, a1 = 0.0;
-    int k = k0 + ((part - k0) & 3);
-    for (; k + 4 < k1; k += 8) {
+    int k = k0 + ((part - k0) & (LP - 1));
+    for (; k + LP < k1; k += 2 * LP) {
       a0 += f(i, k) * (double)xc[k];
-      a1 += f(i, k + 4) * (double)xc[k + 4];
+      a1 += f(i, k + LP) * (double)xc[k + LP];
     }
     if (k < k1) a0 += f(i, k) * (double)xc[k];
     double sum = a0 + a1;
-    sum += __shfl_xor(sum, 1, 64);
-    sum += __shfl_xor(sum, 2, 64);
+#pragma unroll
+    for (int off = 1; off < LP; off <<= 1) sum += __shfl_xor(sum, off, 64);
     return sum;
   };
-  if (LOWER) {
-    for (int i = 0; i < nb; ++i) {
-      const double x = ((i == c) ? 1.0 : 0.0) - dot(i, 0, i);
-      if (part == 0) xc[i] = (T)x;
-    }
-  } else {
-    for (int i = nb - 1; i >= 0; --i) {
-      const double x = (((i == c) ? 1.0 : 0.0) - dot(i, i + 1, nb)) / f(i, i);
-      if (part == 0) xc[i] = (T)x;
+  if (c < nb) {
+    if (LOWER) {
+      for (int i = part; i < c; i += LP) xc[i] = T(0);
+      for (int i = c; i < nb; ++i) {
+        const double x = ((i == c) ? 1.0 : 0.0) - dot(i, c, i);
+        if (part == 0) xc[i] = (T)x;
+      }
+    } else {
+      for (int i = c + 1 + part; i < nb; i += LP) xc[i] = T(0);
+      for (int i = c; i >= 0; --i) {
+        const double x = (((i == c) ? 1.0 : 0.0) - dot(i, i + 1, c + 1)) / f(i, i);
+        if (part == 0) xc[i] = (T)x;
+      }
     }
   }
   __syncthreads();
   T* D = Dinv + (size_t)b * NB * NB;
   // D(i, c) = X(i, c), coalesced over i
-  for (int idx = tid; idx < NB * NB; idx += kTriInvThreads) {
+  for (int idx = tid; idx < NB * kTriInvCols; idx += kTriInvThreads) {
     const int i = idx & (NB - 1), cc = idx / NB;
-    if (i < nb) D[i + (size_t)cc * NB] = (cc < nb) ? Xt[cc * XP + i] : T(0);
+    if (i < nb) D[i + (size_t)(cg0 + cc) * NB] = (cg0 + cc < nb) ? Xt[cc * XP + i] : T(0);
   }
 }
-
-// Block triangular solve in one launch: forward with the unit-lower L
-// (LOWER) or backward with the upper U, FP64 accumulation over the factor T.
-// Workgroups take row blocks in dependency order from a ticket counter; a
-// workgroup only waits on blocks whose tickets were taken before its own by
-// workgroups already running, so the chain always progresses whatever the
-// residency. Row block rb: acc = y_rb - sum over the solved column blocks q of
-// F[rb, q] x_q, taking x_q as soon as its flag carries this launch's epoch;
-// then x_rb = Dinv_rb acc is published: the values as agent-scope stores,
-// completed, then the flag (agent-scope, relaxed loads on the reading side).
-// The 128x128 tile of F for the next column block is loaded before its flag
-// is awaited and Dinv_rb waits in LDS. Waits are bounded (abort word).
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restrict__ F, int lda,
                                                          const T* __restrict__ Dinv,
@@ -1298,7 +1297,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       if (aligned && ((s + kb) % kSolveNB == 0 || s + kb == ob)) {
         const int jb = (s + kb - 1) / kSolveNB;
         const int d0 = k0 + jb * kSolveNB;
-        hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(1), dim3(kTriInvThreads),
+        hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(kTriInvGroups), dim3(kTriInvThreads),
                            tri_inv_lds(sizeof(T)), st, std::min(kSolveNB, ob - jb * kSolveNB),
                            (const T*)(dA + (size_t)d0 + (size_t)d0 * lda), lda,
                            linv + (size_t)jb * kSolveNB * kSolveNB);
@@ -1468,10 +1467,10 @@ int ensure_dinv(bemhip_ctx* c, int n, const T* dLU, int lda) {
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     BEMHIP_HIP_TRY(c, hipFuncSetAttribute((const void*)tri_inv_kernel<T, false>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(nblk), dim3(kTriInvThreads), lds, c->stream, n,
-                       dLU, lda, DL);
-    hipLaunchKernelGGL((tri_inv_kernel<T, false>), dim3(nblk), dim3(kTriInvThreads), lds, c->stream, n,
-                       dLU, lda, DU);
+    hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(nblk * kTriInvGroups), dim3(kTriInvThreads),
+                       lds, c->stream, n, dLU, lda, DL);
+    hipLaunchKernelGGL((tri_inv_kernel<T, false>), dim3(nblk * kTriInvGroups),
+                       dim3(kTriInvThreads), lds, c->stream, n, dLU, lda, DU);
     BEMHIP_HIP_TRY(c, hipGetLastError());
     c->dinv_of = dLU;
     c->dinv_n = n;
