@@ -10,6 +10,9 @@ Python counterparts of the reference's student-facing calls:
 * ``apply_bc(dad)`` — ``applyBC`` (src/Core/Boundary_conditions.jl:20-75).
 * ``solve(dad, mode="fp64")`` — ``solve`` (src/Core/Solver.jl:24-64) with
   ``split_sol!`` (:88-98); sets ``dad.T`` (nt) and ``dad.q`` (n).
+* ``solve_Hmat(dad; atol, rtol, itmax)`` — GMRES on the device
+  (src/Core/Solver.jl:66-86), the solve of a ``precision="matrix_free"``
+  assembly (no stored H, G), which ``solve`` dispatches to.
 
 Matrices are returned to the host lazily: the device copies stay resident in
 the context (``dad.hip``); ``dad.H`` / ``dad.G`` are fetched on first access
@@ -28,7 +31,7 @@ from . import _lib
 from ._lib import check, dptr, fptr, i64ptr, iptr, lib
 from .problem import BEMData
 
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "matrix_free": 2}
 MODES = {"fp64": 0, "mixed": 1}
 
 
@@ -99,6 +102,8 @@ def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = 
         near_factor = 1.5
     if not isinstance(near_factor, (int, float)):
         raise ArgumentError(f"near_factor must be a Real, :auto, or nothing; got {near_factor!r}")
+    if precision == "matrix_free":
+        fetch = False
     nf = float(near_factor)
     if nf == float("inf"):
         nf = 1e300
@@ -182,6 +187,10 @@ def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: i
     (FP32 LU + FP64 refinement); default follows the assembly precision."""
     if dad.hip is None or "assembly_info" not in dad.cache:
         raise RuntimeError("cache.H is not set — assemble/solve first?")
+    if dad.assembly_info["precision"] == "matrix_free" and mode in (None, "gmres"):
+        return solve_Hmat(dad)
+    if mode == "gmres":
+        return solve_Hmat(dad)
     if mode is None:
         mode = "fp64" if dad.assembly_info["precision"] == "fp64" else "mixed"
     if mode not in MODES:
@@ -196,6 +205,36 @@ def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: i
     T, q = split_sol(dad, x)
     dad.set_cache(T=T, q=q, solve_info={"mode": mode, "iters": it.value, "relres": rel.value,
                                         "x": x})
+    return dad.T
+
+
+def solve_Hmat(dad: BEMData, Pl=None, atol: float = 1e-10, rtol: float = 1e-8, itmax: int = 0,
+               restart: int = 50):
+    """GMRES on the device for the applied (not stored) system, as
+    ``solve_Hmat`` (src/Core/Solver.jl:66-86): stop when ||b - A x|| <= atol +
+    rtol ||b||; ``itmax = 0`` means max(4 nt, 200) products. Right Jacobi
+    preconditioning is built in (``Pl`` is not supported). Sets dad.T, dad.q
+    and ``gmres_stats``."""
+    if Pl is not None:
+        raise ArgumentError("solve_Hmat: a user preconditioner is not supported "
+                            "(diagonal scaling is built in)")
+    if dad.hip is None or "assembly_info" not in dad.cache:
+        raise RuntimeError("cache.H is not set — assemble/solve first?")
+    apply_bc(dad)
+    ctx = dad.hip
+    itm = int(itmax) if itmax > 0 else max(4 * dad.nt, 200)
+    x = np.empty(dad.nt, dtype=np.float64)
+    it = C.c_int(0)
+    rel = C.c_double(0.0)
+    status = lib().bemhip_solve_gmres(ctx.h, float(atol), float(rtol), int(restart), itm,
+                                      dptr(x), C.byref(it), C.byref(rel))
+    converged = status == 0
+    if status not in (0, 5):          # 5: BEMHIP_ERR_NOCONV, x is still the last iterate
+        ctx.check(status)
+    T, q = split_sol(dad, x)
+    dad.set_cache(T=T, q=q, gmres_stats={"niter": it.value, "relres": rel.value,
+                                         "solved": converged},
+                  solve_info={"mode": "gmres", "iters": it.value, "relres": rel.value, "x": x})
     return dad.T
 
 

@@ -26,6 +26,7 @@ SOURCES = {
     "lu.hip": ["-mllvm", "-pragma-unroll-threshold=200000"],
     "solve.hip": [],
     "adjoint.hip": [],
+    "gmres.hip": [],
 }
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -56,7 +57,7 @@ def build(verbose: bool = False) -> str:
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
         objs = list(ex.map(compile_one, SOURCES.items()))
     tmp = OUT + ".tmp"
     cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]

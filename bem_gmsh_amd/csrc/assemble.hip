@@ -1233,6 +1233,13 @@ int launch_near_pairs(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Qua
 
 }  // namespace
 
+// x = a, y = b over n entries.
+__global__ void fill2_kernel(int n, double a, double* __restrict__ x, double b,
+                             double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) { x[i] = a; y[i] = b; }
+}
+
 // Pinned host word for the near-pair count, written by near_scan through its
 // device address.
 static hipError_t ensure_count_word(bemhip_ctx* c) {
@@ -1276,12 +1283,22 @@ int assemble_impl(bemhip_ctx* c) {
   const size_t nt = (size_t)c->nt;
   const int ncb = (c->ne + kEB - 1) / kEB;
   const bool fp64 = (c->precision == BEMHIP_FP64);
+  const bool mfree = (c->precision == BEMHIP_MATRIX_FREE);
   BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
   BEMHIP_HIP_TRY(c, c->hpart.ensure((size_t)ncb * nt));
   BEMHIP_HIP_TRY(c, c->diag64.ensure(nt));
   BEMHIP_HIP_TRY(c, c->row_cnt.ensure(nt));
   BEMHIP_HIP_TRY(c, c->row_ptr.ensure(nt + 1));
-  if (fp64) {
+  if (mfree) {
+    // no stored H, G: the far field of the diagonal's row sums comes from an
+    // FP64 operator pass with unit H weights (the same sums as the FP64 far kernel)
+    c->H64.release(); c->G64.release(); c->H32.release(); c->G32.release();
+    BEMHIP_HIP_TRY(c, c->wH.ensure(nt));
+    BEMHIP_HIP_TRY(c, c->wG.ensure(nt));
+    hipLaunchKernelGGL(fill2_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream,
+                       (int)nt, 1.0, c->wH.ptr, 0.0, c->wG.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  } else if (fp64) {
     c->H32.release(); c->G32.release();
     BEMHIP_HIP_TRY(c, c->H64.ensure(c->ld64 * nt));
     BEMHIP_HIP_TRY(c, c->G64.ensure(c->ld64 * (size_t)c->n));
@@ -1339,7 +1356,17 @@ int assemble_impl(bemhip_ctx* c) {
   int st;
   BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
-  if (fp64)
+  if (mfree) {
+    dim3 grid((unsigned)nrb, (unsigned)ncb);
+    if (c->nn == 2)
+      hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, s2, g, cs, 1, c->wH.ptr,
+                         c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
+    else
+      hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, s2, g, cs, 1, c->wH.ptr,
+                         c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    st = BEMHIP_OK;
+  } else if (fp64)
     st = (c->nn == 2) ? launch_far<double, 2>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
                       : launch_far<double, 3>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
   else
@@ -1381,7 +1408,12 @@ int assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, hipStreamWaitEvent(c->stream, ev[2], 0));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[8], c->stream));
   // join: near values into H, G and the rigid-body diagonal
-  if (fp64)
+  if (mfree)
+    hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt,
+                       c->n, c->nn, ncb, 0, (size_t)0, c->hpart.ptr, c->row_ptr.ptr,
+                       c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr,
+                       c->diag64.ptr, nullptr, (double*)nullptr, (double*)nullptr);
+  else if (fp64)
     hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt,
                        c->n, c->nn, ncb, 0, c->ld64, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                        c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr, c->diag64.ptr, nullptr,
@@ -1481,6 +1513,7 @@ int apply_bc_impl(bemhip_ctx* c) {
     int st = operator_pass(c, c->wH2.ptr, c->wG2.ptr, c->b64.ptr);
     if (st) return st;
   }
+  if (c->precision == BEMHIP_MATRIX_FREE) return BEMHIP_OK;  // A is applied, never stored
   {
     ScopedTimer t(c, "apply_bc_swap");
     dim3 grid((unsigned)std::min<size_t>((nt + 255) / 256, 64), (unsigned)nt);

@@ -256,12 +256,14 @@ def _polygon_area(v: np.ndarray) -> float:
 # ---------------------------------------------------------------------------
 
 def quadrado(ndiv: int = 10, ordem: int = 1, Lx: float = 1.0, Ly: float = 1.0,
-             nome: str = "quadrado") -> Mesh2D:
+             nome: str = "quadrado", cells: bool = True) -> Mesh2D:
     """Unit-square mesh of data/Laplace/Laplace_dad.jl:238-280 (no Gmsh).
 
     ``ndiv`` nodes per side (ndiv-1 elements), curve loop bottom, right, top,
-    left; BCs "1;0" (bottom, top), "1;-1" (right), "0;0" (left); a recombined
-    transfinite (ndiv-1)^2 quad grid as domain cells.
+    left (counter-clockwise, so the element normals point outward); BCs "1;0"
+    (bottom, top), "1;-1" (right), "0;0" (left); a recombined transfinite
+    (ndiv-1)^2 quad grid as domain cells (``cells=False``: boundary only, for
+    large ndiv).
     """
     if ndiv < 2:
         raise ValueError("ndiv must be >= 2")
@@ -285,9 +287,9 @@ def quadrado(ndiv: int = 10, ordem: int = 1, Lx: float = 1.0, Ly: float = 1.0,
     lines = _to_gmsh_order(np.array(lines))
     xs = np.linspace(0.0, Lx, m + 1)
     ys = np.linspace(0.0, Ly, m + 1)
-    cells = [np.array([[xs[i], ys[j]], [xs[i + 1], ys[j]], [xs[i + 1], ys[j + 1]],
-                       [xs[i], ys[j + 1]]]) for j in range(m) for i in range(m)]
-    return Mesh2D(nome, ordem, lines, phys, np.array(reg), cells)
+    quads = [np.array([[xs[i], ys[j]], [xs[i + 1], ys[j]], [xs[i + 1], ys[j + 1]],
+                       [xs[i], ys[j + 1]]]) for j in range(m) for i in range(m)] if cells else []
+    return Mesh2D(nome, ordem, lines, phys, np.array(reg), quads)
 
 
 class SplitMix64:

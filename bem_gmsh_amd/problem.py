@@ -52,7 +52,7 @@ class BEMData:
     """
 
     _CACHE_KEYS = ("H", "G", "A", "B", "b", "T", "q", "qsi", "w", "near_factor",
-                   "analytical", "cells", "solve_info", "assembly_info")
+                   "analytical", "cells", "solve_info", "assembly_info", "gmres_stats")
 
     def __init__(self, name: str, p: int, collocation: np.ndarray, normal: np.ndarray,
                  elem_J: np.ndarray, elem_L: np.ndarray, elem_geo: np.ndarray,

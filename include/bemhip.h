@@ -48,6 +48,8 @@ typedef struct bemhip_ctx bemhip_ctx;
 /* Assembly precision. */
 #define BEMHIP_FP64 0           /* H, G stored and computed in FP64               */
 #define BEMHIP_MIXED 1          /* far field FP32, near/singular/diagonal FP64    */
+#define BEMHIP_MATRIX_FREE 2    /* no stored H, G: near pairs and FP64 diagonal   */
+                                /* only; the system is applied on the fly (GMRES) */
 
 /* Solve modes. */
 #define BEMHIP_SOLVE_FP64 0     /* FP64 LU (MFMA trailing updates)                */
@@ -101,7 +103,11 @@ int bemhip_set_geometry(bemhip_ctx* ctx, int p, int ne, int n, int ni,
  * src/Core/Integration.jl:1597-1845) and the rigid-body diagonal (:384-390).
  *   npg          Gauss points of the near rule (>= 1, <= 64)
  *   near_factor  far-lumping cutoff factor (reference default 1.5; > 1e20 = all near)
- *   precision    BEMHIP_FP64 or BEMHIP_MIXED */
+ *   precision    BEMHIP_FP64, BEMHIP_MIXED, or BEMHIP_MATRIX_FREE (near pairs and
+ *                the FP64 rigid-body diagonal only: O(N) memory beside the pair
+ *                list; the far field is recomputed in every product, for
+ *                bemhip_solve_gmres; bemhip_get_HG and the LU solves then return
+ *                BEMHIP_ERR_STATE) */
 int bemhip_assemble(bemhip_ctx* ctx, int npg, double near_factor, int precision);
 
 /* Copy the assembled H, G to host buffers (either may be NULL), converting to
@@ -135,6 +141,24 @@ int bemhip_apply_bc(bemhip_ctx* ctx, const int64_t* BC, const double* BV, double
  * The factors stay resident (re-used by later solves on the same system). */
 int bemhip_solve(bemhip_ctx* ctx, int mode, double tol, int maxit, double* x, int* iters,
                  double* relres);
+
+/* Restarted GMRES on the device for the current system, the iterative path of
+ * solve_Hmat (src/Core/Solver.jl:66-86) for operators that are not stored:
+ * every product y = A x is the FP64 operator of bemhip_apply_operator (far
+ * field recomputed, near / on-element pairs, rigid-body diagonal), with right
+ * Jacobi preconditioning (the diagonal of A). Works after any assembly; the
+ * only solve after a BEMHIP_MATRIX_FREE assembly.
+ *   atol, rtol  stop when ||b - A x||_2 <= atol + rtol ||b||_2 (reference
+ *               defaults 1e-10, 1e-8)
+ *   restart     Krylov basis size m (1..512): GMRES(m)
+ *   itmax       cap on the products with A (> 0)
+ *   x           (out, nt) solution
+ *   iters       (out, may be NULL) products with A taken
+ *   relres      (out, may be NULL) final ||b - A x||_2 / ||b||_2 from a fresh
+ *               product
+ * Returns BEMHIP_ERR_NOCONV (x still written) when itmax is reached first. */
+int bemhip_solve_gmres(bemhip_ctx* ctx, double atol, double rtol, int restart, int itmax,
+                       double* x, int* iters, double* relres);
 
 /* y = A x in FP64 for the current system (stored A, or recomputed on the fly
  * from the FP64 kernels in MIXED mode). x, y have nt entries. Used for the

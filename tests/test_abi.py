@@ -135,3 +135,10 @@ def test_geometry_helpers_for_the_adjoint():
     d2 = B.format2d(moved, B.Laplace(1.0))
     diff = np.abs(d2.elem_geo - dad.elem_geo).max(axis=-1)
     assert np.count_nonzero(diff) == np.count_nonzero(idx == 0)
+
+
+def test_gmres_argument_errors():
+    L = _lib.lib()
+    x = np.zeros(4)
+    assert L.bemhip_solve_gmres(None, 0.0, 1e-8, 20, 10, _lib.dptr(x), None, None) == 1
+    assert b"NULL context" in L.bemhip_last_error(None)
