@@ -31,7 +31,7 @@ from . import _lib
 from ._lib import check, dptr, fptr, i64ptr, iptr, lib
 from .problem import BEMData
 
-PRECISIONS = {"fp64": 0, "mixed": 1, "matrix_free": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1, "matrix_free": 2, "fmm": 3}
 MODES = {"fp64": 0, "mixed": 1}
 
 
@@ -102,7 +102,7 @@ def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = 
         near_factor = 1.5
     if not isinstance(near_factor, (int, float)):
         raise ArgumentError(f"near_factor must be a Real, :auto, or nothing; got {near_factor!r}")
-    if precision == "matrix_free":
+    if precision in ("matrix_free", "fmm"):
         fetch = False
     nf = float(near_factor)
     if nf == float("inf"):
@@ -187,7 +187,7 @@ def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: i
     (FP32 LU + FP64 refinement); default follows the assembly precision."""
     if dad.hip is None or "assembly_info" not in dad.cache:
         raise RuntimeError("cache.H is not set — assemble/solve first?")
-    if dad.assembly_info["precision"] == "matrix_free" and mode in (None, "gmres"):
+    if dad.assembly_info["precision"] in ("matrix_free", "fmm") and mode in (None, "gmres"):
         return solve_Hmat(dad)
     if mode == "gmres":
         return solve_Hmat(dad)

@@ -27,6 +27,7 @@ SOURCES = {
     "solve.hip": [],
     "adjoint.hip": [],
     "gmres.hip": [],
+    "fmm.hip": [],
 }
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -57,7 +58,7 @@ def build(verbose: bool = False) -> str:
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(6, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(7, len(SOURCES))) as ex:
         objs = list(ex.map(compile_one, SOURCES.items()))
     tmp = OUT + ".tmp"
     cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]

@@ -136,7 +136,7 @@ template <typename TD>
 int get_HG(bemhip_ctx* c, TD* H, TD* G) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "get_HG: NULL context");
   if (!c->assembled) return fail(c, BEMHIP_ERR_STATE, "get_HG: assemble first");
-  if (c->precision == BEMHIP_MATRIX_FREE)
+  if (c->precision == BEMHIP_MATRIX_FREE || c->precision == BEMHIP_FMM)
     return fail(c, BEMHIP_ERR_STATE, "get_HG: a matrix-free assembly stores no H, G");
   hipSetDevice(c->device);
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -294,9 +294,11 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "assemble: set_geometry first");
   if (npg < 1 || npg > kMaxNpg) return fail(c, BEMHIP_ERR_ARG, "assemble: npg must be in 1..64");
   if (!(near_factor >= 0.0)) return fail(c, BEMHIP_ERR_ARG, "assemble: near_factor must be >= 0");
-  if (precision != BEMHIP_FP64 && precision != BEMHIP_MIXED && precision != BEMHIP_MATRIX_FREE)
+  if (precision != BEMHIP_FP64 && precision != BEMHIP_MIXED && precision != BEMHIP_MATRIX_FREE &&
+      precision != BEMHIP_FMM)
     return fail(c, BEMHIP_ERR_ARG,
-                "assemble: precision must be BEMHIP_FP64, BEMHIP_MIXED or BEMHIP_MATRIX_FREE");
+                "assemble: precision must be BEMHIP_FP64, BEMHIP_MIXED, BEMHIP_MATRIX_FREE or "
+                "BEMHIP_FMM");
   hipSetDevice(c->device);
   c->assembled = false;
   c->has_bc = false;
@@ -415,7 +417,7 @@ int bemhip_apply_bc(bemhip_ctx* c, const int64_t* BC, const double* BV, double* 
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "apply_bc: NULL context");
   if (!c->assembled) return fail(c, BEMHIP_ERR_STATE, "apply_bc: assemble first");
   if (!BC || !BV) return fail(c, BEMHIP_ERR_ARG, "apply_bc: NULL BC/BV");
-  if (A && c->precision == BEMHIP_MATRIX_FREE)
+  if (A && (c->precision == BEMHIP_MATRIX_FREE || c->precision == BEMHIP_FMM))
     return fail(c, BEMHIP_ERR_STATE, "apply_bc: a matrix-free assembly stores no A");
   std::vector<int> hbc(c->n);
   for (int j = 0; j < c->n; ++j) {

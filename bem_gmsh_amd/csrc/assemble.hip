@@ -1283,9 +1283,14 @@ int assemble_impl(bemhip_ctx* c) {
   const size_t nt = (size_t)c->nt;
   const int ncb = (c->ne + kEB - 1) / kEB;
   const bool fp64 = (c->precision == BEMHIP_FP64);
-  const bool mfree = (c->precision == BEMHIP_MATRIX_FREE);
+  const bool fmm = (c->precision == BEMHIP_FMM);
+  const bool mfree = (c->precision == BEMHIP_MATRIX_FREE) || fmm;
+  if (fmm) {
+    const int st0 = fmm_build(c);
+    if (st0) return st0;
+  }
   BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
-  BEMHIP_HIP_TRY(c, c->hpart.ensure((size_t)ncb * nt));
+  BEMHIP_HIP_TRY(c, c->hpart.ensure(fmm ? nt : (size_t)ncb * nt));  // FMM: one partial per row
   BEMHIP_HIP_TRY(c, c->diag64.ensure(nt));
   BEMHIP_HIP_TRY(c, c->row_cnt.ensure(nt));
   BEMHIP_HIP_TRY(c, c->row_ptr.ensure(nt + 1));
@@ -1356,7 +1361,10 @@ int assemble_impl(bemhip_ctx* c) {
   int st;
   BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
-  if (mfree) {
+  if (fmm) {
+    st = fmm_far(c, s2, c->wH.ptr, c->wG.ptr, c->hpart.ptr);  // far row sums of H
+    if (st) return st;
+  } else if (mfree) {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
     if (c->nn == 2)
       hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, s2, g, cs, 1, c->wH.ptr,
@@ -1410,7 +1418,7 @@ int assemble_impl(bemhip_ctx* c) {
   // join: near values into H, G and the rigid-body diagonal
   if (mfree)
     hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt,
-                       c->n, c->nn, ncb, 0, (size_t)0, c->hpart.ptr, c->row_ptr.ptr,
+                       c->n, c->nn, fmm ? 1 : ncb, 0, (size_t)0, c->hpart.ptr, c->row_ptr.ptr,
                        c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, nullptr, nullptr,
                        c->diag64.ptr, nullptr, (double*)nullptr, (double*)nullptr);
   else if (fp64)
@@ -1467,6 +1475,18 @@ static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, doub
   const Geo g = make_geo(c);
   const Consts cs = make_consts(c);
   const size_t nt = (size_t)c->nt;
+  const int nfb = (int)((nt + kFinRows - 1) / kFinRows);
+  if (c->precision == BEMHIP_FMM) {  // far field by the FMM, one partial per row
+    BEMHIP_HIP_TRY(c, c->part1.ensure(nt));
+    const int st = fmm_far(c, c->stream, wH, wG, c->part1.ptr);
+    if (st) return st;
+    hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt, c->n,
+                       c->nn, 1, 1, (size_t)0, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_h.ptr, c->pair_g.ptr, wH, wG, c->diag64.ptr, d_y,
+                       (double*)nullptr, (double*)nullptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    return BEMHIP_OK;
+  }
   const int ncb = (c->ne + kEB - 1) / kEB;
   BEMHIP_HIP_TRY(c, c->part1.ensure((size_t)ncb * nt));
   dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)ncb);
@@ -1476,7 +1496,6 @@ static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, doub
   else
     hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1, wH, wG,
                        nullptr, nullptr, c->part1.ptr, nullptr);
-  const int nfb = (int)((nt + kFinRows - 1) / kFinRows);
   hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt, c->n,
                      c->nn, ncb, 1, (size_t)0, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                      c->pair_h.ptr, c->pair_g.ptr, wH, wG, c->diag64.ptr, d_y,
@@ -1513,7 +1532,8 @@ int apply_bc_impl(bemhip_ctx* c) {
     int st = operator_pass(c, c->wH2.ptr, c->wG2.ptr, c->b64.ptr);
     if (st) return st;
   }
-  if (c->precision == BEMHIP_MATRIX_FREE) return BEMHIP_OK;  // A is applied, never stored
+  if (c->precision == BEMHIP_MATRIX_FREE || c->precision == BEMHIP_FMM)
+    return BEMHIP_OK;  // A is applied, never stored
   {
     ScopedTimer t(c, "apply_bc_swap");
     dim3 grid((unsigned)std::min<size_t>((nt + 255) / 256, 64), (unsigned)nt);

@@ -67,6 +67,25 @@ struct DevBuf {
   }
 };
 
+// Quadtree plan of the far-field FMM (BEMHIP_FMM assembly): boxes of levels
+// 2..L in one global index (level l from lvl_off[l]), Morton-sorted within a
+// level; leaves are the level-L boxes.
+struct FmmPlan {
+  bool built = false;
+  int L = 0, p = 0, nbox = 0, nleaf = 0;
+  double x0 = 0, y0 = 0, S = 0;       // root square [x0, x0+S) x [y0, y0+S)
+  std::vector<int> lvl_off, lvl_n;    // per level (index l), size L+1
+  DevBuf<double> center;              // [nbox][2]
+  DevBuf<int> child_lo, child_hi;     // children (global indices) of levels < L
+  DevBuf<int> parent;                 // parent (global index) of levels > 2
+  DevBuf<int> il_ptr, il_idx;         // M2L source boxes (global), CSR over boxes
+  DevBuf<int> src_lo, src_hi, tgt_lo, tgt_hi;  // per leaf: ranges of src_order / tgt_order
+  DevBuf<int> src_order, tgt_order;   // Morton order -> node / row index
+  DevBuf<int> nb_ptr, nb_idx;         // per leaf: neighbour leaves with sources (leaf indices)
+  DevBuf<double> mpole, local;        // [nbox][p+1] complex (re, im)
+  DevBuf<double> binom;               // [2p+1][2p+1]
+};
+
 // One timed GEMM launch (event pair on the launching stream, flops).
 struct GemmTiming {
   hipEvent_t a, b;
@@ -133,6 +152,9 @@ struct bemhip_ctx {
   bemhip::DevBuf<double> xsol;
   bemhip::DevBuf<double> Jn;        // n |dx/dxi| at each boundary node
   bemhip::DevBuf<double> adj_work;  // adjoint scratch (carved per call)
+
+  // far-field FMM (BEMHIP_FMM assembly)
+  bemhip::FmmPlan fmm;
 
   // interior evaluation (points, weights, its own near-pair list and rule)
   bemhip::DevBuf<bemhip::QuadConst> ev_qc;
@@ -205,6 +227,8 @@ int lu_solve_trans_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int 
                           const int* dperm, double* d_b);
 int build_perm(bemhip_ctx* c, int n, const int* dipiv, int* dperm);
 int operator_apply_trans(bemhip_ctx* c, const double* d_v, double* d_y);
+int fmm_build(bemhip_ctx* c);
+int fmm_far(bemhip_ctx* c, hipStream_t s, const double* wH, const double* wG, double* out);
 int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
                   const double* d_w, const QuadConst* d_qc, int npg, double near_factor,
                   double* d_out);

@@ -92,7 +92,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
   if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "solve: apply_bc first");
   if (mode != BEMHIP_SOLVE_FP64 && mode != BEMHIP_SOLVE_MIXED)
     return fail(c, BEMHIP_ERR_ARG, "solve: mode must be BEMHIP_SOLVE_FP64 or BEMHIP_SOLVE_MIXED");
-  if (c->precision == BEMHIP_MATRIX_FREE)
+  if (c->precision == BEMHIP_MATRIX_FREE || c->precision == BEMHIP_FMM)
     return fail(c, BEMHIP_ERR_STATE, "solve: LU solves need stored matrices; use bemhip_solve_gmres");
   if (mode == BEMHIP_SOLVE_FP64 && c->precision != BEMHIP_FP64)
     return fail(c, BEMHIP_ERR_STATE, "solve: FP64 LU needs an FP64 assembly (precision 0)");

@@ -213,10 +213,11 @@ def interior_leg(ctx, dad, npg, T, q, m_far=65536, m_near=2048, seed=3):
     return out
 
 
-def gmres_large(spec, npg, rtol=1e-10, restart=100, itmax=4000):
+def gmres_large(spec, npg, far="dense", rtol=1e-10, restart=100, itmax=4000):
     """Matrix-free assembly (near pairs + FP64 diagonal, no stored H, G) and
-    GMRES with the FP64 operator recomputed in every product, for boundaries
-    beyond the dense path's memory; accuracy against the exact field."""
+    GMRES, the far field of every product recomputed in FP64 (far "dense") or
+    evaluated by the FMM (far "fmm"), for boundaries beyond the dense path's
+    memory; accuracy against the exact field."""
     kind, size = spec.split(":")
     if kind == "square":
         dad = B.format2d(B.quadrado(ndiv=int(size), cells=False), B.Laplace(1.0),
@@ -233,10 +234,10 @@ def gmres_large(spec, npg, rtol=1e-10, restart=100, itmax=4000):
     BC = np.ascontiguousarray(dad.BC, dtype=np.int64)
     BV = np.ascontiguousarray(dad.BV, dtype=np.float64)
     t = time.perf_counter()
-    ctx.chk(ctx.L.bemhip_assemble(ctx.h, npg, 1.5, 2))
+    ctx.chk(ctx.L.bemhip_assemble(ctx.h, npg, 1.5, 3 if far == "fmm" else 2))
     ctx.chk(ctx.L.bemhip_apply_bc(ctx.h, i64ptr(BC), dptr(BV), None, None))
     ms = ctx.counters()["ms"]
-    log(f"[bench] matrix-free N={dad.nt}: assembly {ms['assemble']:.1f} ms, "
+    log(f"[bench] matrix-free ({far} far field) N={dad.nt}: assembly {ms['assemble']:.1f} ms, "
         f"rhs {ms['apply_bc_rhs']:.1f} ms")
     x = np.empty(dad.nt)
     it = C.c_int(0)
@@ -247,7 +248,7 @@ def gmres_large(spec, npg, rtol=1e-10, restart=100, itmax=4000):
     c = ctx.counters()
     T, q = B.split_sol(dad, x)
     dad.set_cache(T=T, q=q)
-    out = {"gmres_large": {"workload": workload, "converged": status == 0,
+    out = {"gmres_large": {"workload": workload, "far_field": far, "converged": status == 0,
                            "assemble_ms": c["ms"]["assemble"], "rhs_ms": c["ms"]["apply_bc_rhs"],
                            "gmres_ms": c["ms"]["gmres"], "products": it.value,
                            "relres": rel.value,
@@ -295,6 +296,8 @@ def main():
     ap.add_argument("--gmres-large", default="", metavar="SPEC",
                     help="only: matrix-free assembly + GMRES; SPEC square:NDIV (linear elements, "
                          "T = x) or star:NE (quadratic elements)")
+    ap.add_argument("--far", choices=["dense", "fmm"], default="dense",
+                    help="far field of the --gmres-large products")
     ap.add_argument("--phase", choices=["all", "assemble"], default="all",
                     help="'assemble': only assembly + BC passes (for counter runs)")
     ap.add_argument("--write-ceiling", action="store_true",
@@ -309,7 +312,7 @@ def main():
         write_ceiling(args.ne)
         return
     if args.gmres_large:
-        gmres_large(args.gmres_large, args.npg)
+        gmres_large(args.gmres_large, args.npg, far=args.far)
         return
     t0 = time.perf_counter()
     dad = build_problem(args.ne)

@@ -50,6 +50,8 @@ typedef struct bemhip_ctx bemhip_ctx;
 #define BEMHIP_MIXED 1          /* far field FP32, near/singular/diagonal FP64    */
 #define BEMHIP_MATRIX_FREE 2    /* no stored H, G: near pairs and FP64 diagonal   */
                                 /* only; the system is applied on the fly (GMRES) */
+#define BEMHIP_FMM 3            /* as MATRIX_FREE, the far field by a fast        */
+                                /* multipole method (O(N) per product)            */
 
 /* Solve modes. */
 #define BEMHIP_SOLVE_FP64 0     /* FP64 LU (MFMA trailing updates)                */
