@@ -105,11 +105,13 @@ int bemhip_set_geometry(bemhip_ctx* ctx, int p, int ne, int n, int ni,
  * src/Core/Integration.jl:1597-1845) and the rigid-body diagonal (:384-390).
  *   npg          Gauss points of the near rule (>= 1, <= 64)
  *   near_factor  far-lumping cutoff factor (reference default 1.5; > 1e20 = all near)
- *   precision    BEMHIP_FP64, BEMHIP_MIXED, or BEMHIP_MATRIX_FREE (near pairs and
- *                the FP64 rigid-body diagonal only: O(N) memory beside the pair
- *                list; the far field is recomputed in every product, for
- *                bemhip_solve_gmres; bemhip_get_HG and the LU solves then return
- *                BEMHIP_ERR_STATE) */
+ *   precision    BEMHIP_FP64, BEMHIP_MIXED, BEMHIP_MATRIX_FREE (near pairs and
+ *                the FP64 rigid-body diagonal only, no H, G; the far field is
+ *                recomputed in every product, for bemhip_solve_gmres;
+ *                bemhip_get_HG and the LU solves then return BEMHIP_ERR_STATE),
+ *                or BEMHIP_FMM (as MATRIX_FREE, the far field by a 2-D fast
+ *                multipole method, O(N) per product; BEMHIP_ERR_ARG when the
+ *                boundary is too coarse for a two-level quadtree) */
 int bemhip_assemble(bemhip_ctx* ctx, int npg, double near_factor, int precision);
 
 /* Copy the assembled H, G to host buffers (either may be NULL), converting to
