@@ -243,8 +243,7 @@ int bemhip_set_geometry(bemhip_ctx* c, int p, int ne, int n, int ni, const doubl
     return fail(c, BEMHIP_ERR_ARG, "set_geometry: NULL array");
   if (!(k > 0.0) || !std::isfinite(k)) return fail(c, BEMHIP_ERR_ARG, "set_geometry: k must be > 0");
   const int nn = p + 1, nt = n + ni;
-  if ((int64_t)nt * (int64_t)nt > (int64_t)1 << 40)
-    return fail(c, BEMHIP_ERR_ARG, "set_geometry: nt too large");
+  if (nt > (1 << 26)) return fail(c, BEMHIP_ERR_ARG, "set_geometry: nt too large");
   for (int64_t t = 0; t < (int64_t)nt * 2; ++t)
     if (!std::isfinite(coll_xy[t])) return fail(c, BEMHIP_ERR_ARG, "set_geometry: non-finite coordinate");
   for (int e = 0; e < ne; ++e)
@@ -299,6 +298,10 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
     return fail(c, BEMHIP_ERR_ARG,
                 "assemble: precision must be BEMHIP_FP64, BEMHIP_MIXED, BEMHIP_MATRIX_FREE or "
                 "BEMHIP_FMM");
+  if ((precision == BEMHIP_FP64 || precision == BEMHIP_MIXED) &&
+      (int64_t)c->nt * (int64_t)c->nt > (int64_t)1 << 40)
+    return fail(c, BEMHIP_ERR_ARG,
+                "assemble: nt too large for stored H, G (use BEMHIP_MATRIX_FREE or BEMHIP_FMM)");
   hipSetDevice(c->device);
   c->assembled = false;
   c->has_bc = false;

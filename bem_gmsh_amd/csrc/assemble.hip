@@ -575,6 +575,57 @@ __global__ __launch_bounds__(kFarThreads) void near_fill(int nt, int nwords, int
   }
 }
 
+// Near classification from candidate lists (FMM assembly): the targets of
+// leaf lf (tgt_order[tgt_lo[lf] .. tgt_hi[lf])) against the candidate
+// elements of the leaf (cand_idx[cand_ptr[lf] .. cand_ptr[lf + 1]), ascending),
+// with the pair test of near_classify (same arithmetic, same decisions).
+// FILL = false: row_cnt[i] = near pairs of row i. FILL = true: the CSR
+// pair_row / pair_elem at ptr[i] and the on-element / other pair lists, as
+// near_fill writes them.
+template <int NN, bool FILL>
+__global__ __launch_bounds__(64) void cand_near_kernel(
+    Geo g, Consts cs, const int* __restrict__ tgt_lo, const int* __restrict__ tgt_hi,
+    const int* __restrict__ tgt_order, const int* __restrict__ cand_ptr,
+    const int* __restrict__ cand_idx, int* __restrict__ row_cnt, const int* __restrict__ ptr,
+    int* __restrict__ prow, int* __restrict__ pelem, int* __restrict__ sing,
+    int* __restrict__ reg) {
+  const int lf = blockIdx.x;
+  const int nsing = max(0, min(g.nt, g.n - g.row0));
+  for (int q = tgt_lo[lf] + (int)threadIdx.x; q < tgt_hi[lf]; q += 64) {
+    const int i = tgt_order[q];
+    const double px = g.rx[i], py = g.ry[i];
+    const int own = (i < nsing) ? (i + g.row0) / NN : -1;
+    int cnt = 0;
+    int off = FILL ? ptr[i] : 0;
+    int before = min(i, nsing);
+    for (int t = cand_ptr[lf]; t < cand_ptr[lf + 1]; ++t) {
+      const int e = cand_idx[t];
+      double cx[NN], cy[NN], rx[NN], ry[NN];
+      float rxf[NN], ryf[NN], d2f[NN];
+#pragma unroll
+      for (int k = 0; k < NN; ++k) { cx[k] = g.x[e * NN + k]; cy[k] = g.y[e * NN + k]; }
+      const double L = g.elen[e];
+      const double lim = cs.nf * L;
+      const float lim2f = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
+      const int col0 = e * NN;
+      const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
+      if (!classify_pair<NN>(px, py, cx, cy, L, lim2f, cs.nf, onel, rx, ry, rxf, ryf, d2f))
+        continue;
+      if (!FILL) { ++cnt; continue; }
+      prow[off] = i;
+      pelem[off] = e;
+      if (e == own) {
+        sing[i] = off;
+        ++before;
+      } else {
+        reg[off - before] = off;
+      }
+      ++off;
+    }
+    if (!FILL) row_cnt[i] = cnt;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Near-singular and singular element integrals: device math.
 // ---------------------------------------------------------------------------
@@ -1289,7 +1340,10 @@ int assemble_impl(bemhip_ctx* c) {
     const int st0 = fmm_build(c);
     if (st0) return st0;
   }
-  BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
+  if (fmm)  // near pairs from the tree's candidates: no near bits
+    c->nearbits.release();
+  else
+    BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
   BEMHIP_HIP_TRY(c, c->hpart.ensure(fmm ? nt : (size_t)ncb * nt));  // FMM: one partial per row
   BEMHIP_HIP_TRY(c, c->diag64.ensure(nt));
   BEMHIP_HIP_TRY(c, c->row_cnt.ensure(nt));
@@ -1333,12 +1387,27 @@ int assemble_impl(bemhip_ctx* c) {
   // stream 2 (the longer chain: HBM-write bound) while stream 1 builds the
   // near-pair CSR and runs the near integrals.
   const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);  // row blocks
-  BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
-  BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
   BEMHIP_HIP_TRY(c, ensure_count_word(c));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-  BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
-  {
+  const FmmPlan& FP = c->fmm;
+  if (fmm) {
+    if (c->nn == 2)
+      hipLaunchKernelGGL((cand_near_kernel<2, false>), dim3(FP.nleaf), dim3(64), 0, c->stream, g,
+                         cs, FP.tgt_lo.ptr, FP.tgt_hi.ptr, FP.tgt_order.ptr, FP.cand_ptr.ptr,
+                         FP.cand_idx.ptr, c->row_cnt.ptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr);
+    else
+      hipLaunchKernelGGL((cand_near_kernel<3, false>), dim3(FP.nleaf), dim3(64), 0, c->stream, g,
+                         cs, FP.tgt_lo.ptr, FP.tgt_hi.ptr, FP.tgt_order.ptr, FP.cand_ptr.ptr,
+                         FP.cand_idx.ptr, c->row_cnt.ptr, nullptr, nullptr, nullptr, nullptr,
+                         nullptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  } else {
+    BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
+    BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
+    BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
+  }
+  if (!fmm) {
     const int cpb = 32;  // column blocks per classify workgroup
     dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)((ncb + cpb - 1) / cpb));
     if (c->nn == 2) {
@@ -1358,13 +1427,11 @@ int assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[3], c->stream));
   // stream 2: the far field (needs only the geometry; starts after the classify
   // so that the phases can be timed apart)
-  int st;
+  int st = BEMHIP_OK;
+  if (!fmm) {
   BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
-  if (fmm) {
-    st = fmm_far(c, s2, c->wH.ptr, c->wG.ptr, c->hpart.ptr);  // far row sums of H
-    if (st) return st;
-  } else if (mfree) {
+  if (mfree) {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
     if (c->nn == 2)
       hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, s2, g, cs, 1, c->wH.ptr,
@@ -1382,6 +1449,7 @@ int assemble_impl(bemhip_ctx* c) {
                       : launch_far<float, 3>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
   if (st) return st;
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], s2));
+  }
   hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
                      c->row_ptr.ptr, c->d_count);
   BEMHIP_HIP_TRY(c, hipGetLastError());
@@ -1396,11 +1464,31 @@ int assemble_impl(bemhip_ctx* c) {
   const int64_t nreg = (int64_t)total - nsing;
   BEMHIP_HIP_TRY(c, c->sing_idx.ensure((size_t)std::max<int64_t>(nsing, 1)));
   BEMHIP_HIP_TRY(c, c->reg_idx.ensure((size_t)std::max<int64_t>(nreg, 1)));
-  hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, c->stream, (int)nt, ncb, c->nn,
-                     0, c->n, c->nearbits.ptr, c->tiles.ptr, c->row_ptr.ptr, c->pair_row.ptr,
-                     c->pair_elem.ptr, c->sing_idx.ptr, c->reg_idx.ptr);
+  if (fmm) {
+    if (c->nn == 2)
+      hipLaunchKernelGGL((cand_near_kernel<2, true>), dim3(FP.nleaf), dim3(64), 0, c->stream, g,
+                         cs, FP.tgt_lo.ptr, FP.tgt_hi.ptr, FP.tgt_order.ptr, FP.cand_ptr.ptr,
+                         FP.cand_idx.ptr, nullptr, c->row_ptr.ptr, c->pair_row.ptr,
+                         c->pair_elem.ptr, c->sing_idx.ptr, c->reg_idx.ptr);
+    else
+      hipLaunchKernelGGL((cand_near_kernel<3, true>), dim3(FP.nleaf), dim3(64), 0, c->stream, g,
+                         cs, FP.tgt_lo.ptr, FP.tgt_hi.ptr, FP.tgt_order.ptr, FP.cand_ptr.ptr,
+                         FP.cand_idx.ptr, nullptr, c->row_ptr.ptr, c->pair_row.ptr,
+                         c->pair_elem.ptr, c->sing_idx.ptr, c->reg_idx.ptr);
+  } else {
+    hipLaunchKernelGGL(near_fill, dim3(nrb), dim3(kFarThreads), 0, c->stream, (int)nt, ncb, c->nn,
+                       0, c->n, c->nearbits.ptr, c->tiles.ptr, c->row_ptr.ptr, c->pair_row.ptr,
+                       c->pair_elem.ptr, c->sing_idx.ptr, c->reg_idx.ptr);
+  }
   BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[4], c->stream));
+  if (fmm) {  // stream 2: FMM far row sums of H (P2P needs the near-pair CSR)
+    BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[4], 0));
+    BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
+    st = fmm_far(c, s2, c->wH.ptr, c->wG.ptr, c->hpart.ptr);
+    if (st) return st;
+    BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], s2));
+  }
   // Near entries: far_kernel writes 0 into the near entries of H and G, so the
   // near values stay in the pair list and the finalize pass writes them into
   // H and G after both.

@@ -82,6 +82,7 @@ struct FmmPlan {
   DevBuf<int> src_lo, src_hi, tgt_lo, tgt_hi;  // per leaf: ranges of src_order / tgt_order
   DevBuf<int> src_order, tgt_order;   // Morton order -> node / row index
   DevBuf<int> nb_ptr, nb_idx;         // per leaf: neighbour leaves with sources (leaf indices)
+  DevBuf<int> cand_ptr, cand_idx;     // per leaf: elements with a node in its 3 x 3 neighbourhood
   DevBuf<double> mpole, local;        // [nbox][p+1] complex (re, im)
   DevBuf<double> binom;               // [2p+1][2p+1]
 };

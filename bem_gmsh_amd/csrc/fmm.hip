@@ -23,10 +23,12 @@
 // Near pairs: the leaf side is at least (near_factor + 1) x the longest
 // element, so every node of an element that is near a target (distance to the
 // element <= near_factor L_e) lies in the target's 3 x 3 leaf neighbourhood;
-// P2P skips those sources (the assembly's near bits) and the node of the row
-// itself, and the near / on-element values and the rigid-body diagonal are
-// added afterwards by the operator's finalize pass, as for the recomputed
-// far field. The result equals the far part of the dense FP64 operator to
+// P2P skips those sources (the elements of the row's near-pair list) and the
+// node of the row itself; the near / on-element values and the rigid-body
+// diagonal are added afterwards by the operator's finalize pass, as for the
+// recomputed far field. The same neighbourhoods give each leaf its candidate
+// elements for the near classification, so the FMM path needs no all-pairs
+// classification. The result equals the far part of the dense FP64 operator to
 // the FMM's truncation error.
 
 #include "common.hpp"
@@ -225,8 +227,8 @@ __global__ __launch_bounds__(64) void l2l_kernel(int box0, double rp, const doub
 
 // L2P + P2P: one wave per leaf, lanes over its targets. out[i] = Re of the
 // local expansion at z_i plus the direct sum over the sources of the 3 x 3
-// neighbour leaves, without the row's own node and the sources of elements
-// near the row (near bits [ncb][nt]).
+// neighbour leaves, without the row's own node and the sources of the
+// elements in the row's near-pair list.
 __global__ __launch_bounds__(64) void l2p_p2p_kernel(
     Geo g, double inv2pik, const double* __restrict__ wH, const double* __restrict__ wG, int leaf0,
     double r, const double* __restrict__ center, const double* __restrict__ local,
@@ -234,7 +236,8 @@ __global__ __launch_bounds__(64) void l2p_p2p_kernel(
     const int* __restrict__ tgt_order, const int* __restrict__ src_lo,
     const int* __restrict__ src_hi, const int* __restrict__ src_order,
     const int* __restrict__ nb_ptr, const int* __restrict__ nb_idx,
-    const uint32_t* __restrict__ nearbits, int nn, double* __restrict__ out) {
+    const int* __restrict__ row_ptr, const int* __restrict__ pair_elem, int nn,
+    double* __restrict__ out) {
   __shared__ double s_b[2 * (kFmmP + 1)];
   const int leaf = blockIdx.x, b = leaf0 + leaf, lane = threadIdx.x;
   if (lane <= kFmmP) {
@@ -243,7 +246,6 @@ __global__ __launch_bounds__(64) void l2p_p2p_kernel(
   }
   __syncthreads();
   const cpx c = ld2(center + 2 * (size_t)b);
-  const size_t nt = (size_t)g.nt;
   for (int q = tgt_lo[leaf] + lane; q < tgt_hi[leaf]; q += 64) {
     const int i = tgt_order[q];
     const double zx = g.x[i], zy = g.y[i];
@@ -252,13 +254,16 @@ __global__ __launch_bounds__(64) void l2p_p2p_kernel(
     cpx f = cmk(s_b[2 * kFmmP], s_b[2 * kFmmP + 1]);
     for (int l = kFmmP - 1; l >= 0; --l) f = cadd(cmul(f, t), cmk(s_b[2 * l], s_b[2 * l + 1]));
     double phi = f.re;
+    const int p0 = row_ptr[i], p1 = row_ptr[i + 1];  // the row's near elements
     for (int nbq = nb_ptr[leaf]; nbq < nb_ptr[leaf + 1]; ++nbq) {
       const int sl = nb_idx[nbq];
       for (int sq = src_lo[sl]; sq < src_hi[sl]; ++sq) {
         const int j = src_order[sq];
         if (j == i) continue;
         const int e = j / nn;
-        if ((nearbits[(size_t)(e >> 5) * nt + i] >> (e & 31)) & 1u) continue;
+        bool nr = false;
+        for (int pr = p0; pr < p1; ++pr) nr |= pair_elem[pr] == e;
+        if (nr) continue;
         double Q;
         cpx D;
         strengths(g, inv2pik, wH, wG, j, Q, D);
@@ -327,37 +332,51 @@ int fmm_build(bemhip_ctx* c) {
   P.y0 = ymin - 0.5e-9 * ext;
   const double maxL = *std::max_element(elen.begin(), elen.end());
   const double hmin = (c->near_factor + 1.0) * maxL * 1.05;
-  int Lmax = 0;
-  while (Lmax < 30 && P.S / std::ldexp(1.0, Lmax + 1) >= hmin) ++Lmax;
+  if (nt >= (1 << 24)) return fail(c, BEMHIP_ERR_ARG, "assemble: the FMM takes fewer than 2^24 rows");
+  int Lmax = 0;  // at most 20 levels: 40-bit codes beside 24-bit indices in one sort key
+  while (Lmax < 20 && P.S / std::ldexp(1.0, Lmax + 1) >= hmin) ++Lmax;
   if (Lmax < 2)
     return fail(c, BEMHIP_ERR_ARG,
                 "assemble: boundary too coarse for the FMM (use BEMHIP_MATRIX_FREE)");
-  auto leaf_code = [&](int i, int L) {
-    const double h = P.S / std::ldexp(1.0, L);
-    const uint32_t mx = (1u << L) - 1u;
-    const uint32_t ix = (uint32_t)std::min<double>(mx, std::max(0.0, std::floor((x[i] - P.x0) / h)));
-    const uint32_t iy = (uint32_t)std::min<double>(mx, std::max(0.0, std::floor((y[i] - P.y0) / h)));
-    return morton(ix, iy);
-  };
+  // Morton codes at the finest admissible level, one sort of (code, index)
+  // keys; coarser levels are the codes shifted right by two bits per level
+  const int Lm = Lmax;
+  std::vector<uint64_t> cmax(nt);
+  {
+    const double h = P.S / std::ldexp(1.0, Lm);
+    const double mx = std::ldexp(1.0, Lm) - 1.0;
+    for (int i = 0; i < nt; ++i) {
+      const double fx = std::min(mx, std::max(0.0, std::floor((x[i] - P.x0) / h)));
+      const double fy = std::min(mx, std::max(0.0, std::floor((y[i] - P.y0) / h)));
+      cmax[i] = morton((uint32_t)fx, (uint32_t)fy);
+    }
+  }
+  std::vector<uint64_t> key(nt);  // code at Lm in the high bits, index in the low 24
+  for (int i = 0; i < nt; ++i) key[i] = (cmax[i] << 24) | (uint64_t)i;
+  std::sort(key.begin(), key.end());
+  std::vector<int> tgt(nt), src;
+  src.reserve(n);
+  for (int q = 0; q < nt; ++q) {
+    tgt[q] = (int)(key[q] & 0xffffffull);
+    if (tgt[q] < n) src.push_back(tgt[q]);
+  }
   // depth: the finest level that keeps the near zone inside the neighbours,
   // stopping once the mean occupancy of the source leaves drops to kFmmLeafOcc
   int L = 2;
-  for (; L < Lmax; ++L) {
-    std::vector<uint64_t> cs(n);
-    for (int j = 0; j < n; ++j) cs[j] = leaf_code(j, L);
-    std::sort(cs.begin(), cs.end());
-    const size_t nu = std::unique(cs.begin(), cs.end()) - cs.begin();
+  for (; L < Lm; ++L) {
+    const int sh = 2 * (Lm - L);
+    size_t nu = 0;
+    uint64_t prev = ~0ull;
+    for (int j : src) {
+      const uint64_t cd = cmax[j] >> sh;
+      if (cd != prev) { ++nu; prev = cd; }
+    }
     if ((double)n / (double)nu <= kFmmLeafOcc) break;
   }
   P.L = L;
   P.p = kFmmP;
   std::vector<uint64_t> tcode(nt);
-  for (int i = 0; i < nt; ++i) tcode[i] = leaf_code(i, L);
-  std::vector<int> src(n), tgt(nt);
-  std::iota(src.begin(), src.end(), 0);
-  std::iota(tgt.begin(), tgt.end(), 0);
-  std::stable_sort(src.begin(), src.end(), [&](int a, int b) { return tcode[a] < tcode[b]; });
-  std::stable_sort(tgt.begin(), tgt.end(), [&](int a, int b) { return tcode[a] < tcode[b]; });
+  for (int i = 0; i < nt; ++i) tcode[i] = cmax[i] >> (2 * (Lm - L));
   // boxes per level (codes sorted), from the leaves up
   std::vector<std::vector<uint64_t>> codes(L + 1);
   codes[L] = tcode;  // targets include the boundary nodes (sources)
@@ -460,6 +479,23 @@ int fmm_build(bemhip_ctx* c) {
       }
     nbp[lf + 1] = (int)nbi.size();
   }
+  // candidate elements per leaf: elements with a node in the 3 x 3 neighbourhood
+  // (every element near a target of the leaf is one of them)
+  std::vector<int> cdp(nl + 1, 0), cdi;
+  {
+    std::vector<int> tmp;
+    for (int lf = 0; lf < nl; ++lf) {
+      tmp.clear();
+      for (int q = nbp[lf]; q < nbp[lf + 1]; ++q) {
+        const int sl = nbi[q];
+        for (int k = slo[sl]; k < shi[sl]; ++k) tmp.push_back(src[k] / c->nn);
+      }
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+      cdi.insert(cdi.end(), tmp.begin(), tmp.end());
+      cdp[lf + 1] = (int)cdi.size();
+    }
+  }
   // binomials C(a, b), a, b <= 2p
   constexpr int NB = 2 * kFmmP + 1;
   std::vector<double> bin((size_t)NB * NB, 0.0);
@@ -477,6 +513,7 @@ int fmm_build(bemhip_ctx* c) {
       (st = upload(c, P.tgt_lo, tlo)) || (st = upload(c, P.tgt_hi, thi)) ||
       (st = upload(c, P.src_order, src)) || (st = upload(c, P.tgt_order, tgt)) ||
       (st = upload(c, P.nb_ptr, nbp)) || (st = upload(c, P.nb_idx, nbi)) ||
+      (st = upload(c, P.cand_ptr, cdp)) || (st = upload(c, P.cand_idx, cdi)) ||
       (st = upload(c, P.binom, bin)))
     return st;
   BEMHIP_HIP_TRY(c, P.mpole.ensure(2 * (size_t)tot * (kFmmP + 1)));
@@ -486,6 +523,7 @@ int fmm_build(bemhip_ctx* c) {
   c->stats["fmm_boxes"] = tot;
   c->stats["fmm_leaves"] = nl;
   c->stats["fmm_m2l_pairs"] = (double)ili.size();
+  c->stats["fmm_candidates"] = (double)cdi.size();
   P.built = true;
   return BEMHIP_OK;
 }
@@ -520,7 +558,7 @@ int fmm_far(bemhip_ctx* c, hipStream_t s, const double* wH, const double* wG, do
   hipLaunchKernelGGL(l2p_p2p_kernel, dim3(P.nleaf), dim3(64), 0, s, g, inv2pik, wH, wG,
                      P.lvl_off[L], side(L), P.center.ptr, P.local.ptr, P.tgt_lo.ptr, P.tgt_hi.ptr,
                      P.tgt_order.ptr, P.src_lo.ptr, P.src_hi.ptr, P.src_order.ptr, P.nb_ptr.ptr,
-                     P.nb_idx.ptr, c->nearbits.ptr, c->nn, out);
+                     P.nb_idx.ptr, c->row_ptr.ptr, c->pair_elem.ptr, c->nn, out);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   return BEMHIP_OK;
 }

@@ -44,6 +44,18 @@ def test_fmm_operator_matches_fp64(case):
     assert eb <= 1e-11 and ey <= 1e-11
 
 
+@pytest.mark.parametrize("case", list(CASES))
+def test_fmm_near_pairs_equal_all_pairs_classification(case):
+    """The FMM assembly classifies only the elements with a node in the target's
+    3 x 3 leaf neighbourhood; its pair list (rows, elements, values) equals the
+    all-pairs classification's exactly."""
+    dm, df = CASES[case](), CASES[case]()
+    B.assemble(dm, 16, precision="matrix_free")
+    B.assemble(df, 16, precision="fmm")
+    for a, b in zip(B.near_pairs(dm), B.near_pairs(df)):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("case", ["square_p1_internal", "star_p2"])
 def test_fmm_gmres_matches_lu(case):
     d64, df = CASES[case](), CASES[case]()
@@ -61,3 +73,16 @@ def test_fmm_needs_a_fine_boundary():
     dad = square(ndiv=5)
     with pytest.raises(ValueError):
         B.assemble(dad, 12, precision="fmm")
+
+
+def test_stored_modes_refuse_what_fmm_assembles():
+    """N > 2^20 nodes: H and G would take > 8 TiB, so the stored modes refuse
+    the size at assembly, while the FMM mode assembles it (O(N) memory)."""
+    dad = B.format2d(B.quadrado(ndiv=262146, cells=False), B.Laplace(1.0),
+                     pontointerno=False)
+    assert dad.nt > 1 << 20
+    with pytest.raises(ValueError, match="too large"):
+        B.assemble(dad, 16, precision="fp64", fetch=False)
+    B.assemble(dad, 16, precision="fmm")
+    st = dad.hip.counters()["stats"]
+    assert st["fmm_levels"] >= 8 and st["fmm_candidates"] > 0
