@@ -7,11 +7,11 @@
     rel_error(dad)
 """
 
-from .problem import BEMData, Element, Laplace
+from .problem import BEMData, Elasticity, Element, Laplace, n_dof
 from .mesh import (Mesh2D, SplitMix64, ellipse_mesh, format2d, formatdata, move_geometric_node,
-                   n_eff, quadrado, read_msh, star_mesh, star_param_jacobian, star_params,
-                   write_msh)
-from .analytical import (AnalyticalSolution, ana_exp_cos, ana_laplace_linear,
+                   n_eff, parse_bc_pairs, quadrado, quadrado_elasticity, read_msh, star_mesh,
+                   star_param_jacobian, star_params, write_msh)
+from .analytical import (AnalyticalSolution, ana_elasticity_patch, ana_exp_cos, ana_laplace_linear,
                          ana_quadratic_mixed, analytical, analytical_flux, apply_analytical_bc,
                          apply_bc_from_physical, attach_analytical, rel_error, rel_error_flux)
 from .api import (ArgumentError, HipContext, adjoint, apply_bc, apply_operator, assemble,
@@ -21,10 +21,11 @@ from .interior import (gera_p_in, interior_laplace, internal_grid, internal_grid
                        internal_layer_, point_in_domain, set_internal_nodes)
 
 __all__ = [
-    "BEMData", "Element", "Laplace", "Mesh2D", "SplitMix64", "ellipse_mesh", "format2d",
-    "formatdata", "move_geometric_node", "n_eff", "quadrado", "read_msh", "star_mesh",
+    "BEMData", "Elasticity", "Element", "Laplace", "n_dof", "Mesh2D", "SplitMix64",
+    "ellipse_mesh", "format2d", "formatdata", "move_geometric_node", "n_eff", "parse_bc_pairs",
+    "quadrado", "quadrado_elasticity", "read_msh", "star_mesh",
     "star_param_jacobian", "star_params", "write_msh",
-    "AnalyticalSolution", "ana_exp_cos", "ana_laplace_linear", "ana_quadratic_mixed",
+    "AnalyticalSolution", "ana_elasticity_patch", "ana_exp_cos", "ana_laplace_linear", "ana_quadratic_mixed",
     "analytical", "analytical_flux", "apply_analytical_bc", "apply_bc_from_physical",
     "attach_analytical", "rel_error", "rel_error_flux", "ArgumentError", "HipContext",
     "adjoint", "apply_bc", "apply_operator", "assemble", "get_HG", "lu_factor", "lu_solve",

@@ -34,6 +34,7 @@ SIGNATURES = [
     ("bemhip_set_geometry", C.c_int,
      [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, c_double_p, c_double_p, c_double_p,
       c_double_p, c_double_p, c_double_p, C.c_double]),
+    ("bemhip_set_elasticity", C.c_int, [C.c_void_p, C.c_double, C.c_double]),
     ("bemhip_assemble", C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int]),
     ("bemhip_get_HG", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     ("bemhip_get_HG_f32", C.c_int, [C.c_void_p, c_float_p, c_float_p]),

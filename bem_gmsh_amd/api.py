@@ -10,6 +10,11 @@ Python counterparts of the reference's student-facing calls:
 * ``apply_bc(dad)`` — ``applyBC`` (src/Core/Boundary_conditions.jl:20-75).
 * ``solve(dad, mode="fp64")`` — ``solve`` (src/Core/Solver.jl:24-64) with
   ``split_sol!`` (:88-98); sets ``dad.T`` (nt) and ``dad.q`` (n).
+* Elasticity (``format2d(mesh, Elasticity(...))``): the same calls build the
+  2-D Kelvin system (H_G_full_direct for a Vectorial problem, :120-175; the
+  per-dof applyBC, src/Core/Boundary_conditions.jl:153-175; solve and
+  split_sol!, src/Core/Solver.jl:115-146) and set ``dad.u``, ``dad.traction``
+  (2n, node j's component b at 2j + b) and ``dad.T = dad.u``.
 * ``solve_Hmat(dad; atol, rtol, itmax)`` — GMRES on the device
   (src/Core/Solver.jl:66-86), the solve of a ``precision="matrix_free"``
   assembly (no stored H, G), which ``solve`` dispatches to.
@@ -29,7 +34,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, dptr, fptr, i64ptr, iptr, lib
-from .problem import BEMData
+from .problem import BEMData, Elasticity
 
 PRECISIONS = {"fp64": 0, "mixed": 1, "matrix_free": 2, "fmm": 3}
 MODES = {"fp64": 0, "mixed": 1}
@@ -86,9 +91,13 @@ def upload_geometry(dad: BEMData, ctx: HipContext) -> None:
     ew = np.ascontiguousarray(dad.elem_weight, dtype=np.float64)
     geo = np.ascontiguousarray(dad.elem_geo.reshape(-1, 2), dtype=np.float64)
     elen = np.ascontiguousarray(dad.elem_L, dtype=np.float64)
+    elastic = isinstance(dad.properties, Elasticity)
+    k = 1.0 if elastic else float(dad.properties.k)
     ctx.check(lib().bemhip_set_geometry(ctx.h, dad.p, dad.ne, dad.n, dad.ni, dptr(coll),
-                                        dptr(nrm), dptr(J), dptr(ew), dptr(geo), dptr(elen),
-                                        float(dad.properties.k)))
+                                        dptr(nrm), dptr(J), dptr(ew), dptr(geo), dptr(elen), k))
+    if elastic:
+        pr = dad.properties
+        ctx.check(lib().bemhip_set_elasticity(ctx.h, float(pr.mu), float(pr.effective_nu())))
 
 
 def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = "fp64",
@@ -104,6 +113,8 @@ def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = 
         raise ArgumentError(f"near_factor must be a Real, :auto, or nothing; got {near_factor!r}")
     if precision in ("matrix_free", "fmm"):
         fetch = False
+    if isinstance(dad.properties, Elasticity) and precision != "fp64":
+        raise ArgumentError(f"elasticity is assembled in FP64 (precision='fp64'); got {precision!r}")
     nf = float(near_factor)
     if nf == float("inf"):
         nf = 1e300
@@ -111,7 +122,7 @@ def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = 
     upload_geometry(dad, ctx)
     ctx.check(lib().bemhip_assemble(ctx.h, int(npg), nf, PRECISIONS[precision]))
     qsi, w = _lib.gauss_legendre(int(npg))
-    for key in ("H", "G", "A", "B", "b", "T", "q"):
+    for key in ("H", "G", "A", "B", "b", "T", "q", "u", "traction"):
         dad.cache.pop(key, None)
     dad.set_cache(qsi=qsi, w=w, near_factor=float(near_factor),
                   assembly_info={"precision": precision, "npg": int(npg)})
@@ -122,10 +133,11 @@ def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = 
 
 
 def get_HG(dad: BEMData):
-    """Copy H, G from the device into the cache (column-major nt x nt, nt x n)."""
+    """Copy H, G from the device into the cache (column-major nt x nt, nt x n;
+    elasticity 2nt x 2nt, 2nt x 2n)."""
     ctx = dad.hip
     prec = dad.assembly_info["precision"]
-    nt, n = dad.nt, dad.n
+    nt, n = dad.dof * dad.nt, dad.dof * dad.n
     if prec == "fp64":
         H = np.empty((nt, nt), dtype=np.float64, order="F")
         G = np.empty((nt, n), dtype=np.float64, order="F")
@@ -145,10 +157,11 @@ def near_pairs(dad: BEMData):
     ctx.check(lib().bemhip_near_pairs(ctx.h, C.byref(cnt)))
     m = cnt.value
     nn = dad.p + 1
+    shape = (m, nn) if dad.dof == 1 else (m, nn, 2, 2)  # elasticity: [pair][node][a][b]
     rows = np.empty(m, dtype=np.int32)
     elems = np.empty(m, dtype=np.int32)
-    h = np.empty((m, nn), dtype=np.float64)
-    g = np.empty((m, nn), dtype=np.float64)
+    h = np.empty(shape, dtype=np.float64)
+    g = np.empty(shape, dtype=np.float64)
     if m:
         ctx.check(lib().bemhip_get_near_pairs(ctx.h, iptr(rows), iptr(elems), dptr(h), dptr(g)))
     return rows, elems, h, g
@@ -161,9 +174,10 @@ def apply_bc(dad: BEMData, fetch: bool = False) -> None:
         raise RuntimeError("cache.H is not set — assemble/solve first?")
     BC = np.ascontiguousarray(dad.BC, dtype=np.int64)
     BV = np.ascontiguousarray(dad.BV, dtype=np.float64)
-    b = np.empty(dad.nt, dtype=np.float64)
+    ns = dad.dof * dad.nt
+    b = np.empty(ns, dtype=np.float64)
     if fetch:
-        A = np.empty((dad.nt, dad.nt), dtype=np.float64, order="F")
+        A = np.empty((ns, ns), dtype=np.float64, order="F")
         ctx.check(lib().bemhip_apply_bc(ctx.h, i64ptr(BC), dptr(BV), dptr(A), dptr(b)))
         dad.set_cache(A=A)
     else:
@@ -172,7 +186,14 @@ def apply_bc(dad: BEMData, fetch: bool = False) -> None:
 
 
 def split_sol(dad: BEMData, x: np.ndarray):
-    """split_sol! (src/Core/Solver.jl:88-98): x -> (T over nt, q over n)."""
+    """split_sol! (src/Core/Solver.jl:88-98): x -> (T over nt, q over n);
+    elasticity (:137-146): x -> (u, traction) over the 2n boundary dofs."""
+    if isinstance(dad.properties, Elasticity):
+        nb = 2 * dad.n
+        d = dad.BC == 0
+        u = np.where(d, dad.BV, x[:nb])
+        t = np.where(d, x[:nb], dad.BV)
+        return u, t
     T = x[: dad.nt].copy()
     q = np.zeros(dad.n)
     dirichlet = dad.BC == 0
@@ -197,11 +218,16 @@ def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: i
         raise ArgumentError(f"mode must be 'fp64' or 'mixed'; got {mode!r}")
     apply_bc(dad)
     ctx = dad.hip
-    x = np.empty(dad.nt, dtype=np.float64)
+    x = np.empty(dad.dof * dad.nt, dtype=np.float64)
     it = C.c_int(0)
     rel = C.c_double(0.0)
     ctx.check(lib().bemhip_solve(ctx.h, MODES[mode], float(tol), int(maxit), dptr(x),
                                  C.byref(it), C.byref(rel)))
+    if isinstance(dad.properties, Elasticity):
+        u, t = split_sol(dad, x)
+        dad.set_cache(u=u, traction=t, T=u,
+                      solve_info={"mode": mode, "iters": it.value, "relres": rel.value, "x": x})
+        return dad.u
     T, q = split_sol(dad, x)
     dad.set_cache(T=T, q=q, solve_info={"mode": mode, "iters": it.value, "relres": rel.value,
                                         "x": x})
@@ -220,6 +246,8 @@ def solve_Hmat(dad: BEMData, Pl=None, atol: float = 1e-10, rtol: float = 1e-8, i
                             "(diagonal scaling is built in)")
     if dad.hip is None or "assembly_info" not in dad.cache:
         raise RuntimeError("cache.H is not set — assemble/solve first?")
+    if isinstance(dad.properties, Elasticity):
+        raise ArgumentError("solve_Hmat: Laplace problems only (elasticity uses solve)")
     apply_bc(dad)
     ctx = dad.hip
     itm = int(itmax) if itmax > 0 else max(4 * dad.nt, 200)

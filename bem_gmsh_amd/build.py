@@ -3,7 +3,7 @@
     python -m bem_gmsh_amd.build            # compile csrc/*.hip -> bem_gmsh_amd/libbemhip.so
 
 Every translation unit is compiled with packed-FP32 instructions disabled.
-assemble.hip is compiled with -ffp-contract=off (FP64 parity path: no fused
+assemble.hip and elastic.hip are compiled with -ffp-contract=off (FP64 parity path: no fused
 multiply-add where the reference's FP64 arithmetic has none).
 """
 
@@ -28,6 +28,7 @@ SOURCES = {
     "adjoint.hip": [],
     "gmres.hip": [],
     "fmm.hip": [],
+    "elastic.hip": ["-ffp-contract=off"],
 }
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

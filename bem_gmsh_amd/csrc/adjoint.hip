@@ -462,6 +462,7 @@ extern "C" int bemhip_adjoint(bemhip_ctx* c, const double* q_target, const doubl
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "adjoint: NULL context");
   if (!q_target || !dJ_dgeo) return fail(c, BEMHIP_ERR_ARG, "adjoint: NULL q_target or dJ_dgeo");
   if (!c->has_sol) return fail(c, BEMHIP_ERR_STATE, "adjoint: solve first");
+  if (c->dof != 1) return fail(c, BEMHIP_ERR_STATE, "adjoint: Laplace problems only");
   if (c->sol_mode == BEMHIP_SOLVE_FP64 ? !c->factored64 : !c->factored32)
     return fail(c, BEMHIP_ERR_STATE, "adjoint: the factors of the last solve are gone");
   hipSetDevice(c->device);

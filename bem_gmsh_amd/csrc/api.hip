@@ -140,7 +140,7 @@ int get_HG(bemhip_ctx* c, TD* H, TD* G) {
     return fail(c, BEMHIP_ERR_STATE, "get_HG: a matrix-free assembly stores no H, G");
   hipSetDevice(c->device);
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
-  const size_t nt = c->nt, n = c->n;
+  const size_t nt = c->nsys(), n = c->nbnd();
   int st = 0;
   if (c->precision == BEMHIP_FP64) {
     if (H && (st = copy_out(c, H, c->H64.ptr, nt, nt, c->ld64))) return st;
@@ -256,6 +256,8 @@ int bemhip_set_geometry(bemhip_ctx* c, int p, int ne, int n, int ni, const doubl
   c->factored32 = c->factored64 = false;
   c->has_sol = false;
   c->p = p; c->nn = nn; c->ne = ne; c->n = n; c->ni = ni; c->nt = nt; c->k = k;
+  c->dof = 1;  // Laplace until bemhip_set_elasticity
+  c->mu = c->nu = 0.0;
   c->ld32 = padded_ld(nt, 64);
   c->ld64 = padded_ld(nt, 32);
   c->h_elem_w.assign(elem_w, elem_w + nn);
@@ -288,10 +290,33 @@ int bemhip_set_geometry(bemhip_ctx* c, int p, int ne, int n, int ni, const doubl
   return BEMHIP_OK;
 }
 
+int bemhip_set_elasticity(bemhip_ctx* c, double mu, double nu) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "set_elasticity: NULL context");
+  if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "set_elasticity: set_geometry first");
+  if (!(mu > 0.0) || !std::isfinite(mu))
+    return fail(c, BEMHIP_ERR_ARG, "set_elasticity: mu must be finite and > 0");
+  if (!(nu > -1.0 && nu < 0.5))
+    return fail(c, BEMHIP_ERR_ARG, "set_elasticity: nu must be in (-1, 0.5)");
+  if ((int64_t)2 * c->nt > (int64_t)1 << 26)
+    return fail(c, BEMHIP_ERR_ARG, "set_elasticity: nt too large");
+  c->assembled = false;
+  c->has_bc = false;
+  c->factored32 = c->factored64 = false;
+  c->has_sol = false;
+  c->dof = 2;
+  c->mu = mu;
+  c->nu = nu;
+  c->ld32 = padded_ld(2 * c->nt, 64);
+  c->ld64 = padded_ld(2 * c->nt, 32);
+  return BEMHIP_OK;
+}
+
 int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "assemble: NULL context");
   if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "assemble: set_geometry first");
   if (npg < 1 || npg > kMaxNpg) return fail(c, BEMHIP_ERR_ARG, "assemble: npg must be in 1..64");
+  if (c->dof == 2 && precision != BEMHIP_FP64)
+    return fail(c, BEMHIP_ERR_ARG, "assemble: elasticity is assembled in FP64 (BEMHIP_FP64)");
   if (!(near_factor >= 0.0)) return fail(c, BEMHIP_ERR_ARG, "assemble: near_factor must be >= 0");
   if (precision != BEMHIP_FP64 && precision != BEMHIP_MIXED && precision != BEMHIP_MATRIX_FREE &&
       precision != BEMHIP_FMM)
@@ -299,7 +324,7 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
                 "assemble: precision must be BEMHIP_FP64, BEMHIP_MIXED, BEMHIP_MATRIX_FREE or "
                 "BEMHIP_FMM");
   if ((precision == BEMHIP_FP64 || precision == BEMHIP_MIXED) &&
-      (int64_t)c->nt * (int64_t)c->nt > (int64_t)1 << 40)
+      (int64_t)c->nsys() * (int64_t)c->nsys() > (int64_t)1 << 40)
     return fail(c, BEMHIP_ERR_ARG,
                 "assemble: nt too large for stored H, G (use BEMHIP_MATRIX_FREE or BEMHIP_FMM)");
   hipSetDevice(c->device);
@@ -319,7 +344,7 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   hipEventCreate(&t0);
   hipEventCreate(&t1);
   hipEventRecord(t0, c->stream);
-  int st = assemble_impl(c);
+  int st = (c->dof == 2) ? kelvin_assemble_impl(c) : assemble_impl(c);
   hipEventRecord(t1, c->stream);
   hipEventSynchronize(t1);
   float ms = 0.f;
@@ -338,6 +363,7 @@ int bemhip_interior_laplace(bemhip_ctx* c, int npts, const double* pts_xy, const
                             const double* q, int npg, double near_factor, double* u) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "interior_laplace: NULL context");
   if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "interior_laplace: set_geometry first");
+  if (c->dof != 1) return fail(c, BEMHIP_ERR_STATE, "interior_laplace: Laplace problems only");
   if (npts < 0) return fail(c, BEMHIP_ERR_ARG, "interior_laplace: npts must be >= 0");
   if (npg < 1 || npg > kMaxNpg)
     return fail(c, BEMHIP_ERR_ARG, "interior_laplace: npg must be in 1..64");
@@ -408,11 +434,12 @@ int bemhip_get_near_pairs(bemhip_ctx* c, int* row, int* elem, double* h, double*
   hipSetDevice(c->device);
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   const size_t np = (size_t)c->npairs;
+  const size_t per = (size_t)c->nn * c->dof * c->dof;  // values per pair
   if (np == 0) return BEMHIP_OK;
   if (row) BEMHIP_HIP_TRY(c, hipMemcpy(row, c->pair_row.ptr, sizeof(int) * np, hipMemcpyDeviceToHost));
   if (elem) BEMHIP_HIP_TRY(c, hipMemcpy(elem, c->pair_elem.ptr, sizeof(int) * np, hipMemcpyDeviceToHost));
-  if (h) BEMHIP_HIP_TRY(c, hipMemcpy(h, c->pair_h.ptr, sizeof(double) * np * c->nn, hipMemcpyDeviceToHost));
-  if (g) BEMHIP_HIP_TRY(c, hipMemcpy(g, c->pair_g.ptr, sizeof(double) * np * c->nn, hipMemcpyDeviceToHost));
+  if (h) BEMHIP_HIP_TRY(c, hipMemcpy(h, c->pair_h.ptr, sizeof(double) * np * per, hipMemcpyDeviceToHost));
+  if (g) BEMHIP_HIP_TRY(c, hipMemcpy(g, c->pair_g.ptr, sizeof(double) * np * per, hipMemcpyDeviceToHost));
   return BEMHIP_OK;
 }
 
@@ -422,8 +449,9 @@ int bemhip_apply_bc(bemhip_ctx* c, const int64_t* BC, const double* BV, double* 
   if (!BC || !BV) return fail(c, BEMHIP_ERR_ARG, "apply_bc: NULL BC/BV");
   if (A && (c->precision == BEMHIP_MATRIX_FREE || c->precision == BEMHIP_FMM))
     return fail(c, BEMHIP_ERR_STATE, "apply_bc: a matrix-free assembly stores no A");
-  std::vector<int> hbc(c->n);
-  for (int j = 0; j < c->n; ++j) {
+  const int nbc = c->nbnd();
+  std::vector<int> hbc(nbc);
+  for (int j = 0; j < nbc; ++j) {
     if (BC[j] != 0 && BC[j] != 1)
       return fail(c, BEMHIP_ERR_ARG, "apply_bc: BC codes must be 0 (Dirichlet) or 1 (Neumann)");
     if (!std::isfinite(BV[j])) return fail(c, BEMHIP_ERR_ARG, "apply_bc: non-finite BV");
@@ -433,19 +461,19 @@ int bemhip_apply_bc(bemhip_ctx* c, const int64_t* BC, const double* BV, double* 
   c->has_bc = false;
   c->factored32 = c->factored64 = false;
   c->has_sol = false;
-  BEMHIP_HIP_TRY(c, c->d_bc.ensure(c->n));
-  BEMHIP_HIP_TRY(c, c->d_bv.ensure(c->n));
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->d_bc.ptr, hbc.data(), sizeof(int) * c->n,
+  BEMHIP_HIP_TRY(c, c->d_bc.ensure(nbc));
+  BEMHIP_HIP_TRY(c, c->d_bv.ensure(nbc));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->d_bc.ptr, hbc.data(), sizeof(int) * nbc,
                                    hipMemcpyHostToDevice, c->stream));
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->d_bv.ptr, BV, sizeof(double) * c->n,
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->d_bv.ptr, BV, sizeof(double) * nbc,
                                    hipMemcpyHostToDevice, c->stream));
-  c->h_bc.assign(BC, BC + c->n);
-  c->h_bv.assign(BV, BV + c->n);
+  c->h_bc.assign(BC, BC + nbc);
+  c->h_bv.assign(BV, BV + nbc);
   int st = apply_bc_impl(c);
   if (st) return st;
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->has_bc = true;
-  const size_t nt = c->nt;
+  const size_t nt = c->nsys();
   if (b) BEMHIP_HIP_TRY(c, hipMemcpy(b, c->b64.ptr, sizeof(double) * nt, hipMemcpyDeviceToHost));
   if (A) {
     if (c->precision == BEMHIP_FP64) {
@@ -461,7 +489,7 @@ int bemhip_apply_operator(bemhip_ctx* c, const double* x, double* y) {
   if (!c || !x || !y) return fail(c, BEMHIP_ERR_ARG, "apply_operator: NULL argument");
   if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "apply_operator: apply_bc first");
   hipSetDevice(c->device);
-  const size_t nt = c->nt;
+  const size_t nt = c->nsys();
   BEMHIP_HIP_TRY(c, c->vec64a.ensure(nt));
   BEMHIP_HIP_TRY(c, c->vec64b.ensure(nt));
   BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->vec64a.ptr, x, sizeof(double) * nt, hipMemcpyHostToDevice,
@@ -479,7 +507,7 @@ int bemhip_counters(bemhip_ctx* c, char* json, size_t cap) {
   std::ostringstream os;
   os.precision(9);
   os << "{\"device\":" << c->device << ",\"num_cu\":" << c->num_cu << ",\"p\":" << c->p
-     << ",\"ne\":" << c->ne << ",\"n\":" << c->n << ",\"nt\":" << c->nt
+     << ",\"ne\":" << c->ne << ",\"n\":" << c->n << ",\"nt\":" << c->nt << ",\"dof\":" << c->dof
      << ",\"precision\":" << c->precision << ",\"npg\":" << c->npg << ",\"ms\":{";
   bool first = true;
   for (auto& kv : c->ms) {

@@ -25,6 +25,7 @@
 // the un-fused operation order of the reference's FP64 arithmetic.
 
 #include "common.hpp"
+#include "assembly_shared.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -32,232 +33,6 @@
 namespace bemhip {
 
 namespace {
-
-constexpr double kPi = 3.141592653589793;
-constexpr double kEps = 2.220446049250313e-16;
-
-// ---------------------------------------------------------------------------
-// Classification of one (row, element) pair. rx/ry: node - source; d2 per node.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double dist_chord(double px, double py, double ax, double ay,
-                                             double bx, double by) {
-  double abx = bx - ax, aby = by - ay;
-  double t = ((px - ax) * abx + (py - ay) * aby) / ((abx * abx + aby * aby) + kEps);
-  t = t > 1.0 ? 1.0 : (t < 0.0 ? 0.0 : t);
-  double qx = ax + t * abx, qy = ay + t * aby;
-  double dx = px - qx, dy = py - qy;
-  return sqrt(dx * dx + dy * dy);
-}
-
-template <int NN>
-__device__ __forceinline__ bool is_near(double px, double py, const double* cx,
-                                        const double* cy, const double* d2, double L,
-                                        double nf) {
-  if (nf > 1e20) return true;  // includes Inf
-  double lim = nf * L;
-  double lim2 = lim * lim;
-  double dn2 = INFINITY;
-  double lox = cx[0], loy = cy[0], hix = cx[0], hiy = cy[0];
-#pragma unroll
-  for (int k = 0; k < NN; ++k) {
-    lox = fmin(lox, cx[k]); loy = fmin(loy, cy[k]);
-    hix = fmax(hix, cx[k]); hiy = fmax(hiy, cy[k]);
-    if (d2[k] < dn2) dn2 = d2[k];
-  }
-  if (dn2 < lim2) return true;
-  double s = 0.0;
-  if (px < lox) { double d = lox - px; s += d * d; }
-  else if (px > hix) { double d = px - hix; s += d * d; }
-  if (py < loy) { double d = loy - py; s += d * d; }
-  else if (py > hiy) { double d = py - hiy; s += d * d; }
-  if (sqrt(s) >= lim) return false;
-  double d = INFINITY;
-#pragma unroll
-  for (int k = 0; k + 1 < NN; ++k)
-    d = fmin(d, dist_chord(px, py, cx[k], cy[k], cx[k + 1], cy[k + 1]));
-  return d < lim;
-}
-
-struct Consts {
-  double cU;  // 0.5/(2 pi k)
-  double cT;  // 1/(2 pi)
-  double nf;  // near factor
-};
-
-// Conservative FP32 pre-test: true only when the pair is far by a relative
-// margin (lim2m = lim^2 (1 + 1e-3)) on both the node-distance and the
-// bounding-box tests, so that the exact FP64 test (is_near) would also call
-// it far. r*f are the FP32 roundings of the FP64 node - source offsets.
-template <int NN>
-__device__ __forceinline__ bool clearly_far(const float* rxf, const float* ryf, const float* d2f,
-                                            float lim2m) {
-  float dn2 = d2f[0], lox = rxf[0], hix = rxf[0], loy = ryf[0], hiy = ryf[0];
-#pragma unroll
-  for (int k = 1; k < NN; ++k) {
-    dn2 = fminf(dn2, d2f[k]);
-    lox = fminf(lox, rxf[k]); hix = fmaxf(hix, rxf[k]);
-    loy = fminf(loy, ryf[k]); hiy = fmaxf(hiy, ryf[k]);
-  }
-  const float dx = lox > 0.f ? lox : (hix < 0.f ? -hix : 0.f);
-  const float dy = loy > 0.f ? loy : (hiy < 0.f ? -hiy : 0.f);
-  return dn2 > lim2m && (dx * dx + dy * dy) > lim2m;
-}
-
-// 1/x in FP64 from the FP32 reciprocal and two Newton steps (about 1 ulp).
-__device__ __forceinline__ double rcp_f64(double x) {
-  double r = (double)__builtin_amdgcn_rcpf((float)x);
-  r = fma(r, fma(-x, r, 1.0), r);
-  return fma(r, fma(-x, r, 1.0), r);
-}
-
-// Classification of pair (row at (px, py), element with collocation nodes
-// cx/cy): the node - source offsets (FP64 and their FP32 roundings) and the
-// near decision: on-element pairs are near; otherwise the FP32 pre-test settles
-// the clearly-far pairs and the rest take the exact FP64 test. Shared by
-// near_classify and far_kernel so that both make the same decision.
-template <int NN>
-__device__ __forceinline__ bool classify_pair(double px, double py, const double* cx,
-                                              const double* cy, double L, float lim2f,
-                                              double nf, bool onel, double* rx, double* ry,
-                                              float* rxf, float* ryf, float* d2f) {
-#pragma unroll
-  for (int k = 0; k < NN; ++k) {
-    rx[k] = cx[k] - px;
-    ry[k] = cy[k] - py;
-    rxf[k] = (float)rx[k];
-    ryf[k] = (float)ry[k];
-    d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
-  }
-  if (onel) return true;
-  if (clearly_far<NN>(rxf, ryf, d2f, lim2f)) return false;
-  double d2[NN];
-#pragma unroll
-  for (int k = 0; k < NN; ++k) d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
-  return is_near<NN>(px, py, cx, cy, d2, L, nf);
-}
-
-// Element data of one 32-element column block in LDS for the classification.
-template <int NN>
-__device__ __forceinline__ void stage_class_block(const Geo& g, const Consts& cs, int e0,
-                                                  int ecount, double* s_cx, double* s_cy,
-                                                  double* s_L, float* s_lim2f) {
-  for (int t = threadIdx.x; t < ecount * NN; t += blockDim.x) {
-    const int j = e0 * NN + t;
-    s_cx[t] = g.x[j];
-    s_cy[t] = g.y[j];
-  }
-  for (int t = threadIdx.x; t < ecount; t += blockDim.x) {
-    const double L = g.elen[e0 + t];
-    s_L[t] = L;
-    const double lim = cs.nf * L;
-    // +inf (no far lumping) when the cutoff is disabled or overflows
-    s_lim2f[t] = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Bounding box of the collocation nodes of each 32-element column block and
-// its largest cutoff nf * L: box[cb] = {lox, loy, hix, hiy, maxlim}.
-// ---------------------------------------------------------------------------
-template <int NN>
-__global__ void elem_block_boxes(Geo g, Consts cs, int ncb, double* __restrict__ box) {
-  const int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cb >= ncb) return;
-  const int e0 = cb * kEB, e1 = min(g.ne, e0 + kEB);
-  double lox = INFINITY, loy = INFINITY, hix = -INFINITY, hiy = -INFINITY, ml = 0.0;
-  for (int e = e0; e < e1; ++e) {
-    for (int k = 0; k < NN; ++k) {
-      const double x = g.x[e * NN + k], y = g.y[e * NN + k];
-      lox = fmin(lox, x); hix = fmax(hix, x);
-      loy = fmin(loy, y); hiy = fmax(hiy, y);
-    }
-    ml = fmax(ml, cs.nf * g.elen[e]);
-  }
-  double* b = box + (size_t)cb * 5;
-  b[0] = lox; b[1] = loy; b[2] = hix; b[3] = hiy; b[4] = ml;
-}
-
-// ---------------------------------------------------------------------------
-// Near bits of every (row, element) pair (src/Core/Assembly_full.jl:736-754)
-// and the per-row near-pair counts: the classification of far_kernel without
-// the far entries, so that the near-pair list and the near/singular integrals
-// do not wait for the far field.
-// Block = 256 rows x a range of 32-element column blocks. A column block whose
-// node bounding box lies farther than (1 + 1e-6) x its largest cutoff from the
-// rows' bounding box holds only far pairs (every node distance and every
-// element bounding-box distance is at least the box distance, so the exact
-// test answers far); its bit words are written as zero without the pair tests.
-// Other column blocks take the pair tests of far_kernel (same arithmetic, same
-// decisions). row_cnt must be zero on entry; tiles[row block][column block]
-// records which column blocks took the pair tests (the others hold no near bit).
-// ---------------------------------------------------------------------------
-template <int NN>
-__global__ __launch_bounds__(kFarThreads) void near_classify(Geo g, Consts cs, int ncb,
-                                                             int cb_per_block,
-                                                             const double* __restrict__ box,
-                                                             uint32_t* __restrict__ nearbits,
-                                                             int* __restrict__ row_cnt,
-                                                             uint8_t* __restrict__ tiles) {
-  __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_L[kEB];
-  __shared__ float s_lim2f[kEB];
-  __shared__ double s_red[4][kFarThreads / 64];
-  const int i = blockIdx.x * kFarThreads + threadIdx.x;
-  const bool ok = i < g.nt;
-  const double px = ok ? g.rx[i] : 0.0, py = ok ? g.ry[i] : 0.0;
-  // bounding box of this block's rows
-  double rlox = ok ? px : INFINITY, rhix = ok ? px : -INFINITY;
-  double rloy = ok ? py : INFINITY, rhiy = ok ? py : -INFINITY;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    rlox = fmin(rlox, __shfl_xor(rlox, off, 64));
-    rhix = fmax(rhix, __shfl_xor(rhix, off, 64));
-    rloy = fmin(rloy, __shfl_xor(rloy, off, 64));
-    rhiy = fmax(rhiy, __shfl_xor(rhiy, off, 64));
-  }
-  const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    s_red[0][wv] = rlox; s_red[1][wv] = rhix; s_red[2][wv] = rloy; s_red[3][wv] = rhiy;
-  }
-  __syncthreads();
-  rlox = s_red[0][0]; rhix = s_red[1][0]; rloy = s_red[2][0]; rhiy = s_red[3][0];
-#pragma unroll
-  for (int q = 1; q < kFarThreads / 64; ++q) {
-    rlox = fmin(rlox, s_red[0][q]); rhix = fmax(rhix, s_red[1][q]);
-    rloy = fmin(rloy, s_red[2][q]); rhiy = fmax(rhiy, s_red[3][q]);
-  }
-  const int cb0 = blockIdx.y * cb_per_block, cb1 = min(ncb, cb0 + cb_per_block);
-  int cnt = 0;
-  for (int cb = cb0; cb < cb1; ++cb) {
-    const double* bx = box + (size_t)cb * 5;
-    const double dx = fmax(0.0, fmax(bx[0] - rhix, rlox - bx[2]));
-    const double dy = fmax(0.0, fmax(bx[1] - rhiy, rloy - bx[3]));
-    const double ml = bx[4];
-    const bool all_far = !(cs.nf > 1e20) && (dx * dx + dy * dy) > (ml * ml) * (1.0 + 1e-6);
-    if (threadIdx.x == 0) tiles[(size_t)blockIdx.x * ncb + cb] = all_far ? 0 : 1;
-    uint32_t bits = 0u;
-    if (!all_far) {  // uniform over the block
-      const int e0 = cb * kEB;
-      const int ecount = min(kEB, g.ne - e0);
-      __syncthreads();
-      stage_class_block<NN>(g, cs, e0, ecount, s_cx, s_cy, s_L, s_lim2f);
-      __syncthreads();
-      if (ok) {
-        for (int el = 0; el < ecount; ++el) {
-          double rx[NN], ry[NN];
-          float rxf[NN], ryf[NN], d2f[NN];
-          const int col0 = (e0 + el) * NN;
-          const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
-          if (classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el], s_lim2f[el],
-                                cs.nf, onel, rx, ry, rxf, ryf, d2f))
-            bits |= 1u << el;
-        }
-      }
-    }
-    if (ok) nearbits[(size_t)cb * g.nt + i] = bits;
-    cnt += __popc(bits);
-  }
-  if (ok && cnt) atomicAdd(&row_cnt[i], cnt);
-}
 
 // ---------------------------------------------------------------------------
 // Far-field tile kernel.
@@ -430,22 +205,6 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
 //                       written to diag[] (and to H's diagonal when given);
 //   mode 1 (apply):     y_i = far + near (self column: G_ii wG_i + diag_i wH_i)
 //                       + diag_i wH_i for internal rows.
-// Row sums of column-block partials part[cb * nt + i] for kFinRows rows per
-// 256-thread block: wave w adds blocks w, w + 4, w + 8, ... in order, then the
-// four wave sums are added in order (a fixed order for every launch).
-constexpr int kFinRows = 64;
-__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int ncb, int nt,
-                                               int i, double (*s_red)[kFinRows]) {
-  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  double s = 0.0;
-  if (i < nt) {
-#pragma unroll 8
-    for (int cb = wv; cb < ncb; cb += 4) s += part[(size_t)cb * nt + i];
-  }
-  s_red[wv][lane] = s;
-  __syncthreads();
-  return ((s_red[0][lane] + s_red[1][lane]) + s_red[2][lane]) + s_red[3][lane];
-}
 
 template <typename TS>
 __global__ __launch_bounds__(256) void op64_finalize(
@@ -492,88 +251,6 @@ __global__ __launch_bounds__(256) void op64_finalize(
   }
 }
 
-// ---------------------------------------------------------------------------
-// Near-pair CSR from the bit matrix [word][row].
-// ---------------------------------------------------------------------------
-// Exclusive scan of cnt[0..nt) into ptr[0..nt] with one workgroup of 16
-// wavefronts: each wave owns a contiguous segment, reads it 64 rows at a time
-// (coalesced) for its total, then scans it with lane shuffles from the sum of
-// the segments before it. The total also goes to host_total (pinned host
-// memory, may be null), so the host reads it without a copy on the stream.
-__global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict__ cnt,
-                                                  int* __restrict__ ptr,
-                                                  int* __restrict__ host_total) {
-  __shared__ int s_tot[16];
-  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  const int seg = (nt + 15) / 16;
-  const int lo = min(nt, wv * seg), hi = min(nt, lo + seg);
-  int tot = 0;
-  for (int r = lo + lane; r < hi; r += kWave) tot += cnt[r];
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, kWave);
-  if (lane == 0) s_tot[wv] = tot;
-  __syncthreads();
-  int run = 0;
-  for (int w = 0; w < wv; ++w) run += s_tot[w];
-  for (int r0 = lo; r0 < hi; r0 += kWave) {
-    const int r = r0 + lane;
-    const int v = (r < hi) ? cnt[r] : 0;
-    int x = v;  // inclusive scan over the wave
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const int y = __shfl_up(x, off, kWave);
-      if (lane >= off) x += y;
-    }
-    if (r < hi) ptr[r] = run + x - v;
-    run += __shfl(x, kWave - 1, kWave);
-  }
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int w = 0; w < 16; ++w) t += s_tot[w];
-    ptr[nt] = t;
-    if (host_total) __hip_atomic_store(host_total, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-}
-
-// One thread per row, in the row blocks of near_classify: only the column
-// blocks that took the pair tests can hold near bits. The pairs are also
-// listed by kind: sing[i] = CSR index of the on-element pair of row i (the
-// rows whose global index row0 + i is a boundary node, one such pair each,
-// i < nsing) and reg[] = CSR indices of the other pairs in order (null: not
-// written).
-__global__ __launch_bounds__(kFarThreads) void near_fill(int nt, int nwords, int nn, int row0,
-                                                         int n, const uint32_t* __restrict__ bits,
-                                                         const uint8_t* __restrict__ tiles,
-                                                         const int* __restrict__ ptr,
-                                                         int* __restrict__ prow,
-                                                         int* __restrict__ pelem,
-                                                         int* __restrict__ sing,
-                                                         int* __restrict__ reg) {
-  const int i = blockIdx.x * kFarThreads + threadIdx.x;
-  const uint8_t* tl = tiles + (size_t)blockIdx.x * nwords;
-  const int nsing = max(0, min(nt, n - row0));
-  const int own = (i < nsing) ? (i + row0) / nn : -1;  // element holding row i's node
-  int off = (i < nt) ? ptr[i] : 0;
-  int before = min(i, nsing);  // on-element pairs ahead of this row's next pair
-  for (int w = 0; w < nwords; ++w) {
-    if (!tl[w]) continue;  // uniform over the block
-    uint32_t b = (i < nt) ? bits[(size_t)w * nt + i] : 0u;
-    while (b) {
-      const int t = __ffs(b) - 1;
-      b &= b - 1u;
-      const int e = w * 32 + t;
-      prow[off] = i;
-      pelem[off] = e;
-      if (e == own) {
-        sing[i] = off;
-        ++before;
-      } else if (reg) {
-        reg[off - before] = off;
-      }
-      ++off;
-    }
-  }
-}
 
 // Near classification from candidate lists (FMM assembly): the targets of
 // leaf lf (tgt_order[tgt_lo[lf] .. tgt_hi[lf])) against the candidate
@@ -626,309 +303,6 @@ __global__ __launch_bounds__(64) void cand_near_kernel(
   }
 }
 
-// ---------------------------------------------------------------------------
-// Near-singular and singular element integrals: device math.
-// ---------------------------------------------------------------------------
-struct Cx {
-  double re, im;
-};
-__device__ __forceinline__ Cx cadd(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ Cx csub(Cx a, Cx b) { return {a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ Cx cmul(Cx a, Cx b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-__device__ __forceinline__ Cx cscale(Cx a, double s) { return {a.re * s, a.im * s}; }
-__device__ __forceinline__ Cx cdiv(Cx a, Cx b) {  // Smith's algorithm
-  if (fabs(b.re) >= fabs(b.im)) {
-    double r = b.im / b.re, d = b.re + b.im * r;
-    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
-  }
-  double r = b.re / b.im, d = b.re * r + b.im;
-  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
-}
-__device__ __forceinline__ double cabs(Cx a) { return hypot(a.re, a.im); }
-__device__ __forceinline__ Cx csqrt_(Cx z) {  // principal branch
-  if (z.re == 0.0 && z.im == 0.0) return {0.0, z.im};
-  double r = hypot(z.re, z.im);
-  if (z.re >= 0.0) {
-    double t = sqrt(0.5 * (r + z.re));
-    return {t, z.im / (2.0 * t)};
-  }
-  double t = sqrt(0.5 * (r - z.re));
-  return {fabs(z.im) / (2.0 * t), copysign(t, z.im)};
-}
-
-__device__ __forceinline__ double clampd(double x, double lo, double hi) {
-  return x > hi ? hi : (x < lo ? lo : x);
-}
-
-// Barycentric interpolation row (src/Core/Interpolation.jl:148-178), NN <= 4.
-template <int NN>
-__device__ __forceinline__ void interp_row(const double* x0, const double* bw, double xx,
-                                           double* M) {
-  double msum = 0.0;
-  int exact = -1;
-#pragma unroll
-  for (int i = 0; i < NN; ++i) {
-    if (xx == x0[i]) exact = i;
-    M[i] = bw[i] / (xx - x0[i]);
-    msum += M[i];
-  }
-  if (msum == 0.0) {
-#pragma unroll
-    for (int i = 0; i < NN; ++i) M[i] = 0.0;
-  } else if (exact >= 0) {
-#pragma unroll
-    for (int i = 0; i < NN; ++i) M[i] = (i == exact) ? 1.0 : 0.0;
-  } else {
-#pragma unroll
-    for (int i = 0; i < NN; ++i) M[i] = M[i] / msum;
-  }
-}
-
-// shapefun: N and dN = N * D (src/Core/Interpolation.jl:215-218).
-template <int NN>
-__device__ __forceinline__ void shapefun(const double* x0, const double* bw, const double* D,
-                                         double xx, double* N, double* dN) {
-  interp_row<NN>(x0, bw, xx, N);
-#pragma unroll
-  for (int j = 0; j < NN; ++j) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < NN; ++i) s += N[i] * D[i * NN + j];
-    dN[j] = s;
-  }
-}
-
-// Complex pole zeta + i eta of z(xi) through the collocation nodes
-// (src/Core/Integration.jl:1117-1160) for p = 1, 2.
-template <int NN>
-__device__ void complex_pole(const double* lx, const double* cx, const double* cy, double px,
-                             double py, double& zeta, double& eta) {
-  const Cx y = {px, py};
-  if (NN == 2) {
-    const double d = lx[1] - lx[0] + 1e-30;
-    const Cx dz = {cx[1] - cx[0], cy[1] - cy[0]};
-    const Cx B = {dz.re / d, dz.im / d};
-    const Cx C = csub(Cx{cx[0], cy[0]}, cscale(B, lx[0]));
-    if (cabs(B) < 1e-30) { zeta = 0.0; eta = 1.0; return; }
-    const Cx r = cdiv(csub(y, C), B);
-    zeta = r.re;
-    eta = fabs(r.im);
-    return;
-  }
-  const Cx Z0 = {cx[0], cy[0]}, Z1 = {cx[1], cy[1]}, Z2 = {cx[2], cy[2]};
-  const double a11 = lx[0] * lx[0], a12 = lx[0], a21 = lx[1] * lx[1], a22 = lx[1],
-               a31 = lx[2] * lx[2], a32 = lx[2];
-  const double det = a11 * (a22 - a32) - a12 * (a21 - a31) + (a21 * a32 - a22 * a31);
-  // Cramer's rule for [xi^2 xi 1] c = Z (_fit_quad_complex, :1153-1160)
-  auto cdivr = [](Cx z, double d) { return Cx{z.re / d, z.im / d}; };
-  const Cx A = cdivr(cadd(csub(cscale(Z0, a22 - a32), cscale(csub(Z1, Z2), a12)),
-                          csub(cscale(Z1, a32), cscale(Z2, a22))), det);
-  const Cx B = cdivr(cadd(csub(cscale(csub(Z1, Z2), a11), cscale(Z0, a21 - a31)),
-                          csub(cscale(Z2, a21), cscale(Z1, a31))), det);
-  const Cx C = cdivr(cadd(csub(cscale(csub(cscale(Z2, a22), cscale(Z1, a32)), a11),
-                               cscale(csub(cscale(Z2, a21), cscale(Z1, a31)), a12)),
-                          cscale(Z0, a21 * a32 - a22 * a31)), det);
-  if (cabs(A) < 1e-14) {
-    if (cabs(B) < 1e-30) { zeta = 0.0; eta = 1.0; return; }
-    const Cx r = cdiv(csub(y, C), B);
-    zeta = r.re;
-    eta = fabs(r.im);
-    return;
-  }
-  const Cx disc = csqrt_(csub(cmul(B, B), cmul(cscale(A, 4.0), csub(C, y))));
-  const Cx twoA = cscale(A, 2.0);
-  const Cx mB = {-B.re, -B.im};
-  const Cx r1 = cdiv(cadd(mB, disc), twoA);
-  const Cx r2 = cdiv(csub(mB, disc), twoA);
-  const Cx r = (fabs(r1.im) <= fabs(r2.im)) ? r1 : r2;
-  zeta = r.re;
-  eta = fabs(r.im);
-}
-
-// p3c inverse pole (src/Core/Integration.jl:1220-1266).
-__device__ void p3c_inv_pole(double zeta, double eta, double& zt_out, double& et_out) {
-  if (fabs(eta) < 1e-14) {
-    double x = zeta;
-    for (int it = 0; it < 25; ++it) {
-      const double g = (((x * x * x) - (3.0 * zeta) * (x * x)) + 3.0 * x) - zeta;
-      const double gp = ((3.0 * (x * x)) - (6.0 * zeta) * x) + 3.0;
-      if (fabs(gp) < 1e-30) break;
-      x -= g / gp;
-      if (fabs(g) < 1e-14) break;
-    }
-    zt_out = x;
-    et_out = 0.0;
-    return;
-  }
-  const double sgn = eta > 0.0 ? 1.0 : (eta < 0.0 ? -1.0 : 0.0);
-  double et = cbrt(fabs(eta)) * sgn;
-  for (int it = 0; it < 20; ++it) {
-    const double g = ((2.0 * (et * et * et)) - (3.0 * eta) * (et * et)) - eta;
-    const double gp = (6.0 * (et * et)) - (6.0 * eta) * et;
-    if (fabs(gp) < 1e-30) break;
-    et -= g / gp;
-  }
-  et += cbrt(eta) * sqrt(zeta * zeta + eta * eta);
-  double zt = zeta;
-  for (int it = 0; it < 30; ++it) {
-    const double den = (1.0 + 3.0 * (zt * zt)) + 3.0 * (et * et);
-    const double nz = ((3.0 + zt * zt) + 3.0 * (et * et)) * zt;
-    const double ne = 2.0 * (et * et * et);
-    const double f1 = nz / den - zeta;
-    const double f2 = ne / den - eta;
-    if (fabs(f1) + fabs(f2) < 1e-14) break;
-    const double denz = 6.0 * zt, dene = 6.0 * et;
-    const double nz_z = (3.0 + 3.0 * (zt * zt)) + 3.0 * (et * et);
-    const double nz_e = (6.0 * et) * zt;
-    const double ne_e = 6.0 * (et * et);
-    const double den2 = den * den;
-    const double j11 = (nz_z * den - nz * denz) / den2;
-    const double j12 = (nz_e * den - nz * dene) / den2;
-    const double j21 = (0.0 * den - ne * denz) / den2;
-    const double j22 = (ne_e * den - ne * dene) / den2;
-    const double detJ = j11 * j22 - j12 * j21;
-    if (fabs(detJ) < 1e-30) break;
-    zt -= (j22 * f1 - j12 * f2) / detJ;
-    et -= (-j21 * f1 + j11 * f2) / detJ;
-  }
-  zt_out = zt;
-  et_out = et;
-}
-
-struct P3C {
-  double zt, et, A, B;
-};
-__device__ __forceinline__ P3C p3c_make(double zeta, double eta) {
-  P3C c;
-  p3c_inv_pole(zeta, eta, c.zt, c.et);
-  const double den = (1.0 + 3.0 * (c.zt * c.zt)) + 3.0 * (c.et * c.et);
-  c.A = 3.0 / den;
-  c.B = ((3.0 + c.zt * c.zt) * c.zt) / den;
-  return c;
-}
-__device__ __forceinline__ void p3c_node(const P3C& c, double s, double w, double& x,
-                                         double& wo) {
-  const double ds = s - c.zt;
-  x = c.A * ((ds * ds * ds) / 3.0 + (c.et * c.et) * s) + c.B;
-  wo = w * (c.A * (ds * ds + c.et * c.et));
-}
-
-__device__ __forceinline__ bool eq41_collinear(double zeta, double eta, bool has_den,
-                                               double den) {
-  if (fabs(zeta) <= 1.0) return false;
-  if (fabs(eta) < 1e-8) return true;
-  if (has_den && fabs(den) < 1e-10) return true;
-  return false;
-}
-
-__device__ __forceinline__ void mobius_node(double zeta, double u, double w, double& x,
-                                            double& wo) {
-  if (fabs(zeta) <= 1.0) { x = u; wo = w; return; }
-  const double A = zeta * zeta - 1.0, B = -zeta;
-  const double d = u - B;
-  x = zeta - A / d;
-  wo = w * (A / (d * d));
-}
-
-__device__ __forceinline__ void tan_back(double zeta, double eta, double At, double Bt,
-                                         double ut, double wt, double& x, double& wo) {
-  const double halfpi = kPi / 2.0 - 1e-8;
-  const double th = clampd((ut - Bt) / At, -halfpi, halfpi);
-  const double tn = tan(th);
-  x = zeta + eta * tn;
-  wo = wt * ((eta / At) * (1.0 + tn * tn));
-}
-
-// _tangenttrans for one node (src/Core/Integration.jl:1177-1200).
-__device__ void tangent_node(double zeta, double eta, double u, double w, double& x,
-                             double& wo) {
-  if (eq41_collinear(zeta, eta, false, 0.0)) { mobius_node(zeta, u, w, x, wo); return; }
-  eta = fmax(eta, 1e-14);
-  const double tp = atan((1.0 - zeta) / eta);
-  const double tm = atan((-1.0 - zeta) / eta);
-  const double den = tp - tm;
-  if (eq41_collinear(zeta, eta, true, den)) { mobius_node(zeta, u, w, x, wo); return; }
-  const double A = (2.0 * eta) / den;
-  const double B = -(tp + tm) / den;
-  const double halfpi = kPi / 2.0 - 1e-8;
-  const double phi = clampd((eta * (u - B)) / A, -halfpi, halfpi);
-  const double tn = tan(phi);
-  x = zeta + eta * tn;
-  wo = w * (((eta * eta) / A) * (1.0 + tn * tn));
-}
-
-// Plan of the tan-p3c rule (src/Core/Integration.jl:1291-1365), uniform per pair.
-struct TanPlan {
-  int kind;       // 0 Mobius, 1 p3c only, 2 compose, 3 split
-  double zeta, eta, At, Bt;
-  P3C c1, c2;     // compose: c1; split: c1 left, c2 right
-  double mid1, h1, mid2, h2;
-  bool tan1, tan2;  // split half degenerated to the tangent map
-};
-
-__device__ void tanp3c_plan(double zeta, double eta, TanPlan& P) {
-  P.zeta = zeta;
-  P.eta = eta;
-  P.tan1 = P.tan2 = false;
-  if (eq41_collinear(zeta, eta, false, 0.0)) { P.kind = 0; return; }
-  eta = fmax(eta, 1e-14);
-  P.eta = eta;
-  const double ap = atan((1.0 - zeta) / eta);
-  const double am = atan((1.0 + zeta) / eta);
-  const double den = ap + am;
-  if (eq41_collinear(zeta, eta, true, den)) { P.kind = 0; return; }
-  if (fabs(den) < 1e-30) { P.kind = 1; P.c1 = p3c_make(zeta, eta); return; }
-  const double At = 2.0 / den;
-  const double Bt = ((-At) * (ap - am)) / 2.0;
-  const double uL = Bt - (At * kPi) / 2.0;
-  const double uR = Bt + (At * kPi) / 2.0;
-  P.At = At;
-  P.Bt = Bt;
-  if (fabs(zeta) < 1.0 && -1.0 < Bt && Bt < 1.0) {
-    P.kind = 3;
-    P.mid1 = (Bt + -1.0) / 2.0; P.h1 = (Bt - -1.0) / 2.0;
-    P.mid2 = (1.0 + Bt) / 2.0;  P.h2 = (1.0 - Bt) / 2.0;
-    P.tan1 = P.h1 <= 1e-14;
-    P.tan2 = P.h2 <= 1e-14;
-    if (!P.tan1) P.c1 = p3c_make((uL - P.mid1) / P.h1, 1e-14);
-    if (!P.tan2) P.c2 = p3c_make((uR - P.mid2) / P.h2, 1e-14);
-    return;
-  }
-  P.kind = 2;
-  P.c1 = p3c_make(zeta >= 1.0 ? uL : uR, 1e-14);
-}
-
-__device__ __forceinline__ int tanp3c_count(const TanPlan& P, int npg) {
-  return P.kind == 3 ? 2 * npg : npg;
-}
-
-// Node q of the tan-p3c rule built from the Gauss rule (u, w).
-__device__ void tanp3c_node(const TanPlan& P, int q, int npg, const double* u, const double* w,
-                            double& x, double& wo) {
-  switch (P.kind) {
-    case 0: mobius_node(P.zeta, u[q], w[q], x, wo); return;
-    case 1: p3c_node(P.c1, u[q], w[q], x, wo); return;
-    case 2: {
-      double ut, wt;
-      p3c_node(P.c1, u[q], w[q], ut, wt);
-      tan_back(P.zeta, P.eta, P.At, P.Bt, ut, wt, x, wo);
-      return;
-    }
-    default: {
-      const bool left = q < npg;
-      const int qq = left ? q : q - npg;
-      if (left ? P.tan1 : P.tan2) { tangent_node(P.zeta, P.eta, u[qq], w[qq], x, wo); return; }
-      const P3C& c = left ? P.c1 : P.c2;
-      const double mid = left ? P.mid1 : P.mid2, h = left ? P.h1 : P.h2;
-      double s, ws;
-      p3c_node(c, u[qq], w[qq], s, ws);
-      tan_back(P.zeta, P.eta, P.At, P.Bt, mid + h * s, ws * h, x, wo);
-      return;
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Near-pair integrals, one group of lanes per pair. near_fill lists the two
@@ -941,13 +315,6 @@ __device__ void tanp3c_node(const TanPlan& P, int q, int npg, const double* u, c
 // Lanes run over quadrature points; the values go to the pair list at the
 // pair's CSR index. list == nullptr: the k-th pair is CSR index k.
 // ---------------------------------------------------------------------------
-// Sum over the G lanes of a group (xor offsets < G stay in the group).
-template <int G>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
 
 template <int NN, int G>
 __global__ __launch_bounds__(256) void near_reg_kernel(Geo g, const QuadConst* __restrict__ qc,
@@ -1293,7 +660,7 @@ __global__ void fill2_kernel(int n, double a, double* __restrict__ x, double b,
 
 // Pinned host word for the near-pair count, written by near_scan through its
 // device address.
-static hipError_t ensure_count_word(bemhip_ctx* c) {
+hipError_t ensure_count_word(bemhip_ctx* c) {
   if (c->h_count && c->d_count) return hipSuccess;
   if (!c->h_count) {
     hipError_t e = hipHostMalloc((void**)&c->h_count, sizeof(int));
@@ -1594,6 +961,7 @@ static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, doub
 
 // y = A64 x for the mixed-BC system (Dirichlet columns of A are -G).
 int operator_apply(bemhip_ctx* c, const double* d_x, double* d_y) {
+  if (c->dof == 2) return kelvin_operator_apply(c, d_x, d_y);
   const size_t nt = (size_t)c->nt;
   BEMHIP_HIP_TRY(c, c->wH2.ensure(nt));
   BEMHIP_HIP_TRY(c, c->wG2.ensure(nt));
@@ -1607,6 +975,7 @@ int operator_apply(bemhip_ctx* c, const double* d_x, double* d_y) {
 // applyBC on the device: b from an FP64 operator pass (identical in both
 // precisions), A from the column swap of the stored H, G.
 int apply_bc_impl(bemhip_ctx* c) {
+  if (c->dof == 2) return kelvin_apply_bc_impl(c);
   const size_t nt = (size_t)c->nt;
   BEMHIP_HIP_TRY(c, c->wH2.ensure(nt));
   BEMHIP_HIP_TRY(c, c->wG2.ensure(nt));

@@ -108,6 +108,13 @@ struct bemhip_ctx {
   int p = 0, nn = 0, ne = 0, n = 0, ni = 0, nt = 0;
   size_t ld32 = 0, ld64 = 0;  // leading dimensions of the FP32 / FP64 matrices
   double k = 1.0;
+  // problem: dof = 1 Laplace (conductivity k); dof = 2 2-D Kelvin elasticity
+  // (shear modulus mu, effective Poisson ratio nu). Node j's component b is
+  // unknown dof * j + b; H and A have nsys() rows and columns, G nbnd() columns.
+  int dof = 1;
+  double mu = 0.0, nu = 0.0;
+  int nsys() const { return dof * nt; }
+  int nbnd() const { return dof * n; }
   std::vector<double> h_elem_w;
   bemhip::DevBuf<double> x, y, nx, ny, jw, elen, gx, gy;
 
@@ -229,6 +236,11 @@ int lu_solve_trans_device(bemhip_ctx* c, int n, int dtype, const void* dLU, int 
 int build_perm(bemhip_ctx* c, int n, const int* dipiv, int* dperm);
 int operator_apply_trans(bemhip_ctx* c, const double* d_v, double* d_y);
 int fmm_build(bemhip_ctx* c);
+Geo make_geo(bemhip_ctx* c);
+hipError_t ensure_count_word(bemhip_ctx* c);  // pinned host word for the near-pair count
+int kelvin_assemble_impl(bemhip_ctx* c);
+int kelvin_apply_bc_impl(bemhip_ctx* c);
+int kelvin_operator_apply(bemhip_ctx* c, const double* d_x, double* d_y);
 int fmm_far(bemhip_ctx* c, hipStream_t s, const double* wH, const double* wG, double* out);
 int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
                   const double* d_w, const QuadConst* d_qc, int npg, double near_factor,

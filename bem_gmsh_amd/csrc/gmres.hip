@@ -157,6 +157,8 @@ extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int r
                                   double* x, int* iters, double* relres) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "solve_gmres: NULL context");
   if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "solve_gmres: apply_bc first");
+  if (c->dof != 1)
+    return fail(c, BEMHIP_ERR_STATE, "solve_gmres: Laplace problems only (use bemhip_solve)");
   if (!(atol >= 0.0) || !(rtol >= 0.0) || (atol == 0.0 && rtol == 0.0))
     return fail(c, BEMHIP_ERR_ARG, "solve_gmres: need atol, rtol >= 0, not both zero");
   if (restart < 1 || restart > 512 || itmax < 1)

@@ -33,23 +33,25 @@ __global__ void axpy_kernel(int n, const double* __restrict__ d, double* __restr
 }
 
 // A32 = FP32 of the column swap of H64, G64 (used once A64 holds FP64 factors).
+// Workgroup row y takes columns y, y + gridDim.y, ...
 __global__ void swap_round_kernel(int nt, int n, size_t lds, size_t ldd, const int* __restrict__ bc,
                                   const double* __restrict__ H, const double* __restrict__ G,
                                   float* __restrict__ A) {
-  const int j = blockIdx.y;
-  const bool dir = (j < n) && bc[j] == 0;
-  const double* src = dir ? G + (size_t)j * lds : H + (size_t)j * lds;
-  const double sg = dir ? -1.0 : 1.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
-    A[(size_t)j * ldd + i] = (float)(sg * src[i]);
+  for (int j = blockIdx.y; j < nt; j += gridDim.y) {
+    const bool dir = (j < n) && bc[j] == 0;
+    const double* src = dir ? G + (size_t)j * lds : H + (size_t)j * lds;
+    const double sg = dir ? -1.0 : 1.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
+      A[(size_t)j * ldd + i] = (float)(sg * src[i]);
+  }
 }
 
 // A32 = FP32 of A64 (nt x nt; leading dimensions lds, ldd).
 __global__ void round_cols_kernel(int nt, size_t lds, size_t ldd, const double* __restrict__ s,
                                   float* __restrict__ d) {
-  const int j = blockIdx.y;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
-    d[(size_t)j * ldd + i] = (float)s[(size_t)j * lds + i];
+  for (int j = blockIdx.y; j < nt; j += gridDim.y)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nt; i += gridDim.x * blockDim.x)
+      d[(size_t)j * ldd + i] = (float)s[(size_t)j * lds + i];
 }
 
 // max |v| over n entries into out[slot] (one workgroup).
@@ -99,7 +101,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
   if (mode == BEMHIP_SOLVE_MIXED && (!(tol > 0.0) || maxit < 0))
     return fail(c, BEMHIP_ERR_ARG, "solve: tol must be > 0 and maxit >= 0");
   hipSetDevice(c->device);
-  const int nt = c->nt;
+  const int nt = c->nsys();  // rows of A: dof * collocation points
   BEMHIP_HIP_TRY(c, c->ipiv.ensure(nt));
   BEMHIP_HIP_TRY(c, c->perm.ensure(nt));
   BEMHIP_HIP_TRY(c, c->ipiv32.ensure(nt));
@@ -144,9 +146,9 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
       if (c->precision == BEMHIP_FP64) {
         // FP32 copy of the FP64-assembled system (from H, G once A64 is factored)
         BEMHIP_HIP_TRY(c, c->A32.ensure(c->ld32 * nt));
-        const dim3 grid((unsigned)std::min((nt + 255) / 256, 64), (unsigned)nt);
+        const dim3 grid((unsigned)std::min((nt + 255) / 256, 64), (unsigned)std::min(nt, 32768));
         if (c->factored64)
-          hipLaunchKernelGGL(swap_round_kernel, grid, dim3(256), 0, c->stream, nt, c->n, c->ld64,
+          hipLaunchKernelGGL(swap_round_kernel, grid, dim3(256), 0, c->stream, nt, c->nbnd(), c->ld64,
                              c->ld32, c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A32.ptr);
         else
           hipLaunchKernelGGL(round_cols_kernel, grid, dim3(256), 0, c->stream, nt, c->ld64,

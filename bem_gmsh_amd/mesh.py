@@ -30,7 +30,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from ._lib import gauss_legendre
-from .problem import BEMData, Laplace
+from .problem import BEMData, Elasticity, Laplace, n_dof
 
 # ---------------------------------------------------------------------------
 # Mesh description
@@ -66,19 +66,30 @@ def _to_gmsh_order(X: np.ndarray) -> np.ndarray:
 
 
 def parse_pairs(s: str) -> Tuple[int, float]:
-    """BC type and value from a physical name (src/Core/Input.jl:564-582)."""
+    """First BC type and value of a physical name (src/Core/Input.jl:564-582)."""
+    t, v = parse_bc_pairs(s)
+    return t[0], v[0]
+
+
+def parse_bc_pairs(s: str) -> Tuple[List[int], List[float]]:
+    """Every (type, value) pair of a physical name, ``"t;v"`` (scalar) or
+    ``"tx;vx;ty;vy"`` (elasticity), as parse_pairs (src/Core/Input.jl:564-582):
+    bare labels give ``[1], [0.0]``; a malformed pair raises ValueError."""
     parts = [t.strip() for t in s.split(";") if t.strip() != ""]
     try:
-        t0 = int(parts[0])
+        int(parts[0])
     except (ValueError, IndexError):
-        return 1, 0.0
+        return [1], [0.0]
     if len(parts) < 2:
-        return 1, 0.0
-    try:
-        v0 = float(parts[1])
-    except ValueError:
-        raise ValueError(f"Bad BC pair in '{s}'")
-    return t0, v0
+        return [1], [0.0]
+    ts, vs = [], []
+    for k in range(len(parts) // 2):
+        try:
+            ts.append(int(parts[2 * k]))
+            vs.append(float(parts[2 * k + 1]))
+        except ValueError:
+            raise ValueError(f"Bad BC pair in '{s}'")
+    return ts, vs
 
 
 # ---------------------------------------------------------------------------
@@ -166,7 +177,7 @@ def _point_in_polygon(px: float, py: float, verts: np.ndarray) -> bool:
     return inside
 
 
-def format2d(mesh: Mesh2D, properties: Laplace, tipo: Optional[int] = None,
+def format2d(mesh: Mesh2D, properties, tipo: Optional[int] = None,
              pontointerno: bool = True) -> BEMData:
     """Build the collocation container from a boundary mesh.
 
@@ -174,11 +185,13 @@ def format2d(mesh: Mesh2D, properties: Laplace, tipo: Optional[int] = None,
     Lagrange elements with (p+1) Gauss-Legendre collocation nodes, geometry from
     the equispaced Lagrange interpolant of the element's geometric nodes,
     ``J = |dx/dxi|``, ``n = (dx_2, -dx_1)/J`` (tan2normal, :584),
-    ``L = |sum_k J_k w_k|`` and BCs from the physical names; internal nodes at
-    the domain-cell centroids when ``pontointerno``.
+    ``L = |sum_k J_k w_k|`` and BCs from the physical names (``"t;v"`` for
+    Laplace, ``"tx;vx;ty;vy"`` per dof for Elasticity, ``_assign_bc!``
+    :537-559); internal nodes at the domain-cell centroids when ``pontointerno``.
     """
-    if not isinstance(properties, Laplace):
-        raise TypeError("format2d: only Laplace problems are on this path")
+    if not isinstance(properties, (Laplace, Elasticity)):
+        raise TypeError("format2d: properties must be Laplace or Elasticity")
+    dof = n_dof(properties)
     p = int(mesh.order)
     if tipo is not None and int(tipo) != p:
         raise ValueError(f"format2d: tipo={tipo} differs from the mesh order {p}; "
@@ -207,12 +220,16 @@ def format2d(mesh: Mesh2D, properties: Laplace, tipo: Optional[int] = None,
     for k in range(nn):
         Lsum = Lsum + J[:, k] * wi[k]
     L = np.abs(Lsum)
-    BC = np.ones(ne * nn, dtype=np.int64)
-    BV = np.zeros(ne * nn, dtype=np.float64)
+    BC = np.ones(dof * ne * nn, dtype=np.int64)
+    BV = np.zeros(dof * ne * nn, dtype=np.float64)
     for e in range(ne):
-        t, v = parse_pairs(mesh.physical[e])
-        BC[e * nn:(e + 1) * nn] = t
-        BV[e * nn:(e + 1) * nn] = v
+        ts, vs = parse_bc_pairs(mesh.physical[e])
+        tc, vc = [1] * dof, [0.0] * dof
+        m = min(len(ts), dof)
+        tc[:m], vc[:m] = ts[:m], vs[:m]
+        for d in range(dof):
+            BC[dof * e * nn + d:dof * (e + 1) * nn:dof] = tc[d]
+            BV[dof * e * nn + d:dof * (e + 1) * nn:dof] = vc[d]
     cells = [np.asarray(c, dtype=np.float64) for c in mesh.cells]
     cells = [c for c in cells if abs(_polygon_area(c)) >= 1e-18]
     # _orient_normals_outward! (Input.jl:376-389)
@@ -265,13 +282,26 @@ def quadrado(ndiv: int = 10, ordem: int = 1, Lx: float = 1.0, Ly: float = 1.0,
     (ndiv-1)^2 quad grid as domain cells (``cells=False``: boundary only, for
     large ndiv).
     """
+    return _square(ndiv, ordem, Lx, Ly, nome, cells, ("1;0", "1;-1", "1;0", "0;0"))
+
+
+def quadrado_elasticity(ndiv: int = 10, ordem: int = 1, Lx: float = 1.0, Ly: float = 1.0,
+                        nome: str = "quadrado_elast", cells: bool = True) -> Mesh2D:
+    """Square of data/Laplace/Laplace_dad.jl:289-320 for 2-D elasticity (no
+    Gmsh): the mesh of ``quadrado`` with 4-token BC names ``"tx;vx;ty;vy"``,
+    left edge clamped ``"0;0;0;0"``, the others traction-free ``"1;0;1;0"``."""
+    return _square(ndiv, ordem, Lx, Ly, nome, cells, ("1;0;1;0", "1;0;1;0", "1;0;1;0",
+                                                      "0;0;0;0"))
+
+
+def _square(ndiv, ordem, Lx, Ly, nome, cells, names) -> Mesh2D:
     if ndiv < 2:
         raise ValueError("ndiv must be >= 2")
     if ordem < 1:
         raise ValueError("ordem must be >= 1")
     m = ndiv - 1
     corners = np.array([[0.0, 0.0], [Lx, 0.0], [Lx, Ly], [0.0, Ly]])
-    sides = [(0, 1, "1;0", 1), (1, 2, "1;-1", 2), (2, 3, "1;0", 3), (3, 0, "0;0", 4)]
+    sides = [(0, 1, names[0], 1), (1, 2, names[1], 2), (2, 3, names[2], 3), (3, 0, names[3], 4)]
     t = np.linspace(0.0, 1.0, m + 1)
     lines, phys, reg = [], [], []
     for a, b, name, tag in sides:

@@ -1,7 +1,8 @@
 """Problem description and the collocation container.
 
-Mirrors the reference's data model for the 2-D scalar path:
-``Laplace`` (src/Core/Structures.jl:101-103), ``Element``
+Mirrors the reference's data model for the 2-D path:
+``Laplace`` (src/Core/Structures.jl:101-103), ``Elasticity``
+(src/Core/Structures.jl:149-263), ``Element``
 (src/Core/Structures.jl:317-324) and ``BEMdata`` (src/Core/Structures.jl:504-519)
 with its cache forwarding (:522-553). Collocation points are boundary nodes
 ``0:n`` followed by internal nodes ``n:nt``; element ``e`` owns the contiguous
@@ -26,6 +27,46 @@ class Laplace:
     k: float = 1.0
 
 
+class Elasticity:
+    """2-D isotropic linear elasticity (src/Core/Structures.jl:149-263).
+
+    ``Elasticity(E, nu, rho; plane_strain=True)`` or ``plane_stress=True``.
+    The Kelvin kernels use the effective Poisson ratio (plane stress maps to
+    the equivalent plane-strain ``nu / (1 + nu)``, :260-263); the shear
+    modulus is ``mu = E / (2 (1 + nu))`` with the material ``nu`` and the first
+    Lame constant uses the effective ratio (``lame_constants``, :156-165).
+    """
+
+    def __init__(self, E: float = 1.0, nu: float = 0.3, rho: float = 1.0,
+                 plane_strain: bool = True, plane_stress: bool = False):
+        self.E = float(E)
+        self.nu = float(nu)
+        self.rho = float(rho)
+        self.plane_strain = False if plane_stress else bool(plane_strain)
+
+    def effective_nu(self) -> float:
+        return self.nu if self.plane_strain else self.nu / (1.0 + self.nu)
+
+    @property
+    def mu(self) -> float:
+        return self.E / (2.0 * (1.0 + self.nu))
+
+    @property
+    def lambda_(self) -> float:
+        ne = self.effective_nu()
+        den = (1.0 + ne) * (1.0 - 2.0 * ne)
+        return float("inf") if abs(den) < np.finfo(float).eps else self.E * ne / den
+
+    def __repr__(self) -> str:
+        mode = "plane_strain" if self.plane_strain else "plane_stress"
+        return f"Elasticity(E={self.E}, nu={self.nu}, rho={self.rho}, {mode})"
+
+
+def n_dof(properties) -> int:
+    """Unknowns per node: 1 (Laplace) or 2 (2-D elasticity)."""
+    return 2 if isinstance(properties, Elasticity) else 1
+
+
 @dataclass
 class Element:
     """One boundary element (view onto the SoA arrays of a BEMData)."""
@@ -41,8 +82,9 @@ class BEMData:
     """Collocation container for a 2-D boundary-element problem.
 
     Attributes follow the reference names: ``collocation`` (nt, 2),
-    ``Normal`` (n, 2), ``BC`` (n,) with 0 = Dirichlet (unknown q) and
-    1 = Neumann (unknown T), ``BV`` (n,), ``n``, ``ni``, ``nt``, ``elem_weight``
+    ``Normal`` (n, 2), ``BC`` (dof*n,) with 0 = Dirichlet (unknown q, or the
+    traction component) and 1 = Neumann (unknown T, or the displacement
+    component), ``BV`` (dof*n,), ``n``, ``ni``, ``nt``, ``elem_weight``
     (p+1,) and the per-element arrays ``elem_J`` (ne, p+1), ``elem_L`` (ne,),
     ``elem_geo`` (ne, p+1, 2), ``elem_region`` (ne,).
 
@@ -51,12 +93,12 @@ class BEMData:
     raising an error when missing, like src/Core/Structures.jl:539-553.
     """
 
-    _CACHE_KEYS = ("H", "G", "A", "B", "b", "T", "q", "qsi", "w", "near_factor",
+    _CACHE_KEYS = ("H", "G", "A", "B", "b", "T", "q", "u", "traction", "qsi", "w", "near_factor",
                    "analytical", "cells", "solve_info", "assembly_info", "gmres_stats")
 
     def __init__(self, name: str, p: int, collocation: np.ndarray, normal: np.ndarray,
                  elem_J: np.ndarray, elem_L: np.ndarray, elem_geo: np.ndarray,
-                 elem_region: np.ndarray, elem_weight: np.ndarray, properties: Laplace,
+                 elem_region: np.ndarray, elem_weight: np.ndarray, properties,
                  BC: np.ndarray, BV: np.ndarray, n: int, ni: int):
         object.__setattr__(self, "cache", {})
         self.name = name
@@ -125,6 +167,10 @@ class BEMData:
     def has_cache(self, name: str) -> bool:
         return name in self.cache
 
+    @property
+    def dof(self) -> int:
+        return n_dof(self.properties)
+
     def __repr__(self) -> str:
         return (f"BEMData({self.name!r}, p={self.p}, ne={self.ne}, n={self.n}, "
-                f"ni={self.ni}, nt={self.nt}, k={self.properties.k})")
+                f"ni={self.ni}, nt={self.nt}, {self.properties!r})")

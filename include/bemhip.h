@@ -96,6 +96,24 @@ int bemhip_set_geometry(bemhip_ctx* ctx, int p, int ne, int n, int ni,
                         const double* elem_w, const double* geo_xy, const double* elen,
                         double k);
 
+/* Switch the context's problem to 2-D isotropic elasticity (the Kelvin kernel
+ * of fundamental(props::Elasticity, r, n), src/Elasticity/Fundamental.jl:28-57),
+ * as format2d(mesh, Elasticity(E, nu, rho)) selects it for the reference
+ * (src/Core/Structures.jl:149-263). Call after bemhip_set_geometry (which
+ * resets the problem to Laplace).
+ *   mu   shear modulus E / (2 (1 + nu)) (Elasticity.mu)
+ *   nu   effective Poisson ratio (effective_nu, src/Core/Structures.jl:260-263:
+ *        nu for plane strain, nu / (1 + nu) for plane stress), in (-1, 0.5)
+ * Every system size doubles: node j's component b is unknown 2j + b; H and A
+ * are 2nt x 2nt, G is 2nt x 2n, BC and BV have 2n entries (per dof), x has
+ * 2nt. bemhip_assemble then builds H_G_full_direct for a Vectorial problem
+ * (src/Core/Assembly_full.jl:120-175; BEMHIP_FP64 only), bemhip_apply_bc the
+ * per-dof applyBC (src/Core/Boundary_conditions.jl:153-175) and bemhip_solve
+ * the solve of src/Core/Solver.jl:115-135 (split_sol!, :137-146, is left to
+ * the caller). GMRES, the adjoint and interior evaluation are Laplace-only
+ * (BEMHIP_ERR_STATE). Invalidates any assembled matrices and factors. */
+int bemhip_set_elasticity(bemhip_ctx* ctx, double mu, double nu);
+
 /* Assemble H (nt x nt) and G (nt x n) on the device. Replaces
  * H_G_full_direct (src/Core/Assembly_full.jl:61-85) behind
  * assemble!(dad; method, npg) (src/Core/Assembly_full.jl:107-118): far

@@ -793,6 +793,161 @@ void integrate_element(const Mesh& M, const Rule& gl, int e, int i, double* h, d
   integrate_near(M, gl, e, px, py, h, g);
 }
 
+
+// ---------------------------------------------------------------------------
+// 2-D isotropic elasticity (Kelvin), H_G_full_direct for a Vectorial problem
+// (src/Core/Assembly_full.jl:120-175). Blocks are 2x2, component c = 2a + b
+// (row component a, column component b); dof of node j component b is 2j + b.
+// ---------------------------------------------------------------------------
+// fundamental(props::Elasticity, r, n) (src/Elasticity/Fundamental.jl:28-57),
+// r = x_field - x_source, n the field normal; mu and nu are the shear modulus
+// and the effective Poisson ratio (effective_nu, src/Core/Structures.jl:260-263).
+inline void kelvin(double rx, double ry, double nx, double ny, double mu, double nu,
+                   double* U, double* T) {
+  double R2 = rx * rx + ry * ry;
+  double R = std::sqrt(R2);
+  double dr1 = rx / R, dr2 = ry / R;
+  double drdn = dr1 * nx + dr2 * ny;
+  double prod1 = (4.0 * kPi) * (1.0 - nu);
+  double prod2 = (3.0 - 4.0 * nu) * std::log(1.0 / R);
+  double base = (2.0 * prod1) * mu;
+  U[0] = (prod2 + dr1 * dr1) / base;
+  U[1] = (dr1 * dr2) / base;
+  U[2] = U[1];
+  U[3] = (prod2 + dr2 * dr2) / base;
+  double fat = 1.0 / (prod1 * R);
+  double c12 = 1.0 - 2.0 * nu;
+  T[0] = ((-drdn) * (c12 + 2.0 * (dr1 * dr1))) * fat;
+  T[3] = ((-drdn) * (c12 + 2.0 * (dr2 * dr2))) * fat;
+  T[1] = (-(((drdn * 2.0) * dr1) * dr2 - c12 * (dr1 * ny - dr2 * nx))) * fat;
+  T[2] = (-(((drdn * 2.0) * dr1) * dr2 - c12 * (dr2 * nx - dr1 * ny))) * fat;
+}
+
+struct KMesh {
+  const Mesh* M;
+  double mu, nu;
+};
+
+// _sample_kernel (src/Core/Assembly_full.jl:607-643) with the Kelvin kernel.
+bool sample_kelvin(const KMesh& K, int e, double px, double py, double xi, double nrefx,
+                   double nrefy, double* U, double* T, double* Nrow, double* Jout) {
+  const Mesh& M = *K.M;
+  int nn = M.p + 1;
+  double dN[16], Ng[16], dNg[16];
+  shapefun(M.field, xi, Nrow, dN);
+  shapefun(M.geom, xi, Ng, dNg);
+  const double* g = M.geo + (size_t)e * nn * 2;
+  double pgx = 0, pgy = 0, dxx = 0, dxy = 0;
+  for (int k = 0; k < nn; ++k) {
+    pgx += Ng[k] * g[2 * k]; pgy += Ng[k] * g[2 * k + 1];
+    dxx += dNg[k] * g[2 * k]; dxy += dNg[k] * g[2 * k + 1];
+  }
+  double J = norm2d(dxx, dxy);
+  if (J < 1e-30) return false;
+  double rx = pgx - px, ry = pgy - py;
+  if (norm2d(rx, ry) < 1e-30) return false;
+  double nx = dxy / J, ny = -(dxx / J);
+  if (nx * nrefx + ny * nrefy < 0) { nx = -nx; ny = -ny; }
+  kelvin(rx, ry, nx, ny, K.mu, K.nu, U, T);
+  *Jout = J;
+  return true;
+}
+
+// Vectorial _integrate_singular_guiggiani! (src/Core/Assembly_full.jl:426-457):
+// the interpolated Laurent rule of the scalar case applied to every component,
+// orders (0, -1) (singularity_orders, src/Core/Structures.jl:84).
+// h, g: [node k][component c] (nn x 4).
+void integrate_singular_kelvin(const KMesh& K, int e, double px, double py, double a,
+                               double* h, double* g) {
+  const Mesh& M = *K.M;
+  int nn = M.p + 1;
+  a = std::min(std::max(a, std::nextafter(-1.0, 0.0)), std::nextafter(1.0, 0.0));
+  InterpRule R;
+  interp_rule(a, 20, &R);
+  size_t m = R.xi.size();
+  const double* nref = M.nrm + 2 * (size_t)e * nn;
+  std::vector<double> Fg(m * nn * 4, 0.0), Fh(m * nn * 4, 0.0);  // [(k*4+c)][i]
+  for (size_t i = 0; i < m; ++i) {
+    double U[4], T[4], Nrow[16], J;
+    if (!sample_kelvin(K, e, px, py, R.xi[i], nref[0], nref[1], U, T, Nrow, &J)) continue;
+    for (int k = 0; k < nn; ++k) {
+      double NjJ = Nrow[k] * J;
+      for (int c = 0; c < 4; ++c) {
+        Fg[(k * 4 + c) * m + i] = U[c] * NjJ;
+        Fh[(k * 4 + c) * m + i] = T[c] * NjJ;
+      }
+    }
+  }
+  for (int kc = 0; kc < nn * 4; ++kc) {
+    Laurent Cg = interp_log_coeff(&Fg[kc * m], R);
+    Laurent Ch = interp_cascade_m1(&Fh[kc * m], R);
+    g[kc] = interp_integrate(&Fg[kc * m], R, Cg, 0, a);
+    h[kc] = interp_integrate(&Fh[kc * m], R, Ch, -1, a);
+  }
+}
+
+// _quad_geom (:649-669) + vectorial _accumulate_kernels! (:697-713).
+void integrate_near_kelvin(const KMesh& K, const Rule& gl, int e, double px, double py,
+                           double* h, double* g) {
+  const Mesh& M = *K.M;
+  int nn = M.p + 1;
+  const double* xn = M.coll + 2 * (size_t)e * nn;
+  double zeta, eta;
+  complex_pole_1d(M.field, xn, nn, px, py, &zeta, &eta);
+  Rule q = tanp3ctrans(gl, zeta, eta);
+  const double* gg = M.geo + (size_t)e * nn * 2;
+  const double* nref = M.nrm + 2 * (size_t)e * nn;
+  size_t nq = q.x.size();
+  std::vector<double> N(nq * nn), rx(nq), ry(nq), nx(nq), ny(nq), wJ(nq);
+  for (size_t i = 0; i < nq; ++i) {
+    double dN[16], Ng[16], dNg[16];
+    shapefun(M.field, q.x[i], &N[i * nn], dN);
+    shapefun(M.geom, q.x[i], Ng, dNg);
+    double pgx = 0, pgy = 0, dxx = 0, dxy = 0;
+    for (int k = 0; k < nn; ++k) {
+      pgx += Ng[k] * gg[2 * k]; pgy += Ng[k] * gg[2 * k + 1];
+      dxx += dNg[k] * gg[2 * k]; dxy += dNg[k] * gg[2 * k + 1];
+    }
+    rx[i] = pgx - px; ry[i] = pgy - py;
+    double J = norm2d(dxx, dxy);
+    nx[i] = dxy / J; ny[i] = -(dxx / J);
+    wJ[i] = J * q.w[i];
+  }
+  if (nq > 0 && (nx[0] * nref[0] + ny[0] * nref[1]) < 0) {
+    for (size_t i = 0; i < nq; ++i) { nx[i] = -nx[i]; ny[i] = -ny[i]; }
+  }
+  for (int kc = 0; kc < nn * 4; ++kc) { h[kc] = 0.0; g[kc] = 0.0; }
+  for (size_t i = 0; i < nq; ++i) {
+    double U[4], T[4];
+    kelvin(rx[i], ry[i], nx[i], ny[i], K.mu, K.nu, U, T);
+    double wi = wJ[i];
+    for (int k = 0; k < nn; ++k) {
+      double Nj = N[i * nn + k] * wi;
+      for (int c = 0; c < 4; ++c) {
+        h[k * 4 + c] += T[c] * Nj;
+        g[k * 4 + c] += U[c] * Nj;
+      }
+    }
+  }
+}
+
+// integrate_element (:262-313) for a 2-D Kelvin pair with source = i.
+void integrate_element_kelvin(const KMesh& K, const Rule& gl, int e, int i, double* h,
+                              double* g) {
+  const Mesh& M = *K.M;
+  int nn = M.p + 1;
+  double px = M.coll[2 * (size_t)i], py = M.coll[2 * (size_t)i + 1];
+  bool singular = (i >= e * nn && i < (e + 1) * nn);
+  if (singular) {
+    const double* xn = M.coll + 2 * (size_t)e * nn;
+    double a, dist;
+    closest_point_1d(M.field, xn, px, py, seed_1d(M.field, xn, nn, px, py), &a, &dist);
+    integrate_singular_kelvin(K, e, px, py, a, h, g);
+    return;
+  }
+  integrate_near_kelvin(K, gl, e, px, py, h, g);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -997,6 +1152,109 @@ int oracle_apply_bc(int nt, int n, const double* H, const double* G, const int64
     else
       for (int i = 0; i < nt; ++i) b[i] += G[i + j * ld] * BV[j];
   }
+  return 0;
+}
+
+void oracle_kelvin(double rx, double ry, double nx, double ny, double mu, double nu, double* U,
+                   double* T) {
+  kelvin(rx, ry, nx, ny, mu, nu, U, T);
+}
+
+// Dense Kelvin H (2nt x 2nt) and G (2nt x 2n), column-major, as H_G_full_direct
+// for a Vectorial problem (src/Core/Assembly_full.jl:120-175): near pairs by
+// integrate_element, far pairs by _far_nodal_vec! (:812-822), then the
+// rigid-body free term _vectorial_free_term! (:777-786). row_lo/row_hi as in
+// oracle_assemble (node rows; the free term is skipped for a sub-range).
+int oracle_assemble_kelvin(int p, int ne, int n, int ni, const double* coll, const double* nrm,
+                           const double* J, const double* ew, const double* geo,
+                           const double* L, double mu, double nu, int npg, double near_factor,
+                           int nthreads, int row_lo, int row_hi, double* H, double* G) {
+  if (p < 1 || p > 14 || n != ne * (p + 1) || npg < 1) return -1;
+  Mesh M;
+  M.p = p; M.ne = ne; M.n = n; M.ni = ni; M.nt = n + ni;
+  M.coll = coll; M.nrm = nrm; M.J = J; M.ew = ew; M.geo = geo; M.L = L; M.k = 1.0;
+  M.field = legendre_poly(p);
+  M.geom = equispaced_poly(p);
+  KMesh K{&M, mu, nu};
+  const int nt = M.nt, nn = p + 1;
+  Rule gl;
+  gl.x.resize(npg); gl.w.resize(npg);
+  gausslegendre(npg, gl.x.data(), gl.w.data());
+  bool full = (row_lo <= 0 && row_hi >= nt);
+  if (row_lo < 0) row_lo = 0;
+  if (row_hi > nt) row_hi = nt;
+  const size_t ld = 2 * (size_t)nt;
+  if (full) {
+    std::memset(H, 0, sizeof(double) * ld * ld);
+    std::memset(G, 0, sizeof(double) * ld * 2 * (size_t)n);
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(static)
+#endif
+  for (int i = row_lo; i < row_hi; ++i) {
+    double px = coll[2 * (size_t)i], py = coll[2 * (size_t)i + 1];
+    double h[64], g[64];
+    for (int e = 0; e < ne; ++e) {
+      const double* xn = coll + 2 * (size_t)e * nn;
+      if (near_element(px, py, xn, nn, L[e], near_factor)) {
+        integrate_element_kelvin(K, gl, e, i, h, g);
+        for (int k = 0; k < nn; ++k) {
+          size_t j = (size_t)e * nn + k;
+          for (int c = 0; c < 4; ++c) {
+            size_t r = 2 * (size_t)i + (c >> 1), col = 2 * j + (c & 1);
+            H[r + col * ld] += h[k * 4 + c];
+            G[r + col * ld] += g[k * 4 + c];
+          }
+        }
+      } else {
+        for (int k = 0; k < nn; ++k) {
+          size_t node = (size_t)e * nn + k;
+          double U[4], T[4];
+          kelvin(coll[2 * node] - px, coll[2 * node + 1] - py, nrm[2 * node],
+                 nrm[2 * node + 1], mu, nu, U, T);
+          double wjk = J[node] * ew[k];
+          for (int c = 0; c < 4; ++c) {
+            size_t r = 2 * (size_t)i + (c >> 1), col = 2 * node + (c & 1);
+            H[r + col * ld] += T[c] * wjk;
+            G[r + col * ld] += U[c] * wjk;
+          }
+        }
+      }
+    }
+  }
+  if (full) {
+    // _vectorial_free_term!: H[ii, ii] = 0, then H[ii, ii[b]] = -sum over the
+    // columns of component b (the zeroed block included)
+    for (int i = 0; i < nt; ++i) {
+      for (int c = 0; c < 4; ++c)
+        H[2 * (size_t)i + (c >> 1) + (2 * (size_t)i + (c & 1)) * ld] = 0.0;
+      for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a) {
+          size_t r = 2 * (size_t)i + a;
+          double s = 0.0;
+          for (int j = 0; j < nt; ++j) s += H[r + (2 * (size_t)j + b) * ld];
+          H[r + (2 * (size_t)i + b) * ld] = -s;
+        }
+    }
+  }
+  return 0;
+}
+
+// One near or on-element Kelvin pair (i, e): [node k][component c] blocks.
+int oracle_pair_kelvin(int p, int ne, int n, int ni, const double* coll, const double* nrm,
+                       const double* J, const double* ew, const double* geo, const double* L,
+                       double mu, double nu, int npg, int i, int e, double* h, double* g) {
+  Mesh M;
+  M.p = p; M.ne = ne; M.n = n; M.ni = ni; M.nt = n + ni;
+  M.coll = coll; M.nrm = nrm; M.J = J; M.ew = ew; M.geo = geo; M.L = L; M.k = 1.0;
+  M.field = legendre_poly(p);
+  M.geom = equispaced_poly(p);
+  KMesh K{&M, mu, nu};
+  Rule gl;
+  gl.x.resize(npg); gl.w.resize(npg);
+  gausslegendre(npg, gl.x.data(), gl.w.data());
+  integrate_element_kelvin(K, gl, e, i, h, g);
   return 0;
 }
 

@@ -52,6 +52,12 @@ def lib():
         L.oracle_pair.argtypes = [C.c_int] * 4 + [dp] * 6 + [C.c_double, C.c_int, C.c_int,
                                                              C.c_int, dp, dp]
         L.oracle_apply_bc.argtypes = [C.c_int, C.c_int, dp, dp, ip64, dp, dp, dp]
+        L.oracle_kelvin.argtypes = [C.c_double] * 6 + [dp, dp]
+        L.oracle_kelvin.restype = None
+        L.oracle_assemble_kelvin.argtypes = [C.c_int] * 4 + [dp] * 6 + [
+            C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.oracle_pair_kelvin.argtypes = [C.c_int] * 4 + [dp] * 6 + [
+            C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, dp, dp]
         _L = L
     return _L
 
@@ -237,3 +243,79 @@ def solve(dad, H, G):
     T[: dad.n][d] = dad.BV[d]
     q[~d] = dad.BV[~d]
     return T, q, x
+
+
+# ---------------------------------------------------------------------------
+# 2-D isotropic elasticity (Kelvin): H_G_full_direct for a Vectorial problem.
+# ---------------------------------------------------------------------------
+
+def _kelvin_consts(dad):
+    pr = dad.properties
+    return float(pr.mu), float(pr.effective_nu())
+
+
+def kelvin(r, n, mu, nu):
+    """U, T (2x2) of fundamental(props::Elasticity, r, n)
+    (src/Elasticity/Fundamental.jl:28-57); r = x_field - x_source."""
+    U = np.empty(4)
+    T = np.empty(4)
+    lib().oracle_kelvin(float(r[0]), float(r[1]), float(n[0]), float(n[1]), float(mu),
+                        float(nu), _p(U), _p(T))
+    return U.reshape(2, 2), T.reshape(2, 2)
+
+
+def assemble_kelvin(dad, npg=20, near_factor=1.5, nthreads=0, rows=None):
+    """H (2nt x 2nt), G (2nt x 2n), column-major, dof 2j + b for node j
+    component b (src/Core/Assembly_full.jl:120-175)."""
+    coll, nrm, J, ew, geo, L = _geom_args(dad)
+    nt, n = dad.nt, dad.n
+    mu, nu = _kelvin_consts(dad)
+    H = np.zeros((2 * nt, 2 * nt), order="F")
+    G = np.zeros((2 * nt, 2 * n), order="F")
+    lo, hi = (0, nt) if rows is None else rows
+    st = lib().oracle_assemble_kelvin(dad.p, dad.ne, n, dad.ni, _p(coll), _p(nrm), _p(J), _p(ew),
+                                      _p(geo), _p(L), mu, nu, int(npg), float(near_factor),
+                                      int(nthreads), int(lo), int(hi), H.ctypes.data_as(dp),
+                                      G.ctypes.data_as(dp))
+    if st != 0:
+        raise ValueError("oracle_assemble_kelvin: bad arguments")
+    return H, G
+
+
+def pair_kelvin(dad, i, e, npg=20):
+    """Near / on-element Kelvin pair (i, e): h, g of shape (p+1, 2, 2)."""
+    coll, nrm, J, ew, geo, L = _geom_args(dad)
+    mu, nu = _kelvin_consts(dad)
+    nn = dad.p + 1
+    h = np.empty(nn * 4)
+    g = np.empty(nn * 4)
+    lib().oracle_pair_kelvin(dad.p, dad.ne, dad.n, dad.ni, _p(coll), _p(nrm), _p(J), _p(ew),
+                             _p(geo), _p(L), mu, nu, int(npg), int(i), int(e), _p(h), _p(g))
+    return h.reshape(nn, 2, 2), g.reshape(nn, 2, 2)
+
+
+def apply_bc_kelvin(dad, H, G):
+    """applyBC for elasticity (src/Core/Boundary_conditions.jl:153-175): per dof."""
+    nd, nb = 2 * dad.nt, 2 * dad.n
+    H = np.asfortranarray(H, dtype=np.float64)
+    G = np.asfortranarray(G, dtype=np.float64)
+    A = np.empty((nd, nd), order="F")
+    b = np.empty(nd)
+    BC = np.ascontiguousarray(dad.BC, dtype=np.int64)
+    BV = np.ascontiguousarray(dad.BV, dtype=np.float64)
+    lib().oracle_apply_bc(nd, nb, H.ctypes.data_as(dp), G.ctypes.data_as(dp),
+                          BC.ctypes.data_as(ip64), _p(BV), A.ctypes.data_as(dp), _p(b))
+    return A, b
+
+
+def solve_kelvin(dad, H, G):
+    """applyBC + dense LU + split_sol! for elasticity (src/Core/Solver.jl:115-146):
+    returns u (2n), traction (2n) and the solution x (2nt)."""
+    import scipy.linalg as sl
+    A, b = apply_bc_kelvin(dad, H, G)
+    x = sl.lu_solve(sl.lu_factor(A), b)
+    nb = 2 * dad.n
+    d = dad.BC == 0
+    u = np.where(d, dad.BV, x[:nb])
+    t = np.where(d, x[:nb], dad.BV)
+    return u, t, x

@@ -1,0 +1,282 @@
+"""2-D elasticity (Kelvin) on the GPU against the FP64 oracle.
+
+H_G_full_direct for a Vectorial problem (src/Core/Assembly_full.jl:120-175),
+the per-dof applyBC (src/Core/Boundary_conditions.jl:153-175) and the solve
+(src/Core/Solver.jl:115-146) through libbemhip.so, compared on the same inputs
+with oracle.assemble_kelvin / apply_bc_kelvin / solve_kelvin:
+  far and on-element blocks   1e-12 per entry (row-scale floor 1e-3)
+  near blocks                 1e-12 per entry, or within 10x of the oracle's
+                              own change under a 1-ulp move of the source or
+                              element nodes (the rule's nodes come from device
+                              atan/tan/log: ulp-level differences, amplified
+                              by the pair's conditioning)
+  rigid-body diagonal blocks  within the row's summed entry differences
+  A, b of the BC swap         bit-identical to the oracle's applyBC applied to
+                              the GPU's own H, G (same operation order)
+  FP64 / mixed solutions      1e-9 relative to the oracle's LAPACK solve
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import bem_gmsh_amd as B
+from bem_gmsh_amd._lib import lib
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ANA = B.ana_elasticity_patch(1.0, 0.3, 0.01, 0.004, 0.002)
+
+
+def square_internal():
+    d = B.format2d(B.quadrado_elasticity(ndiv=10), B.Elasticity(1.0, 0.3, 1.0),
+                   pontointerno=True)
+    B.apply_bc_from_physical(d, ANA)
+    return d
+
+
+def ellipse_p2():
+    d = B.format2d(B.ellipse_mesh(150), B.Elasticity(1.0, 0.3, 1.0), pontointerno=False)
+    B.apply_analytical_bc(d, ANA, range(0, d.n, 2))
+    return d
+
+
+def star_p2():
+    d = B.format2d(B.star_mesh(160, seed=3), B.Elasticity(2.0, 0.25, 1.0, plane_stress=True),
+                   pontointerno=False)
+    B.apply_analytical_bc(d, B.ana_elasticity_patch(2.0, 0.2, -0.003, 0.006, 0.001),
+                          range(1, d.n, 3))
+    return d
+
+
+CASES = {"square_p1_internal": square_internal, "ellipse_p2": ellipse_p2, "star_p2": star_p2}
+
+
+def _near_mask(d, npg=16):
+    """[node row, node col] True where the oracle integrates the pair (i, elem(j))."""
+    nn = d.p + 1
+    M = np.zeros((d.nt, d.n), dtype=bool)
+    for e in range(d.ne):
+        xg = d.collocation[e * nn:(e + 1) * nn]
+        for i in range(d.nt):
+            if (e * nn <= i < (e + 1) * nn) or O.near_element(d.collocation[i], xg,
+                                                              d.elem_L[e]):
+                M[i, e * nn:(e + 1) * nn] = True
+    return M
+
+
+def _pair_ulp_sensitivity(d, i, e, npg):
+    """Largest relative change of the oracle's Kelvin (h, g) for pair (i, e)
+    when one coordinate of the source point or of the element's collocation
+    nodes moves by one ulp: the pair's intrinsic FP64 conditioning."""
+    import copy
+    h0, g0 = O.pair_kelvin(d, i, e, npg)
+    nn = d.p + 1
+    worst = 0.0
+    for pt in [i] + [e * nn + k for k in range(nn)]:
+        for comp in (0, 1):
+            for sgn in (1.0, -1.0):
+                d2 = copy.copy(d)
+                d2.collocation = d.collocation.copy()
+                d2.collocation[pt, comp] = np.nextafter(d2.collocation[pt, comp], sgn * np.inf)
+                h1, g1 = O.pair_kelvin(d2, i, e, npg)
+                for a, b in ((h1, h0), (g1, g0)):
+                    floor = 1e-3 * np.abs(b).max()
+                    rel = float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+                    worst = max(worst, rel)
+    return worst
+
+
+def _block_err(A, Ref, nt, ncol):
+    """Per-entry error with a floor of 1e-3 * max|row of Ref|, as (nt, ncol) node blocks."""
+    rowmax = np.abs(Ref).max(axis=1, keepdims=True)
+    den = np.maximum(np.abs(Ref), 1e-3 * rowmax)
+    den = np.where(den > 0, den, 1.0)
+    e = (np.abs(A - Ref) / den).reshape(nt, 2, ncol, 2)
+    return e.max(axis=(1, 3))
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_kelvin_assembly_matches_oracle(case):
+    d = CASES[case]()
+    npg = 16
+    print(f"\n[{case}] nt={d.nt} n={d.n} p={d.p}", flush=True)
+    H, G = B.assemble(d, npg)
+    Hr, Gr = O.assemble_kelvin(d, npg)
+    near = _near_mask(d, npg)
+    nt, n = d.nt, d.n
+    eG = _block_err(G, Gr, nt, n)
+    eH = _block_err(H, Hr, nt, nt)[:, :n]
+    diag = np.zeros((nt, n), dtype=bool)
+    diag[np.arange(n), np.arange(n)] = True
+    nn = d.p + 1
+    onel = np.zeros((nt, n), dtype=bool)
+    for i in range(n):
+        e = i // nn
+        onel[i, e * nn:(e + 1) * nn] = True
+    far = ~near
+    nreg = near & ~onel
+    print(f"  G far {eG[far].max():.2e} onel {eG[onel].max():.2e} near {eG[nreg].max():.2e}; "
+          f"H far {eH[far].max():.2e} onel {eH[onel & ~diag].max():.2e} "
+          f"near {eH[nreg].max():.2e}", flush=True)
+    assert eG[far].max() <= 1e-12 and eH[far].max() <= 1e-12
+    assert eG[onel].max() <= 1e-12 and eH[onel & ~diag].max() <= 1e-12
+    # near pairs: 1e-12, or within 10x of the oracle's own 1-ulp sensitivity
+    worst = 0.0
+    nsens = 0
+    for e in range(d.ne):
+        cols = slice(e * nn, (e + 1) * nn)
+        for i in np.nonzero(nreg[:, e * nn])[0]:
+            err = max(eG[i, cols].max(), eH[i, cols].max())
+            if err <= 1e-12:
+                continue
+            kappa = _pair_ulp_sensitivity(d, int(i), e, npg)
+            nsens += 1
+            worst = max(worst, err / max(kappa, 1e-300))
+            assert err <= 10.0 * kappa, (int(i), e, err, kappa)
+    print(f"  near pairs above 1e-12: {nsens}, worst err / ulp sensitivity {worst:.2f}",
+          flush=True)
+    # rigid-body diagonal blocks: translations in the null space of H, and the
+    # blocks within the row's summed entry differences of the oracle's
+    for b in range(2):
+        tr = np.zeros(2 * nt)
+        tr[b::2] = 1.0
+        assert np.abs(H @ tr).max() <= 1e-12 * np.abs(H).max() * nt ** 0.5
+    Hb, Hrb = H.reshape(nt, 2, nt, 2), Hr.reshape(nt, 2, nt, 2)
+    off = np.abs(H - Hr).reshape(nt, 2, nt, 2).copy()
+    idx = np.arange(nt)
+    off[idx, :, idx, :] = 0.0
+    # + the rounding of the row sums taken in another order (cancellation to ~0)
+    rowabs = np.abs(Hr).sum(axis=1).reshape(nt, 2)[:, :, None]
+    bound = off.sum(axis=(2, 3))[:, :, None] + 1e-13 * rowabs
+    assert np.all(np.abs(Hb[idx, :, idx, :] - Hrb[idx, :, idx, :]) <= bound)
+    # internal-node columns: zero but the diagonal blocks
+    if d.ni:
+        cols = Hb[:, :, n:, :].copy()
+        cols[np.arange(n, nt), :, np.arange(nt - n), :] = 0.0
+        assert not cols.any()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_kelvin_bc_and_solves_match_oracle(case):
+    d = CASES[case]()
+    H, G = B.assemble(d, 16)
+    B.apply_bc(d, fetch=True)
+    A0, b0 = O.apply_bc_kelvin(d, H, G)
+    assert np.array_equal(d.A, A0) and np.array_equal(d.b, b0)
+    Hr, Gr = O.assemble_kelvin(d, 16)
+    ur, tr, xr = O.solve_kelvin(d, Hr, Gr)
+    u = B.solve(d, mode="fp64")
+    x64 = d.solve_info["x"]
+    e64 = np.linalg.norm(x64 - xr) / np.linalg.norm(xr)
+    assert e64 <= 1e-9
+    assert np.array_equal(d.T, d.u) and d.traction.shape == (2 * d.n,)
+    B.solve(d, mode="mixed", tol=1e-12)
+    xm = d.solve_info["x"]
+    em = np.linalg.norm(xm - x64) / np.linalg.norm(x64)
+    print(f"\n[{case}] fp64 vs oracle {e64:.2e}, mixed vs fp64 {em:.2e}, refine "
+          f"{d.solve_info['iters']} relres {d.solve_info['relres']:.1e}", flush=True)
+    assert em <= 1e-9 and d.solve_info["relres"] <= 1e-12
+    # the operator of the mixed refinement is the stored FP64 system
+    z = np.random.default_rng(1).standard_normal(2 * d.nt)
+    y = B.apply_operator(d, z)
+    assert np.abs(y - A0 @ z).max() <= 1e-12 * np.abs(A0).sum(axis=1).max()
+
+
+def test_kelvin_near_pairs_match_oracle_pairs():
+    d = B.format2d(B.quadrado_elasticity(ndiv=9), B.Elasticity(1.0, 0.3), pontointerno=True)
+    B.assemble(d, 16, fetch=False)
+    rows, elems, h, g = B.near_pairs(d)
+    assert h.shape == (rows.size, 2, 2, 2)
+    rng = np.random.default_rng(4)
+    doubled = 0
+    pick = list(rng.choice(rows.size, 60, replace=False))
+    # rows on the extension of a corner element: the rule's collapsed half
+    pick += [k for k in range(rows.size) if (rows[k], elems[k]) in ((45, 24), (32, 15))]
+    for k in pick:
+        hr, gr = O.pair_kelvin(d, int(rows[k]), int(elems[k]), npg=16)
+        s = max(np.abs(hr).max(), np.abs(gr).max())
+        assert np.abs(h[k] - hr).max() <= 1e-10 * s and np.abs(g[k] - gr).max() <= 1e-10 * s
+        doubled += (rows[k], elems[k]) in ((45, 24), (32, 15))
+    assert doubled == 2
+
+
+def test_reference_elasticity_patch_gpu():
+    # test/elasticity.jl:27-36 on the GPU path: rel_error < 0.15, tractions as the oracle's
+    d = B.format2d(B.quadrado_elasticity(ndiv=8), B.Elasticity(1.0, 0.3, 1.0),
+                   pontointerno=False)
+    ana = B.ana_elasticity_patch(E=1.0, nu=0.3, exx=0.01)
+    B.apply_analytical_bc(d, ana)
+    B.assemble(d, 12)
+    B.solve(d)
+    assert B.rel_error(d) < 0.15
+    Hr, Gr = O.assemble_kelvin(d, 12)
+    ur, tr, _ = O.solve_kelvin(d, Hr, Gr)
+    assert np.linalg.norm(d.traction - tr) <= 1e-9 * np.linalg.norm(tr)
+
+
+def test_kelvin_patch_on_ellipse_gpu():
+    d = B.format2d(B.ellipse_mesh(300), B.Elasticity(1.0, 0.3), pontointerno=False)
+    B.apply_analytical_bc(d, ANA, range(0, d.n, 2))
+    B.assemble(d, 16, fetch=False)
+    B.solve(d, mode="mixed")
+    assert B.rel_error(d) < 1e-3
+    assert B.rel_error_flux(d) < 0.05
+
+
+def _ctx_with_geometry(d):
+    ctx = B.HipContext(0)
+    B.api.upload_geometry(d, ctx)
+    return ctx
+
+
+def test_kelvin_argument_and_state_errors():
+    L = lib()
+    d = B.format2d(B.quadrado_elasticity(ndiv=6), B.Elasticity(1.0, 0.3), pontointerno=False)
+    ctx = B.HipContext(0)
+    assert L.bemhip_set_elasticity(ctx.h, 0.4, 0.3) == 3          # no geometry yet
+    ctx.close()
+    ctx = _ctx_with_geometry(d)
+    assert L.bemhip_set_elasticity(ctx.h, 0.4, 0.5) == 1          # nu must be < 0.5
+    assert L.bemhip_set_elasticity(ctx.h, -1.0, 0.3) == 1         # mu must be > 0
+    assert L.bemhip_set_elasticity(ctx.h, 0.4, 0.3) == 0
+    assert L.bemhip_assemble(ctx.h, 16, 1.5, 1) == 1              # FP64 only
+    assert L.bemhip_assemble(ctx.h, 16, 1.5, 3) == 1
+    assert L.bemhip_assemble(ctx.h, 16, 1.5, 0) == 0
+    BC = np.zeros(2 * d.n, dtype=np.int64)
+    BV = np.zeros(2 * d.n)
+    BC[1] = 2
+    assert L.bemhip_apply_bc(ctx.h, BC.ctypes.data_as(C.POINTER(C.c_int64)),
+                             BV.ctypes.data_as(C.POINTER(C.c_double)), None, None) == 1
+    BC[1] = 0
+    assert L.bemhip_apply_bc(ctx.h, BC.ctypes.data_as(C.POINTER(C.c_int64)),
+                             BV.ctypes.data_as(C.POINTER(C.c_double)), None, None) == 0
+    x = np.empty(2 * d.nt)
+    it, rel = C.c_int(0), C.c_double(0.0)
+    assert L.bemhip_solve_gmres(ctx.h, 0.0, 1e-8, 30, 100,
+                                x.ctypes.data_as(C.POINTER(C.c_double)), C.byref(it),
+                                C.byref(rel)) == 3
+    pts = np.array([[0.5, 0.5]])
+    u = np.empty(1)
+    z = np.zeros(d.n)
+    dp = C.POINTER(C.c_double)
+    assert L.bemhip_interior_laplace(ctx.h, 1, pts.ctypes.data_as(dp), z.ctypes.data_as(dp),
+                                     z.ctypes.data_as(dp), 16, 1e300,
+                                     u.ctypes.data_as(dp)) == 3
+    ctx.close()
+    with pytest.raises(B.ArgumentError):
+        B.assemble(d, 16, precision="mixed")
+
+
+def test_laplace_after_elasticity_on_one_context():
+    # set_geometry resets the problem: the same context then assembles Laplace
+    d = B.format2d(B.quadrado_elasticity(ndiv=7), B.Elasticity(1.0, 0.3), pontointerno=True)
+    B.assemble(d, 16, fetch=False)
+    dl = B.format2d(B.quadrado(ndiv=7), B.Laplace(1.0), pontointerno=True)
+    dl.hip = d.hip
+    H, G = B.assemble(dl, 16)
+    Hr, Gr = O.assemble(dl, 16)
+    assert H.shape == (dl.nt, dl.nt)
+    assert np.abs(G - Gr).max() <= 1e-10 * np.abs(Gr).max()
