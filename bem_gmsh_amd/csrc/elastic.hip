@@ -22,8 +22,9 @@
 //   kelvin_finalize  rigid-body free term (_vectorial_free_term!, :777-786):
 //                   H[ii, ii] block = -(sum over the other nodes' blocks) per
 //                   component, and the near blocks into H and G.
-//   kelvin_rhs_kernel, kelvin_swap_kernel   applyBC: b (the reference's loop
-//                   order over dofs) and A = H with Dirichlet columns -G.
+//   kelvin_rhs_part, kelvin_swap_kernel   applyBC: b (chunks of dofs, each in
+//                   the reference's loop order, then a fixed-order sum of the
+//                   chunks) and A = H with Dirichlet columns -G.
 //   kelvin_apply_part / kelvin_apply_sum   y = A x from the stored H, G with
 //                   the BC column selection (chunks of columns, fixed-order
 //                   sum): the FP64 residual of the mixed-precision refinement.
@@ -471,25 +472,33 @@ __global__ __launch_bounds__(256) void kelvin_finalize(int nt, int nn, int ncb, 
   *(d2v*)(H + ad + ld) = d2v{D[1], D[3]};
 }
 
-// b_r = sum over boundary dofs j in order: -H_rj BV_j (BC 0) or + G_rj BV_j
-// (src/Core/Boundary_conditions.jl:163-172).
-__global__ __launch_bounds__(256) void kelvin_rhs_kernel(int ns, int nb, size_t ld,
-                                                         const int* __restrict__ bc,
-                                                         const double* __restrict__ bv,
-                                                         const double* __restrict__ H,
-                                                         const double* __restrict__ G,
-                                                         double* __restrict__ b) {
+// y = A x with A's columns read from H or -G: part[chunk][r] over columns
+// [chunk * kApplyCols, (chunk + 1) * kApplyCols), then the chunks summed in
+// order (kelvin_apply_sum).
+constexpr int kApplyCols = 256;
+
+// b_r = sum over the boundary dofs j of -H_rj BV_j (BC 0) or + G_rj BV_j
+// (src/Core/Boundary_conditions.jl:163-172): each chunk of columns adds its
+// terms in the reference's dof order, the chunks are summed in order.
+__global__ __launch_bounds__(256) void kelvin_rhs_part(int ns, int nb, size_t ld,
+                                                       const int* __restrict__ bc,
+                                                       const double* __restrict__ bv,
+                                                       const double* __restrict__ H,
+                                                       const double* __restrict__ G,
+                                                       double* __restrict__ part) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j0 = blockIdx.y * kApplyCols;
+  const int j1 = min(nb, j0 + kApplyCols);
   if (r >= ns) return;
   double s = 0.0;
-  for (int j = 0; j < nb; ++j) {
+  for (int j = j0; j < j1; ++j) {
     const size_t at = (size_t)j * ld + r;
     if (bc[j] == 0)
       s -= H[at] * bv[j];
     else
       s += G[at] * bv[j];
   }
-  b[r] = s;
+  part[(size_t)blockIdx.y * ns + r] = s;
 }
 
 // A = H with the Dirichlet columns (BC 0) replaced by -G; workgroup row y
@@ -506,9 +515,6 @@ __global__ void kelvin_swap_kernel(int ns, int nb, size_t ld, const int* __restr
   }
 }
 
-// y = A x with A's columns read from H or -G: part[chunk][r] over columns
-// [chunk * cw, (chunk + 1) * cw), then the chunks summed in order.
-constexpr int kApplyCols = 256;
 __global__ __launch_bounds__(256) void kelvin_apply_part(int ns, int nb, size_t ld,
                                                          const int* __restrict__ bc,
                                                          const double* __restrict__ H,
@@ -606,7 +612,7 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
     BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + nb * c->ld64, 0,
                                      sizeof(double) * c->ld64 * (ns - nb), c->stream));
   BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
-  // events: 0 start, 1 classified, 2 far done, 3 far start, 4 pairs listed,
+  // events: 0 start, 1 classified, 2 far done, 3 far start, 4 pairs counted,
   // 5 near done, 6 finalized
   hipEvent_t ev[7];
   for (auto& e : ev) BEMHIP_HIP_TRY(c, hipEventCreate(&e));
@@ -633,8 +639,14 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
+  // the pair count (one workgroup) before the far field fills the GPU
+  BEMHIP_HIP_TRY(c, ensure_count_word(c));
+  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
+                     c->row_ptr.ptr, c->d_count);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[4], c->stream));
   // stream 2: the far field (the memsets above are ordered before it)
-  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[4], 0));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[3], s2));
   {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
@@ -648,10 +660,6 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], s2));
   // stream 1: pair CSR and the near rules
-  BEMHIP_HIP_TRY(c, ensure_count_word(c));
-  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
-                     c->row_ptr.ptr, c->d_count);
-  BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   const int total = *(volatile int*)c->h_count;
   c->npairs = total;
@@ -668,7 +676,6 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
                      0, c->n, c->nearbits.ptr, c->tiles.ptr, c->row_ptr.ptr, c->pair_row.ptr,
                      c->pair_elem.ptr, c->sing_idx.ptr, c->reg_idx.ptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
-  BEMHIP_HIP_TRY(c, hipEventRecord(ev[4], c->stream));
   int st = (c->nn == 2) ? launch_kelvin_pairs<2>(c, c->stream, g, K, nsing, c->sing_idx.ptr,
                                                  nreg, c->reg_idx.ptr)
                         : launch_kelvin_pairs<3>(c, c->stream, g, K, nsing, c->sing_idx.ptr,
@@ -689,7 +696,7 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   };
   c->ms["assemble_classify"] = el(0, 1);
   c->ms["assemble_far"] = el(3, 2);
-  c->ms["assemble_pairs"] = el(1, 4);
+  c->ms["assemble_scan"] = el(1, 4);
   c->ms["assemble_near"] = el(4, 5);
   c->ms["assemble_finalize"] = el(5, 6);
   c->stats["near_pairs"] = (double)total;
@@ -701,9 +708,14 @@ int kelvin_apply_bc_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, c->b64.ensure((size_t)ns));
   {
     ScopedTimer t(c, "apply_bc_rhs");
-    hipLaunchKernelGGL(kelvin_rhs_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
-                       c->stream, ns, nb, c->ld64, c->d_bc.ptr, c->d_bv.ptr, c->H64.ptr,
-                       c->G64.ptr, c->b64.ptr);
+    const int nchunk = (nb + kApplyCols - 1) / kApplyCols;
+    BEMHIP_HIP_TRY(c, c->part1.ensure((size_t)nchunk * ns));
+    const unsigned rb = (unsigned)((ns + 255) / 256);
+    hipLaunchKernelGGL(kelvin_rhs_part, dim3(rb, (unsigned)nchunk), dim3(256), 0, c->stream, ns,
+                       nb, c->ld64, c->d_bc.ptr, c->d_bv.ptr, c->H64.ptr, c->G64.ptr,
+                       c->part1.ptr);
+    hipLaunchKernelGGL(kelvin_apply_sum, dim3(rb), dim3(256), 0, c->stream, ns, nchunk,
+                       c->part1.ptr, c->b64.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
   {

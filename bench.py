@@ -20,7 +20,11 @@ value = H,G entries per second of the assembly phase (2 nt n entries per
 assembly / assembly seconds, HIP events on the library stream, averaged over
 the timed steps). ms_per_step = wall time of a whole step.
 
-After the timed steps (outside them): the adjoint shape gradient on the
+After the timed steps (outside them): 2-D elasticity (Kelvin) on a star of
+N = 32766 dofs (5461 quadratic elements, plane strain E = 1, nu = 0.3, uniform
+strain field, traction on the Neumann arcs): FP64 assembly, mixed and FP64
+solves, the Kelvin far kernel's roofline and an oracle CPU sample; the
+adjoint shape gradient on the
 N = 4095 star (1365 elements, npg = 12, target fluxes from the FP64 solve on
 the shape with a_m shifted by U(-0.01, 0.01)) after an FP64 and after a mixed
 solve, and the CPU baseline (the FP64 restatement of the reference algorithm
@@ -187,6 +191,84 @@ def adjoint_leg(ne=1365, seed=1, npg=12):
     return out
 
 
+def elastic_leg(ne=5461, npg=16, seed=1, cpu_rows=1024, threads=1, reps=2):
+    """Kelvin elasticity on the star: assemble (FP64) -> per-dof BCs -> mixed
+    solve (FP32 LU + FP64 refinement with the stored FP64 system) and FP64
+    solve; HIP-event times from the library, accuracy against the exact
+    uniform-strain field. Roofline of kelvin_far_kernel: 64 bytes of H and G
+    written per far (row node, column node) pair."""
+    mesh = B.star_mesh(ne, seed=seed)
+    d = B.format2d(mesh, B.Elasticity(1.0, 0.3, 1.0))
+    neu = np.nonzero(d.BC.reshape(-1, 2)[:, 0] == 1)[0]
+    ana = B.ana_elasticity_patch(1.0, 0.3, 0.01, -0.004, 0.003)
+    B.apply_analytical_bc(d, ana, neu)
+    nt, n = d.nt, d.n
+    ctx = B.HipContext(0)
+    d.hip = ctx
+    res = []
+    for r in range(reps):
+        B.assemble(d, npg, fetch=False)
+        B.solve(d, mode="mixed", tol=1e-10)
+        c = ctx.counters()
+        res.append({"assemble_ms": c["ms"]["assemble"], "far_ms": c["ms"]["assemble_far"],
+                    "classify_ms": c["ms"]["assemble_classify"],
+                    "scan_ms": c["ms"]["assemble_scan"],
+                    "near_chain_ms": c["ms"]["assemble_near"],
+                    "finalize_ms": c["ms"]["assemble_finalize"],
+                    "rhs_ms": c["ms"]["apply_bc_rhs"], "swap_ms": c["ms"]["apply_bc_swap"],
+                    "lu32_ms": c["ms"]["lu32"],
+                    "refine_ms": c["ms"]["refine"], "refine_iters": d.solve_info["iters"],
+                    "relres": d.solve_info["relres"], "near_pairs": c["stats"]["near_pairs"]})
+        log(f"[bench] elastic rep {r + 1}/{reps}: assemble {res[-1]['assemble_ms']:.2f} ms "
+            f"(far {res[-1]['far_ms']:.2f}), lu32 {res[-1]['lu32_ms']:.1f} ms, refine "
+            f"{res[-1]['refine_ms']:.1f} ms ({res[-1]['refine_iters']} it)")
+    last = res[-1]
+    xm = d.solve_info["x"].copy()
+    err_u_mixed = B.rel_error(d)
+    err_t_mixed = B.rel_error_flux(d)
+    B.solve(d, mode="fp64")
+    c = ctx.counters()
+    x64 = d.solve_info["x"]
+    far_bytes = 64.0 * nt * n
+    far_s = last["far_ms"] * 1e-3
+    out = {
+        "workload": f"star N={2 * nt} dofs ({d.ne} quadratic elements, {nt} nodes), npg={npg}, "
+                    "Kelvin plane strain E=1 nu=0.3, uniform strain, traction on the "
+                    "Neumann arcs",
+        "reps": res, "assemble_ms": last["assemble_ms"],
+        "entries_per_s": 8.0 * nt * n / (last["assemble_ms"] * 1e-3),
+        "lu32_ms": last["lu32_ms"], "refine_ms": last["refine_ms"],
+        "solve_mixed_s": (last["lu32_ms"] + last["refine_ms"]) * 1e-3,
+        "lu64_ms": c["ms"]["lu64"], "solve_fp64_s": c["ms"]["lu64"] * 1e-3,
+        "mixed_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
+        "rel_error_u_mixed": err_u_mixed, "rel_error_traction_mixed": err_t_mixed,
+        "rel_error_u_fp64": B.rel_error(d), "rel_error_traction_fp64": B.rel_error_flux(d),
+        "roofline_far": {"kernel": "kelvin_far_kernel<3> (Kelvin H,G far field, FP64)",
+                         "bound": "hbm", "achieved": far_bytes / far_s / 1e9,
+                         "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                         "frac": far_bytes / far_s / PEAK_HBM, "traffic": None,
+                         "algorithmic_bytes": far_bytes},
+    }
+    ctx.close()
+    if cpu_rows:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        rows = min(cpu_rows, nt)
+        t = time.perf_counter()
+        O.assemble_kelvin(d, npg, nthreads=threads, rows=(0, rows))
+        ts = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 8.0 * rows * n / ts, "unit": "entries/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"oracle Kelvin assembly of rows 0..{rows} "
+                                         f"({ts:.2f} s)"}
+    log(f"[bench] elastic N={2 * nt}: assemble {out['assemble_ms']:.2f} ms "
+        f"({out['entries_per_s']:.3g} entries/s, far kernel "
+        f"{out['roofline_far']['achieved']:.0f} GB/s), solve mixed "
+        f"{out['solve_mixed_s']:.3f} s vs fp64 {out['solve_fp64_s']:.3f} s, "
+        f"u err {out['rel_error_u_mixed']:.2e}, traction err {out['rel_error_traction_mixed']:.2e}")
+    return out
+
+
 def interior_leg(ctx, dad, npg, T, q, m_far=65536, m_near=2048, seed=3):
     """Potential at random points inside the star from the boundary solution:
     m_far points with the assembly's rules (far lumping beyond 1.5 L) and m_near
@@ -293,6 +375,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-interior", action="store_true")
+    ap.add_argument("--no-elastic", action="store_true")
+    ap.add_argument("--elastic-only", action="store_true",
+                    help="only the elasticity leg (for profiling)")
     ap.add_argument("--gmres-large", default="", metavar="SPEC",
                     help="only: matrix-free assembly + GMRES; SPEC square:NDIV (linear elements, "
                          "T = x) or star:NE (quadratic elements)")
@@ -313,6 +398,9 @@ def main():
         return
     if args.gmres_large:
         gmres_large(args.gmres_large, args.npg, far=args.far)
+        return
+    if args.elastic_only:
+        print(json.dumps({"elastic": elastic_leg(npg=args.npg, cpu_rows=0)}))
         return
     t0 = time.perf_counter()
     dad = build_problem(args.ne)
@@ -402,6 +490,10 @@ def main():
 
     if not args.no_adjoint:
         detail["adjoint"] = adjoint_leg()
+    if not args.no_elastic:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        detail["elastic"] = elastic_leg(npg=args.npg, cpu_rows=0 if args.no_cpu else 1024,
+                                        threads=threads)
 
     # roofline of the dominant kernel: the LU's FP32 trailing updates on
     # 256x256 tiles, each launch timed with HIP events on its stream

@@ -11,8 +11,9 @@ with oracle.assemble_kelvin / apply_bc_kelvin / solve_kelvin:
                               atan/tan/log: ulp-level differences, amplified
                               by the pair's conditioning)
   rigid-body diagonal blocks  within the row's summed entry differences
-  A, b of the BC swap         bit-identical to the oracle's applyBC applied to
-                              the GPU's own H, G (same operation order)
+  A, b of the BC swap         A bit-identical to the oracle's applyBC applied
+                              to the GPU's own H, G; b within the rounding of
+                              its sum taken in chunks of 256 dofs
   FP64 / mixed solutions      1e-9 relative to the oracle's LAPACK solve
 """
 
@@ -165,7 +166,9 @@ def test_kelvin_bc_and_solves_match_oracle(case):
     H, G = B.assemble(d, 16)
     B.apply_bc(d, fetch=True)
     A0, b0 = O.apply_bc_kelvin(d, H, G)
-    assert np.array_equal(d.A, A0) and np.array_equal(d.b, b0)
+    assert np.array_equal(d.A, A0)
+    terms = np.abs(np.where(d.BC == 0, H[:, :2 * d.n], G) * d.BV).sum(axis=1)
+    assert np.all(np.abs(d.b - b0) <= 1e-14 * terms)
     Hr, Gr = O.assemble_kelvin(d, 16)
     ur, tr, xr = O.solve_kelvin(d, Hr, Gr)
     u = B.solve(d, mode="fp64")
