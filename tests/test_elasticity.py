@@ -207,7 +207,7 @@ def _exact_pair_kelvin(d, i, e, mu, nu):
 def _collapsed_half(d, i, e):
     """True when the tan-p3c rule of pair (i, e) splits at a tangent parent point
     within 1e-14 of an end: the reference's _tanp3c_half then returns the full
-    tangent rule for the empty half (src/Core/Integration.jl:1345-1348), so the
+    tangent rule for the empty half (src/Core/Integration.jl:1344-1349), so the
     element is integrated twice. This happens for rows on the extension of an
     element that ends at a corner (complex pole within rounding of +-1)."""
     nn = d.p + 1
