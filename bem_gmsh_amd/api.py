@@ -113,8 +113,9 @@ def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = 
         raise ArgumentError(f"near_factor must be a Real, :auto, or nothing; got {near_factor!r}")
     if precision in ("matrix_free", "fmm"):
         fetch = False
-    if isinstance(dad.properties, Elasticity) and precision != "fp64":
-        raise ArgumentError(f"elasticity is assembled in FP64 (precision='fp64'); got {precision!r}")
+    if isinstance(dad.properties, Elasticity) and precision not in ("fp64", "mixed"):
+        raise ArgumentError("elasticity is assembled with stored H, G (precision 'fp64' or "
+                            f"'mixed'); got {precision!r}")
     nf = float(near_factor)
     if nf == float("inf"):
         nf = 1e300

@@ -41,6 +41,24 @@ namespace bemhip {
 namespace {
 
 typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+// The two rows (2i, 2i + 1) of one column: one 16-byte (FP64) or 8-byte (FP32)
+// store; the FP32 storage rounds the FP64 values.
+template <typename TS>
+__device__ __forceinline__ void store_pair_nt(TS* p, double a, double b) {
+  if constexpr (sizeof(TS) == 8)
+    __builtin_nontemporal_store(d2v{a, b}, (d2v*)p);
+  else
+    __builtin_nontemporal_store(f2v{(float)a, (float)b}, (f2v*)p);
+}
+template <typename TS>
+__device__ __forceinline__ void store_pair(TS* p, double a, double b) {
+  if constexpr (sizeof(TS) == 8)
+    *(d2v*)p = d2v{a, b};
+  else
+    *(f2v*)p = f2v{(float)a, (float)b};
+}
 
 // Kelvin constants of fundamental(props::Elasticity, r, n)
 // (src/Elasticity/Fundamental.jl:28-57): prod1 = 4 pi (1 - nu), base =
@@ -83,10 +101,10 @@ __device__ __forceinline__ void kelvin_UT(const Kelvin& K, double rx, double ry,
 // Lane i owns rows 2i, 2i + 1: the store of a column's two rows is 16 bytes,
 // a wave's stores are 1 KB contiguous. hpart: [4][ncb][nt] row sums of H.
 // ---------------------------------------------------------------------------
-template <int NN>
+template <typename TS, int NN>
 __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts cs, Kelvin K,
-                                                                 double* __restrict__ H,
-                                                                 double* __restrict__ G,
+                                                                 TS* __restrict__ H,
+                                                                 TS* __restrict__ G,
                                                                  double* __restrict__ hpart) {
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
       s_jw[kEB * NN], s_L[kEB];
@@ -117,22 +135,24 @@ __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts c
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       const size_t c0 = (size_t)(2 * (col0 + k)) * ld + 2 * (size_t)i;  // column 2j, row 2i
-      d2v h0 = {0.0, 0.0}, h1 = {0.0, 0.0}, g0 = {0.0, 0.0}, g1 = {0.0, 0.0};
+      double h[4] = {0.0, 0.0, 0.0, 0.0}, gv[4] = {0.0, 0.0, 0.0, 0.0};
       if (!nr) {
         const int t = el * NN + k;
         double U[4], T[4];
         kelvin_UT(K, rx[k], ry[k], s_nx[t], s_ny[t], U, T);
         const double w = s_jw[t];
-        h0 = d2v{T[0] * w, T[2] * w};  // column 2j: rows (2i, 2i + 1) = (T11, T21)
-        h1 = d2v{T[1] * w, T[3] * w};  // column 2j + 1: (T12, T22)
-        g0 = d2v{U[0] * w, U[2] * w};
-        g1 = d2v{U[1] * w, U[3] * w};
-        rs[0] += h0.x; rs[2] += h0.y; rs[1] += h1.x; rs[3] += h1.y;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          h[c] = T[c] * w;
+          gv[c] = U[c] * w;
+          rs[c] += h[c];
+        }
       }
-      __builtin_nontemporal_store(h0, (d2v*)(H + c0));
-      __builtin_nontemporal_store(h1, (d2v*)(H + c0 + ld));
-      __builtin_nontemporal_store(g0, (d2v*)(G + c0));
-      __builtin_nontemporal_store(g1, (d2v*)(G + c0 + ld));
+      // column 2j: rows (2i, 2i + 1) = (c 0, c 2); column 2j + 1: (c 1, c 3)
+      store_pair_nt(H + c0, h[0], h[2]);
+      store_pair_nt(H + c0 + ld, h[1], h[3]);
+      store_pair_nt(G + c0, gv[0], gv[2]);
+      store_pair_nt(G + c0 + ld, gv[1], gv[3]);
     }
   }
 #pragma unroll
@@ -427,14 +447,16 @@ __global__ __launch_bounds__(256) void kelvin_near_sing_kernel(
 // D_c = -(far row sums + near values of the other nodes), c = 2a + b; the
 // near blocks of row i into H and G, node i's own H block = D.
 // ---------------------------------------------------------------------------
+template <typename TS>
 __global__ __launch_bounds__(256) void kelvin_finalize(int nt, int nn, int ncb, size_t ld,
                                                        const double* __restrict__ part,
                                                        const int* __restrict__ row_ptr,
                                                        const int* __restrict__ pair_elem,
                                                        const double* __restrict__ ph,
                                                        const double* __restrict__ pg,
-                                                       double* __restrict__ H,
-                                                       double* __restrict__ G) {
+                                                       double* __restrict__ diag,
+                                                       TS* __restrict__ H,
+                                                       TS* __restrict__ G) {
   __shared__ double s_red[4][4][kFinRows];
   const int i = blockIdx.x * kFinRows + (threadIdx.x & (kWave - 1));
   double s[4];
@@ -449,6 +471,9 @@ __global__ __launch_bounds__(256) void kelvin_finalize(int nt, int nn, int ncb, 
         for (int c = 0; c < 4; ++c) s[c] += ph[((size_t)pr * nn + k) * 4 + c];
   }
   const double D[4] = {-s[0], -s[1], -s[2], -s[3]};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) diag[4 * (size_t)i + c] = D[c];
+  if (!H) return;  // operator-only assembly: the FP64 diagonal blocks
   for (int pr = p0; pr < p1; ++pr) {
     const int c0 = pair_elem[pr] * nn;
     for (int k = 0; k < nn; ++k) {
@@ -457,19 +482,128 @@ __global__ __launch_bounds__(256) void kelvin_finalize(int nt, int nn, int ncb, 
       const double* gv = pg + ((size_t)pr * nn + k) * 4;
       const size_t at = (size_t)(2 * j) * ld + 2 * (size_t)i;
       if (j == i) {
-        *(d2v*)(H + at) = d2v{D[0], D[2]};
-        *(d2v*)(H + at + ld) = d2v{D[1], D[3]};
+        store_pair(H + at, D[0], D[2]);
+        store_pair(H + at + ld, D[1], D[3]);
       } else {
-        *(d2v*)(H + at) = d2v{hv[0], hv[2]};
-        *(d2v*)(H + at + ld) = d2v{hv[1], hv[3]};
+        store_pair(H + at, hv[0], hv[2]);
+        store_pair(H + at + ld, hv[1], hv[3]);
       }
-      *(d2v*)(G + at) = d2v{gv[0], gv[2]};
-      *(d2v*)(G + at + ld) = d2v{gv[1], gv[3]};
+      store_pair(G + at, gv[0], gv[2]);
+      store_pair(G + at + ld, gv[1], gv[3]);
     }
   }
   const size_t ad = (size_t)(2 * i) * ld + 2 * (size_t)i;  // internal rows: no own pair
-  *(d2v*)(H + ad) = d2v{D[0], D[2]};
-  *(d2v*)(H + ad + ld) = d2v{D[1], D[3]};
+  store_pair(H + ad, D[0], D[2]);
+  store_pair(H + ad + ld, D[1], D[3]);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 operator pass without stored FP64 matrices (mixed storage):
+// y = H wH + G wG with the far blocks recomputed exactly as kelvin_far_kernel's
+// FP64 values, the near blocks from the pair list and the FP64 diagonal
+// blocks. kelvin_op64_kernel: partial sums of rows 2i, 2i + 1 per 32-element
+// column block, part[a][cb][i]; kelvin_op_finalize adds them in order with the
+// near and diagonal terms.
+// ---------------------------------------------------------------------------
+template <int NN>
+__global__ __launch_bounds__(kFarThreads) void kelvin_op64_kernel(Geo g, Consts cs, Kelvin K,
+                                                                  const double* __restrict__ wH,
+                                                                  const double* __restrict__ wG,
+                                                                  double* __restrict__ part) {
+  __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
+      s_jw[kEB * NN], s_L[kEB];
+  __shared__ double s_w[4][kEB * NN];  // wH, wG of columns 2j and 2j + 1
+  __shared__ float s_lim2f[kEB];
+  const int cb = blockIdx.y;
+  const int ncb = gridDim.y;
+  const int e0 = cb * kEB;
+  const int ecount = min(kEB, g.ne - e0);
+  stage_class_block<NN>(g, cs, e0, ecount, s_cx, s_cy, s_L, s_lim2f);
+  for (int t = threadIdx.x; t < ecount * NN; t += blockDim.x) {
+    const int j = e0 * NN + t;
+    s_nx[t] = g.nx[j]; s_ny[t] = g.ny[j];
+    s_jw[t] = g.jw[j];
+    s_w[0][t] = wH[2 * j];
+    s_w[1][t] = wH[2 * j + 1];
+    s_w[2][t] = wG[2 * j];
+    s_w[3][t] = wG[2 * j + 1];
+  }
+  __syncthreads();
+  const int i = blockIdx.x * kFarThreads + threadIdx.x;
+  if (i >= g.nt) return;
+  const double px = g.rx[i], py = g.ry[i];
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int el = 0; el < ecount; ++el) {
+    double rx[NN], ry[NN];
+    float rxf[NN], ryf[NN], d2f[NN];
+    const int col0 = (e0 + el) * NN;
+    const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
+    if (classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el], s_lim2f[el], cs.nf,
+                          onel, rx, ry, rxf, ryf, d2f))
+      continue;
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      const int t = el * NN + k;
+      double U[4], T[4];
+      kelvin_UT(K, rx[k], ry[k], s_nx[t], s_ny[t], U, T);
+      const double w = s_jw[t];
+      const double h0 = T[0] * w, h1 = T[1] * w, h2 = T[2] * w, h3 = T[3] * w;
+      const double g0 = U[0] * w, g1 = U[1] * w, g2 = U[2] * w, g3 = U[3] * w;
+      acc0 += ((h0 * s_w[0][t] + h1 * s_w[1][t]) + g0 * s_w[2][t]) + g1 * s_w[3][t];
+      acc1 += ((h2 * s_w[0][t] + h3 * s_w[1][t]) + g2 * s_w[2][t]) + g3 * s_w[3][t];
+    }
+  }
+  part[(size_t)cb * g.nt + i] = acc0;
+  part[((size_t)ncb + cb) * g.nt + i] = acc1;
+}
+
+__global__ __launch_bounds__(256) void kelvin_op_finalize(
+    int nt, int n, int nn, int ncb, const double* __restrict__ part,
+    const int* __restrict__ row_ptr, const int* __restrict__ pair_elem,
+    const double* __restrict__ ph, const double* __restrict__ pg, const double* __restrict__ diag,
+    const double* __restrict__ wH, const double* __restrict__ wG, double* __restrict__ y) {
+  __shared__ double s_red[2][4][kFinRows];
+  const int i = blockIdx.x * kFinRows + (threadIdx.x & (kWave - 1));
+  double s0 = sum_partials(part, ncb, nt, i, s_red[0]);
+  double s1 = sum_partials(part + (size_t)ncb * nt, ncb, nt, i, s_red[1]);
+  if (threadIdx.x >= kWave || i >= nt) return;  // wave 0 finishes its 64 rows
+  const double* D = diag + 4 * (size_t)i;
+  for (int pr = row_ptr[i]; pr < row_ptr[i + 1]; ++pr) {
+    const int c0 = pair_elem[pr] * nn;
+    for (int k = 0; k < nn; ++k) {
+      const int j = c0 + k;
+      const double* h = (j == i) ? D : ph + ((size_t)pr * nn + k) * 4;
+      const double* gv = pg + ((size_t)pr * nn + k) * 4;
+      const double a = wH[2 * j], b = wH[2 * j + 1], ga = wG[2 * j], gb = wG[2 * j + 1];
+      s0 += ((h[0] * a + h[1] * b) + gv[0] * ga) + gv[1] * gb;
+      s1 += ((h[2] * a + h[3] * b) + gv[2] * ga) + gv[3] * gb;
+    }
+  }
+  if (i >= n) {  // internal row: its own block is in no pair
+    s0 += D[0] * wH[2 * i] + D[1] * wH[2 * i + 1];
+    s1 += D[2] * wH[2 * i] + D[3] * wH[2 * i + 1];
+  }
+  y[2 * (size_t)i] = s0;
+  y[2 * (size_t)i + 1] = s1;
+}
+
+// Operator weights per dof (boundary dofs j < nb): apply (x): wH = BC 0 ? 0 : x,
+// wG = BC 0 ? -x : 0; right-hand side (bv): wH = BC 0 ? -bv : 0, wG = BC 0 ?
+// 0 : bv. Internal dofs: wH = x (apply) or 0 (right-hand side).
+__global__ void kelvin_weights_kernel(int ns, int nb, int rhs, const int* __restrict__ bc,
+                                      const double* __restrict__ v, double* __restrict__ wH,
+                                      double* __restrict__ wG) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ns) return;
+  if (j >= nb) { wH[j] = rhs ? 0.0 : v[j]; return; }
+  const bool dir = bc[j] == 0;
+  if (rhs) {
+    wH[j] = dir ? -v[j] : 0.0;
+    wG[j] = dir ? 0.0 : v[j];
+  } else {
+    wH[j] = dir ? 0.0 : v[j];
+    wG[j] = dir ? -v[j] : 0.0;
+  }
 }
 
 // y = A x with A's columns read from H or -G: part[chunk][r] over columns
@@ -503,13 +637,14 @@ __global__ __launch_bounds__(256) void kelvin_rhs_part(int ns, int nb, size_t ld
 
 // A = H with the Dirichlet columns (BC 0) replaced by -G; workgroup row y
 // takes columns y, y + gridDim.y, ...
+template <typename TS>
 __global__ void kelvin_swap_kernel(int ns, int nb, size_t ld, const int* __restrict__ bc,
-                                   const double* __restrict__ H, const double* __restrict__ G,
-                                   double* __restrict__ A) {
+                                   const TS* __restrict__ H, const TS* __restrict__ G,
+                                   TS* __restrict__ A) {
   for (int j = blockIdx.y; j < ns; j += gridDim.y) {
     const bool dir = (j < nb) && bc[j] == 0;
-    const double* src = dir ? G + (size_t)j * ld : H + (size_t)j * ld;
-    double* dst = A + (size_t)j * ld;
+    const TS* src = dir ? G + (size_t)j * ld : H + (size_t)j * ld;
+    TS* dst = A + (size_t)j * ld;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < ns; r += gridDim.x * blockDim.x)
       dst[r] = dir ? -src[r] : src[r];
   }
@@ -599,18 +734,30 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   const size_t ns = (size_t)c->nsys(), nb = (size_t)c->nbnd();
   const int ncb = (c->ne + kEB - 1) / kEB;
   const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);
-  c->H32.release(); c->G32.release();  // A64 / A32 stay allocated for the next system
+  const bool fp64 = (c->precision == BEMHIP_FP64);
+  // the other storage's matrices go; A64 / A32 stay allocated for the next system
+  if (fp64) { c->H32.release(); c->G32.release(); }
+  else { c->H64.release(); c->G64.release(); c->A64.release(); }
   BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
+  BEMHIP_HIP_TRY(c, c->diag64.ensure(4 * nt));
   BEMHIP_HIP_TRY(c, c->hpart.ensure(4 * (size_t)ncb * nt));
   BEMHIP_HIP_TRY(c, c->row_cnt.ensure(nt));
   BEMHIP_HIP_TRY(c, c->row_ptr.ensure(nt + 1));
   BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
   BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
-  BEMHIP_HIP_TRY(c, c->H64.ensure(c->ld64 * ns));
-  BEMHIP_HIP_TRY(c, c->G64.ensure(c->ld64 * nb));
-  if (c->ni > 0)  // columns of the internal nodes: zero but the diagonal blocks
-    BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + nb * c->ld64, 0,
-                                     sizeof(double) * c->ld64 * (ns - nb), c->stream));
+  if (fp64) {
+    BEMHIP_HIP_TRY(c, c->H64.ensure(c->ld64 * ns));
+    BEMHIP_HIP_TRY(c, c->G64.ensure(c->ld64 * nb));
+    if (c->ni > 0)  // columns of the internal nodes: zero but the diagonal blocks
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + nb * c->ld64, 0,
+                                       sizeof(double) * c->ld64 * (ns - nb), c->stream));
+  } else {
+    BEMHIP_HIP_TRY(c, c->H32.ensure(c->ld32 * ns));
+    BEMHIP_HIP_TRY(c, c->G32.ensure(c->ld32 * nb));
+    if (c->ni > 0)
+      BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + nb * c->ld32, 0,
+                                       sizeof(float) * c->ld32 * (ns - nb), c->stream));
+  }
   BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
   // events: 0 start, 1 classified, 2 far done, 3 far start, 4 pairs counted,
   // 5 near done, 6 finalized
@@ -673,18 +820,30 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[3], c->stream));
   {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
-    if (c->nn == 2)
-      hipLaunchKernelGGL((kelvin_far_kernel<2>), grid, dim3(kFarThreads), 0, c->stream, g, cs,
-                         K, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+    if (fp64 && c->nn == 2)
+      hipLaunchKernelGGL((kelvin_far_kernel<double, 2>), grid, dim3(kFarThreads), 0, c->stream,
+                         g, cs, K, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+    else if (fp64)
+      hipLaunchKernelGGL((kelvin_far_kernel<double, 3>), grid, dim3(kFarThreads), 0, c->stream,
+                         g, cs, K, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+    else if (c->nn == 2)
+      hipLaunchKernelGGL((kelvin_far_kernel<float, 2>), grid, dim3(kFarThreads), 0, c->stream,
+                         g, cs, K, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
     else
-      hipLaunchKernelGGL((kelvin_far_kernel<3>), grid, dim3(kFarThreads), 0, c->stream, g, cs,
-                         K, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+      hipLaunchKernelGGL((kelvin_far_kernel<float, 3>), grid, dim3(kFarThreads), 0, c->stream,
+                         g, cs, K, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], c->stream));
-  hipLaunchKernelGGL(kelvin_finalize, dim3((unsigned)((nt + kFinRows - 1) / kFinRows)), dim3(256),
-                     0, c->stream, (int)nt, c->nn, ncb, c->ld64, c->hpart.ptr, c->row_ptr.ptr,
-                     c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, c->H64.ptr, c->G64.ptr);
+  const dim3 fgrid((unsigned)((nt + kFinRows - 1) / kFinRows));
+  if (fp64)
+    hipLaunchKernelGGL((kelvin_finalize<double>), fgrid, dim3(256), 0, c->stream, (int)nt, c->nn,
+                       ncb, c->ld64, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_h.ptr, c->pair_g.ptr, c->diag64.ptr, c->H64.ptr, c->G64.ptr);
+  else
+    hipLaunchKernelGGL((kelvin_finalize<float>), fgrid, dim3(256), 0, c->stream, (int)nt, c->nn,
+                       ncb, c->ld32, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_h.ptr, c->pair_g.ptr, c->diag64.ptr, c->H32.ptr, c->G32.ptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[6], c->stream));
   BEMHIP_HIP_TRY(c, hipEventSynchronize(ev[6]));
@@ -702,10 +861,35 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   return BEMHIP_OK;
 }
 
+// One FP64 operator pass y = H wH + G wG (wH over 2nt dofs, wG over 2n) from
+// the geometry, the pair list and the FP64 diagonal blocks.
+static int kelvin_operator_pass(bemhip_ctx* c, const double* wH, const double* wG, double* y) {
+  const Geo g = make_geo(c);
+  const Consts cs = kelvin_consts(c);
+  const Kelvin K = make_kelvin(c);
+  const int nt = c->nt;
+  const int ncb = (c->ne + kEB - 1) / kEB;
+  // part1: bemhip_solve keeps its residual and product vectors in part2
+  BEMHIP_HIP_TRY(c, c->part1.ensure(2 * (size_t)ncb * nt));
+  const dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)ncb);
+  if (c->nn == 2)
+    hipLaunchKernelGGL((kelvin_op64_kernel<2>), grid, dim3(kFarThreads), 0, c->stream, g, cs, K,
+                       wH, wG, c->part1.ptr);
+  else
+    hipLaunchKernelGGL((kelvin_op64_kernel<3>), grid, dim3(kFarThreads), 0, c->stream, g, cs, K,
+                       wH, wG, c->part1.ptr);
+  hipLaunchKernelGGL(kelvin_op_finalize, dim3((unsigned)((nt + kFinRows - 1) / kFinRows)),
+                     dim3(256), 0, c->stream, nt, c->n, c->nn, ncb, c->part1.ptr, c->row_ptr.ptr,
+                     c->pair_elem.ptr, c->pair_h.ptr, c->pair_g.ptr, c->diag64.ptr, wH, wG, y);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  return BEMHIP_OK;
+}
+
 int kelvin_apply_bc_impl(bemhip_ctx* c) {
   const int ns = c->nsys(), nb = c->nbnd();
+  const bool fp64 = (c->precision == BEMHIP_FP64);
   BEMHIP_HIP_TRY(c, c->b64.ensure((size_t)ns));
-  {
+  if (fp64) {
     ScopedTimer t(c, "apply_bc_rhs");
     const int nchunk = (nb + kApplyCols - 1) / kApplyCols;
     BEMHIP_HIP_TRY(c, c->part1.ensure((size_t)nchunk * ns));
@@ -716,20 +900,46 @@ int kelvin_apply_bc_impl(bemhip_ctx* c) {
     hipLaunchKernelGGL(kelvin_apply_sum, dim3(rb), dim3(256), 0, c->stream, ns, nchunk,
                        c->part1.ptr, c->b64.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
+  } else {
+    // FP64 right-hand side from the operator pass (no FP64 H, G stored)
+    ScopedTimer t(c, "apply_bc_rhs");
+    BEMHIP_HIP_TRY(c, c->wH2.ensure((size_t)ns));
+    BEMHIP_HIP_TRY(c, c->wG2.ensure((size_t)nb));
+    hipLaunchKernelGGL(kelvin_weights_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                       c->stream, ns, nb, 1, c->d_bc.ptr, c->d_bv.ptr, c->wH2.ptr, c->wG2.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    const int st = kelvin_operator_pass(c, c->wH2.ptr, c->wG2.ptr, c->b64.ptr);
+    if (st) return st;
   }
   {
     ScopedTimer t(c, "apply_bc_swap");
-    BEMHIP_HIP_TRY(c, c->A64.ensure(c->ld64 * (size_t)ns));
     dim3 grid((unsigned)std::min((ns + 255) / 256, 64), (unsigned)std::min(ns, 32768));
-    hipLaunchKernelGGL(kelvin_swap_kernel, grid, dim3(256), 0, c->stream, ns, nb, c->ld64,
-                       c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A64.ptr);
+    if (fp64) {
+      BEMHIP_HIP_TRY(c, c->A64.ensure(c->ld64 * (size_t)ns));
+      hipLaunchKernelGGL((kelvin_swap_kernel<double>), grid, dim3(256), 0, c->stream, ns, nb,
+                         c->ld64, c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A64.ptr);
+    } else {
+      BEMHIP_HIP_TRY(c, c->A32.ensure(c->ld32 * (size_t)ns));
+      hipLaunchKernelGGL((kelvin_swap_kernel<float>), grid, dim3(256), 0, c->stream, ns, nb,
+                         c->ld32, c->d_bc.ptr, c->H32.ptr, c->G32.ptr, c->A32.ptr);
+    }
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
   return BEMHIP_OK;
 }
 
+// y = A x in FP64: from the stored FP64 H, G (FP64 assembly) or the operator
+// pass (mixed assembly).
 int kelvin_operator_apply(bemhip_ctx* c, const double* d_x, double* d_y) {
   const int ns = c->nsys(), nb = c->nbnd();
+  if (c->precision != BEMHIP_FP64) {
+    BEMHIP_HIP_TRY(c, c->wH.ensure((size_t)ns));
+    BEMHIP_HIP_TRY(c, c->wG.ensure((size_t)nb));
+    hipLaunchKernelGGL(kelvin_weights_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                       c->stream, ns, nb, 0, c->d_bc.ptr, d_x, c->wH.ptr, c->wG.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    return kelvin_operator_pass(c, c->wH.ptr, c->wG.ptr, d_y);
+  }
   const int nchunk = (ns + kApplyCols - 1) / kApplyCols;
   BEMHIP_HIP_TRY(c, c->part1.ensure((size_t)nchunk * ns));
   const unsigned rb = (unsigned)((ns + 255) / 256);

@@ -22,8 +22,9 @@ the timed steps). ms_per_step = wall time of a whole step.
 
 After the timed steps (outside them): 2-D elasticity (Kelvin) on a star of
 N = 32766 dofs (5461 quadratic elements, plane strain E = 1, nu = 0.3, uniform
-strain field, traction on the Neumann arcs): FP64 assembly, mixed and FP64
-solves, the Kelvin far kernel's roofline and an oracle CPU sample; the
+strain field, traction on the Neumann arcs): the mixed pipeline (FP32 storage,
+FP32 LU, FP64 refinement) and the FP64 pipeline, the Kelvin far kernel's
+roofline and an oracle CPU sample; the
 adjoint shape gradient on the
 N = 4095 star (1365 elements, npg = 12, target fluxes from the FP64 solve on
 the shape with a_m shifted by U(-0.01, 0.01)) after an FP64 and after a mixed
@@ -192,11 +193,13 @@ def adjoint_leg(ne=1365, seed=1, npg=12):
 
 
 def elastic_leg(ne=5461, npg=16, seed=1, cpu_rows=1024, threads=1, reps=2):
-    """Kelvin elasticity on the star: assemble (FP64) -> per-dof BCs -> mixed
-    solve (FP32 LU + FP64 refinement with the stored FP64 system) and FP64
-    solve; HIP-event times from the library, accuracy against the exact
-    uniform-strain field. Roofline of kelvin_far_kernel: 64 bytes of H and G
-    written per far (row node, column node) pair."""
+    """Kelvin elasticity on the star, the headline's two pipelines: mixed
+    (FP32 H, G with FP64 near / diagonal values -> per-dof BCs with the FP64
+    right-hand side from the operator pass -> FP32 LU + FP64 refinement with the
+    recomputed FP64 operator) and FP64 (FP64 H, G -> FP64 LU). HIP-event times
+    from the library; accuracy against the exact uniform-strain field.
+    Roofline of kelvin_far_kernel: 2x2 blocks of H and G written per far (row
+    node, column node) pair, 32 B (FP32) or 64 B (FP64)."""
     mesh = B.star_mesh(ne, seed=seed)
     d = B.format2d(mesh, B.Elasticity(1.0, 0.3, 1.0))
     neu = np.nonzero(d.BC.reshape(-1, 2)[:, 0] == 1)[0]
@@ -205,49 +208,60 @@ def elastic_leg(ne=5461, npg=16, seed=1, cpu_rows=1024, threads=1, reps=2):
     nt, n = d.nt, d.n
     ctx = B.HipContext(0)
     d.hip = ctx
+
+    def phases(c):
+        m = c["ms"]
+        return {"assemble_ms": m["assemble"], "far_ms": m["assemble_far"],
+                "classify_ms": m["assemble_classify"], "scan_ms": m["assemble_scan"],
+                "near_chain_ms": m["assemble_near"], "finalize_ms": m["assemble_finalize"],
+                "rhs_ms": m["apply_bc_rhs"], "swap_ms": m["apply_bc_swap"]}
+
     res = []
     for r in range(reps):
-        B.assemble(d, npg, fetch=False)
+        B.assemble(d, npg, precision="mixed", fetch=False)
         B.solve(d, mode="mixed", tol=1e-10)
         c = ctx.counters()
-        res.append({"assemble_ms": c["ms"]["assemble"], "far_ms": c["ms"]["assemble_far"],
-                    "classify_ms": c["ms"]["assemble_classify"],
-                    "scan_ms": c["ms"]["assemble_scan"],
-                    "near_chain_ms": c["ms"]["assemble_near"],
-                    "finalize_ms": c["ms"]["assemble_finalize"],
-                    "rhs_ms": c["ms"]["apply_bc_rhs"], "swap_ms": c["ms"]["apply_bc_swap"],
-                    "lu32_ms": c["ms"]["lu32"],
-                    "refine_ms": c["ms"]["refine"], "refine_iters": d.solve_info["iters"],
-                    "relres": d.solve_info["relres"], "near_pairs": c["stats"]["near_pairs"]})
-        log(f"[bench] elastic rep {r + 1}/{reps}: assemble {res[-1]['assemble_ms']:.2f} ms "
+        res.append(dict(phases(c), lu32_ms=c["ms"]["lu32"], refine_ms=c["ms"]["refine"],
+                        refine_iters=d.solve_info["iters"], relres=d.solve_info["relres"],
+                        near_pairs=c["stats"]["near_pairs"]))
+        log(f"[bench] elastic mixed rep {r + 1}/{reps}: assemble {res[-1]['assemble_ms']:.2f} ms "
             f"(far {res[-1]['far_ms']:.2f}), lu32 {res[-1]['lu32_ms']:.1f} ms, refine "
             f"{res[-1]['refine_ms']:.1f} ms ({res[-1]['refine_iters']} it)")
     last = res[-1]
     xm = d.solve_info["x"].copy()
     err_u_mixed = B.rel_error(d)
     err_t_mixed = B.rel_error_flux(d)
-    B.solve(d, mode="fp64")
+    for r in range(2):  # the first FP64 assembly allocates its matrices
+        B.assemble(d, npg, precision="fp64", fetch=False)
+        B.solve(d, mode="fp64")
     c = ctx.counters()
+    p64 = dict(phases(c), lu64_ms=c["ms"]["lu64"])
     x64 = d.solve_info["x"]
-    far_bytes = 64.0 * nt * n
-    far_s = last["far_ms"] * 1e-3
+
+    def roof(nbytes, ms, name):
+        a = nbytes / (ms * 1e-3)
+        return {"kernel": name, "bound": "hbm", "achieved": a / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": a / PEAK_HBM, "traffic": None, "algorithmic_bytes": nbytes}
+
     out = {
         "workload": f"star N={2 * nt} dofs ({d.ne} quadratic elements, {nt} nodes), npg={npg}, "
                     "Kelvin plane strain E=1 nu=0.3, uniform strain, traction on the "
                     "Neumann arcs",
-        "reps": res, "assemble_ms": last["assemble_ms"],
+        "mixed_reps": res, "fp64": p64,
+        "assemble_ms": last["assemble_ms"],
         "entries_per_s": 8.0 * nt * n / (last["assemble_ms"] * 1e-3),
+        "fp64_assemble_ms": p64["assemble_ms"],
+        "fp64_entries_per_s": 8.0 * nt * n / (p64["assemble_ms"] * 1e-3),
         "lu32_ms": last["lu32_ms"], "refine_ms": last["refine_ms"],
         "solve_mixed_s": (last["lu32_ms"] + last["refine_ms"]) * 1e-3,
-        "lu64_ms": c["ms"]["lu64"], "solve_fp64_s": c["ms"]["lu64"] * 1e-3,
+        "lu64_ms": p64["lu64_ms"], "solve_fp64_s": p64["lu64_ms"] * 1e-3,
         "mixed_vs_fp64": float(np.linalg.norm(xm - x64) / np.linalg.norm(x64)),
         "rel_error_u_mixed": err_u_mixed, "rel_error_traction_mixed": err_t_mixed,
         "rel_error_u_fp64": B.rel_error(d), "rel_error_traction_fp64": B.rel_error_flux(d),
-        "roofline_far": {"kernel": "kelvin_far_kernel<3> (Kelvin H,G far field, FP64)",
-                         "bound": "hbm", "achieved": far_bytes / far_s / 1e9,
-                         "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                         "frac": far_bytes / far_s / PEAK_HBM, "traffic": None,
-                         "algorithmic_bytes": far_bytes},
+        "roofline_far": roof(32.0 * nt * n, last["far_ms"],
+                             "kelvin_far_kernel<float, 3> (Kelvin H,G far field, FP32 storage)"),
+        "roofline_far_fp64": roof(64.0 * nt * n, p64["far_ms"],
+                                  "kelvin_far_kernel<double, 3> (FP64 storage)"),
     }
     ctx.close()
     if cpu_rows:
@@ -261,11 +275,11 @@ def elastic_leg(ne=5461, npg=16, seed=1, cpu_rows=1024, threads=1, reps=2):
                                "cores": threads, "kind": "port",
                                "sample": f"oracle Kelvin assembly of rows 0..{rows} "
                                          f"({ts:.2f} s)"}
-    log(f"[bench] elastic N={2 * nt}: assemble {out['assemble_ms']:.2f} ms "
-        f"({out['entries_per_s']:.3g} entries/s, far kernel "
-        f"{out['roofline_far']['achieved']:.0f} GB/s), solve mixed "
-        f"{out['solve_mixed_s']:.3f} s vs fp64 {out['solve_fp64_s']:.3f} s, "
-        f"u err {out['rel_error_u_mixed']:.2e}, traction err {out['rel_error_traction_mixed']:.2e}")
+    log(f"[bench] elastic N={2 * nt}: mixed assemble {out['assemble_ms']:.2f} ms "
+        f"({out['entries_per_s']:.3g} entries/s, far {out['roofline_far']['achieved']:.0f} GB/s), "
+        f"fp64 assemble {out['fp64_assemble_ms']:.2f} ms; solve mixed {out['solve_mixed_s']:.3f} s "
+        f"vs fp64 {out['solve_fp64_s']:.3f} s, mixed vs fp64 {out['mixed_vs_fp64']:.1e}, "
+        f"u err {out['rel_error_u_mixed']:.2e}")
     return out
 
 

@@ -188,6 +188,38 @@ def test_kelvin_bc_and_solves_match_oracle(case):
     assert np.abs(y - A0 @ z).max() <= 1e-12 * np.abs(A0).sum(axis=1).max()
 
 
+@pytest.mark.parametrize("case", list(CASES))
+def test_kelvin_mixed_storage_operator_and_solve(case):
+    """Mixed assembly: FP32 H, G equal to the FP64 assembly's values rounded to
+    FP32 (same FP64 computation, diagonal from the same FP64 sums); the FP64
+    right-hand side and operator recomputed from the geometry (no FP64 matrix
+    stored) against the stored FP64 system; the mixed solve against FP64."""
+    d = CASES[case]()
+    H64, G64 = B.assemble(d, 16)
+    B.apply_bc(d, fetch=True)
+    A64, b64 = d.A.copy(), d.b.copy()
+    B.solve(d, mode="fp64")
+    x64 = d.solve_info["x"].copy()
+    z = np.random.default_rng(2).standard_normal(2 * d.nt)
+    H32, G32 = B.assemble(d, 16, precision="mixed")
+    assert H32.dtype == np.float32 and G32.dtype == np.float32
+    assert np.array_equal(H32, H64.astype(np.float32))
+    assert np.array_equal(G32, G64.astype(np.float32))
+    B.apply_bc(d)
+    terms = np.abs(np.where(d.BC == 0, H64[:, :2 * d.n], G64) * d.BV).sum(axis=1)
+    assert np.all(np.abs(d.b - b64) <= 1e-13 * terms)
+    y = B.apply_operator(d, z)
+    rowabs = np.abs(A64) @ np.abs(z)
+    assert np.all(np.abs(y - A64 @ z) <= 1e-13 * rowabs)
+    B.solve(d, mode="mixed", tol=1e-12)
+    em = np.linalg.norm(d.solve_info["x"] - x64) / np.linalg.norm(x64)
+    print(f"\n[{case}] mixed storage: solve vs fp64 {em:.2e}, refine {d.solve_info['iters']}, "
+          f"relres {d.solve_info['relres']:.1e}", flush=True)
+    assert em <= 1e-9 and d.solve_info["relres"] <= 1e-12
+    with pytest.raises(Exception):
+        B.solve(d, mode="fp64")        # the FP64 LU needs the FP64 assembly
+
+
 def test_kelvin_near_pairs_match_oracle_pairs():
     d = B.format2d(B.quadrado_elasticity(ndiv=9), B.Elasticity(1.0, 0.3), pontointerno=True)
     B.assemble(d, 16, fetch=False)
@@ -245,7 +277,7 @@ def test_kelvin_argument_and_state_errors():
     assert L.bemhip_set_elasticity(ctx.h, 0.4, 0.5) == 1          # nu must be < 0.5
     assert L.bemhip_set_elasticity(ctx.h, -1.0, 0.3) == 1         # mu must be > 0
     assert L.bemhip_set_elasticity(ctx.h, 0.4, 0.3) == 0
-    assert L.bemhip_assemble(ctx.h, 16, 1.5, 1) == 1              # FP64 only
+    assert L.bemhip_assemble(ctx.h, 16, 1.5, 2) == 1              # stored H, G only
     assert L.bemhip_assemble(ctx.h, 16, 1.5, 3) == 1
     assert L.bemhip_assemble(ctx.h, 16, 1.5, 0) == 0
     BC = np.zeros(2 * d.n, dtype=np.int64)
@@ -270,7 +302,7 @@ def test_kelvin_argument_and_state_errors():
                                      u.ctypes.data_as(dp)) == 3
     ctx.close()
     with pytest.raises(B.ArgumentError):
-        B.assemble(d, 16, precision="mixed")
+        B.assemble(d, 16, precision="matrix_free")
 
 
 def test_laplace_after_elasticity_on_one_context():
