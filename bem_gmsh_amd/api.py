@@ -226,7 +226,9 @@ def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: i
                                  C.byref(it), C.byref(rel)))
     if isinstance(dad.properties, Elasticity):
         u, t = split_sol(dad, x)
-        dad.set_cache(u=u, traction=t, T=u,
+        # internal-point displacements (2 ni), the dad.uint of the reference's
+        # thermoelastic solve (src/Elasticity/Thermoelasticity.jl:222-225)
+        dad.set_cache(u=u, traction=t, T=u, uint=x[2 * dad.n:].copy(),
                       solve_info={"mode": mode, "iters": it.value, "relres": rel.value, "x": x})
         return dad.u
     T, q = split_sol(dad, x)

@@ -93,7 +93,7 @@ class BEMData:
     raising an error when missing, like src/Core/Structures.jl:539-553.
     """
 
-    _CACHE_KEYS = ("H", "G", "A", "B", "b", "T", "q", "u", "traction", "qsi", "w", "near_factor",
+    _CACHE_KEYS = ("H", "G", "A", "B", "b", "T", "q", "u", "traction", "uint", "qsi", "w", "near_factor",
                    "analytical", "cells", "solve_info", "assembly_info", "gmres_stats")
 
     def __init__(self, name: str, p: int, collocation: np.ndarray, normal: np.ndarray,

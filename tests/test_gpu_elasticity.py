@@ -252,6 +252,20 @@ def test_reference_elasticity_patch_gpu():
     assert np.linalg.norm(d.traction - tr) <= 1e-9 * np.linalg.norm(tr)
 
 
+def test_kelvin_internal_point_displacements():
+    # internal nodes of the square (pontointerno): their rows of H, G give u at
+    # the cell centroids; the uniform-strain field is linear, so exact up to
+    # the boundary solution's error
+    d = square_internal()
+    B.assemble(d, 16, fetch=False)
+    B.solve(d)
+    assert d.uint.shape == (2 * d.ni,)
+    exact = ANA.u(d.internalNodes).reshape(-1)
+    err = np.linalg.norm(d.uint - exact) / np.linalg.norm(exact)
+    print(f"\n  internal displacements: rel err {err:.2e}", flush=True)
+    assert err < 1e-2
+
+
 def test_kelvin_patch_on_ellipse_gpu():
     d = B.format2d(B.ellipse_mesh(300), B.Elasticity(1.0, 0.3), pontointerno=False)
     B.apply_analytical_bc(d, ANA, range(0, d.n, 2))
