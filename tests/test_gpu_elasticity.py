@@ -329,3 +329,54 @@ def test_laplace_after_elasticity_on_one_context():
     Hr, Gr = O.assemble(dl, 16)
     assert H.shape == (dl.nt, dl.nt)
     assert np.abs(G - Gr).max() <= 1e-10 * np.abs(Gr).max()
+
+
+def test_kelvin_star_N8190_matches_oracle():
+    """A larger Kelvin case: star of 1365 quadratic elements (4095 nodes, 8190
+    dofs), plane strain. Full FP64 H, G against the oracle: entries above
+    1e-10 within 10x their pair's 1-ulp sensitivity; the FP64 solve against
+    LAPACK on the oracle's applyBC of the same H, G (backward error <= N eps,
+    difference within the conditioning); the mixed pipeline against FP64."""
+    import scipy.linalg as sl
+    d = B.format2d(B.star_mesh(1365, seed=2), B.Elasticity(1.0, 0.3, 1.0))
+    neu = np.nonzero(d.BC.reshape(-1, 2)[:, 0] == 1)[0]
+    B.apply_analytical_bc(d, ANA, neu)
+    nt, n, nn = d.nt, d.n, d.p + 1
+    H, G = B.assemble(d, 16)
+    Hr, Gr = O.assemble_kelvin(d, 16)
+    rowH = np.abs(Hr).max(axis=1, keepdims=True)
+    rowG = np.abs(Gr).max(axis=1, keepdims=True)
+    EH = (np.abs(H - Hr) / np.maximum(np.abs(Hr), 1e-3 * rowH)).reshape(nt, 2, nt, 2)
+    EG = (np.abs(G - Gr) / np.maximum(np.abs(Gr), 1e-3 * rowG)).reshape(nt, 2, n, 2)
+    idx = np.arange(nt)
+    EH[idx, :, idx, :] = 0.0                 # diagonal blocks: whole-row sums
+    eh = EH.max(axis=(1, 3))[:, :n]
+    eg = EG.max(axis=(1, 3))
+    bad = np.argwhere((eh > 1e-10) | (eg > 1e-10))
+    pairs = sorted({(int(i), int(j) // nn) for i, j in bad})
+    worst = 0.0
+    for i, e in pairs:
+        sl_ = slice(e * nn, (e + 1) * nn)
+        err = max(eh[i, sl_].max(), eg[i, sl_].max())
+        kappa = _pair_ulp_sensitivity(d, i, e, 16)
+        worst = max(worst, err / max(kappa, 1e-300))
+        assert err <= 10.0 * kappa, (i, e, err, kappa)
+    print(f"\n[star N={2 * nt}] worst entry {max(eh.max(), eg.max()):.2e}; {len(pairs)} pairs "
+          f"above 1e-10, worst err / ulp sensitivity {worst:.2f}", flush=True)
+    A, b = O.apply_bc_kelvin(d, H, G)
+    lu = sl.lu_factor(A)
+    xr = sl.lu_solve(lu, b)
+    rcond, _ = sl.lapack.dgecon(lu[0], np.abs(A).sum(axis=0).max(), norm="1")
+    B.solve(d, mode="fp64")
+    x64 = d.solve_info["x"].copy()
+    for x in (x64, xr):
+        assert (np.abs(b - A @ x).max() <= 2 * nt * np.finfo(float).eps
+                * np.abs(A).sum(axis=1).max() * np.abs(x).max())
+    eo = np.linalg.norm(x64 - xr) / np.linalg.norm(xr)
+    assert eo <= max(1e-10, 4.0 * np.finfo(float).eps / rcond)
+    B.assemble(d, 16, precision="mixed", fetch=False)
+    B.solve(d, mode="mixed", tol=1e-12)
+    em = np.linalg.norm(d.solve_info["x"] - x64) / np.linalg.norm(x64)
+    print(f"  LAPACK {eo:.2e} (kappa_1 {1 / rcond:.1e}), mixed vs fp64 {em:.2e}, u err "
+          f"{B.rel_error(d):.2e}", flush=True)
+    assert em <= 1e-9
