@@ -113,9 +113,9 @@ def assemble(dad: BEMData, npg: int = 20, method: str = "hip", precision: str = 
         raise ArgumentError(f"near_factor must be a Real, :auto, or nothing; got {near_factor!r}")
     if precision in ("matrix_free", "fmm"):
         fetch = False
-    if isinstance(dad.properties, Elasticity) and precision not in ("fp64", "mixed"):
-        raise ArgumentError("elasticity is assembled with stored H, G (precision 'fp64' or "
-                            f"'mixed'); got {precision!r}")
+    if isinstance(dad.properties, Elasticity) and precision == "fmm":
+        raise ArgumentError("elasticity has no FMM far field (precision 'fp64', 'mixed' or "
+                            "'matrix_free')")
     nf = float(near_factor)
     if nf == float("inf"):
         nf = 1e300
@@ -249,12 +249,11 @@ def solve_Hmat(dad: BEMData, Pl=None, atol: float = 1e-10, rtol: float = 1e-8, i
                             "(diagonal scaling is built in)")
     if dad.hip is None or "assembly_info" not in dad.cache:
         raise RuntimeError("cache.H is not set — assemble/solve first?")
-    if isinstance(dad.properties, Elasticity):
-        raise ArgumentError("solve_Hmat: Laplace problems only (elasticity uses solve)")
     apply_bc(dad)
     ctx = dad.hip
-    itm = int(itmax) if itmax > 0 else max(4 * dad.nt, 200)
-    x = np.empty(dad.nt, dtype=np.float64)
+    ns = dad.dof * dad.nt
+    itm = int(itmax) if itmax > 0 else max(4 * ns, 200)
+    x = np.empty(ns, dtype=np.float64)
     it = C.c_int(0)
     rel = C.c_double(0.0)
     status = lib().bemhip_solve_gmres(ctx.h, float(atol), float(rtol), int(restart), itm,
@@ -262,6 +261,13 @@ def solve_Hmat(dad: BEMData, Pl=None, atol: float = 1e-10, rtol: float = 1e-8, i
     converged = status == 0
     if status not in (0, 5):          # 5: BEMHIP_ERR_NOCONV, x is still the last iterate
         ctx.check(status)
+    stats = {"niter": it.value, "relres": rel.value, "solved": converged}
+    info = {"mode": "gmres", "iters": it.value, "relres": rel.value, "x": x}
+    if isinstance(dad.properties, Elasticity):
+        u, t = split_sol(dad, x)
+        dad.set_cache(u=u, traction=t, T=u, uint=x[2 * dad.n:].copy(), gmres_stats=stats,
+                      solve_info=info)
+        return dad.u
     T, q = split_sol(dad, x)
     dad.set_cache(T=T, q=q, gmres_stats={"niter": it.value, "relres": rel.value,
                                          "solved": converged},

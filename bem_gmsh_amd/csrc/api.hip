@@ -315,9 +315,10 @@ int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "assemble: NULL context");
   if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "assemble: set_geometry first");
   if (npg < 1 || npg > kMaxNpg) return fail(c, BEMHIP_ERR_ARG, "assemble: npg must be in 1..64");
-  if (c->dof == 2 && precision != BEMHIP_FP64 && precision != BEMHIP_MIXED)
+  if (c->dof == 2 && precision == BEMHIP_FMM)
     return fail(c, BEMHIP_ERR_ARG,
-                "assemble: elasticity is assembled with stored H, G (BEMHIP_FP64 or BEMHIP_MIXED)");
+                "assemble: elasticity has no FMM far field (BEMHIP_FP64, BEMHIP_MIXED or "
+                "BEMHIP_MATRIX_FREE)");
   if (!(near_factor >= 0.0)) return fail(c, BEMHIP_ERR_ARG, "assemble: near_factor must be >= 0");
   if (precision != BEMHIP_FP64 && precision != BEMHIP_MIXED && precision != BEMHIP_MATRIX_FREE &&
       precision != BEMHIP_FMM)

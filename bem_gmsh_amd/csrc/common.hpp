@@ -241,6 +241,7 @@ hipError_t ensure_count_word(bemhip_ctx* c);  // pinned host word for the near-p
 int kelvin_assemble_impl(bemhip_ctx* c);
 int kelvin_apply_bc_impl(bemhip_ctx* c);
 int kelvin_operator_apply(bemhip_ctx* c, const double* d_x, double* d_y);
+int kelvin_jacobi_diag(bemhip_ctx* c, double* d);
 int fmm_far(bemhip_ctx* c, hipStream_t s, const double* wH, const double* wG, double* out);
 int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
                   const double* d_w, const QuadConst* d_qc, int npg, double near_factor,

@@ -101,7 +101,8 @@ __device__ __forceinline__ void kelvin_UT(const Kelvin& K, double rx, double ry,
 // Lane i owns rows 2i, 2i + 1: the store of a column's two rows is 16 bytes,
 // a wave's stores are 1 KB contiguous. hpart: [4][ncb][nt] row sums of H.
 // ---------------------------------------------------------------------------
-template <typename TS, int NN>
+// STORE = false: the row sums only (the matrix-free assembly's diagonal).
+template <typename TS, int NN, bool STORE = true>
 __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts cs, Kelvin K,
                                                                  TS* __restrict__ H,
                                                                  TS* __restrict__ G,
@@ -148,11 +149,12 @@ __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts c
           rs[c] += h[c];
         }
       }
-      // column 2j: rows (2i, 2i + 1) = (c 0, c 2); column 2j + 1: (c 1, c 3)
-      store_pair_nt(H + c0, h[0], h[2]);
-      store_pair_nt(H + c0 + ld, h[1], h[3]);
-      store_pair_nt(G + c0, gv[0], gv[2]);
-      store_pair_nt(G + c0 + ld, gv[1], gv[3]);
+      if (STORE) {  // column 2j: rows (2i, 2i + 1) = (c 0, c 2); column 2j + 1: (c 1, c 3)
+        store_pair_nt(H + c0, h[0], h[2]);
+        store_pair_nt(H + c0 + ld, h[1], h[3]);
+        store_pair_nt(G + c0, gv[0], gv[2]);
+        store_pair_nt(G + c0 + ld, gv[1], gv[3]);
+      }
     }
   }
 #pragma unroll
@@ -678,6 +680,27 @@ __global__ __launch_bounds__(256) void kelvin_apply_sum(int ns, int nchunk,
   y[r] = s;
 }
 
+// Diagonal of A per dof r = 2i + a: H's own-block entry D_aa for traction-known
+// and internal dofs, -G_rr (node i's own entry of its on-element pair) for
+// displacement-known (BC 0) dofs; 1 where that is zero or not finite.
+__global__ void kelvin_jacobi_kernel(int nt, int n, int nn, const int* __restrict__ bc,
+                                     const double* __restrict__ diag,
+                                     const int* __restrict__ row_ptr,
+                                     const int* __restrict__ pair_elem,
+                                     const double* __restrict__ pg, double* __restrict__ d) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= 2 * nt) return;
+  const int i = r >> 1, a = r & 1;
+  double v = diag[4 * (size_t)i + 3 * a];
+  if (i < n && bc[r] == 0) {
+    const int own = i / nn;
+    v = 0.0;
+    for (int pr = row_ptr[i]; pr < row_ptr[i + 1]; ++pr)
+      if (pair_elem[pr] == own) v = -pg[((size_t)pr * nn + (i - own * nn)) * 4 + 3 * a];
+  }
+  d[r] = (v != 0.0 && isfinite(v)) ? v : 1.0;
+}
+
 template <int NN>
 int launch_kelvin_pairs(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Kelvin& K,
                         int64_t nsing, const int* sing, int64_t nreg, const int* reg) {
@@ -735,9 +758,13 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   const int ncb = (c->ne + kEB - 1) / kEB;
   const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);
   const bool fp64 = (c->precision == BEMHIP_FP64);
+  const bool mfree = (c->precision == BEMHIP_MATRIX_FREE);
   // the other storage's matrices go; A64 / A32 stay allocated for the next system
   if (fp64) { c->H32.release(); c->G32.release(); }
-  else { c->H64.release(); c->G64.release(); c->A64.release(); }
+  else if (mfree) {
+    c->H64.release(); c->G64.release(); c->A64.release();
+    c->H32.release(); c->G32.release(); c->A32.release();
+  } else { c->H64.release(); c->G64.release(); c->A64.release(); }
   BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
   BEMHIP_HIP_TRY(c, c->diag64.ensure(4 * nt));
   BEMHIP_HIP_TRY(c, c->hpart.ensure(4 * (size_t)ncb * nt));
@@ -745,7 +772,9 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, c->row_ptr.ensure(nt + 1));
   BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
   BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
-  if (fp64) {
+  if (mfree) {
+    // no stored H, G: the far pass below computes the row sums only
+  } else if (fp64) {
     BEMHIP_HIP_TRY(c, c->H64.ensure(c->ld64 * ns));
     BEMHIP_HIP_TRY(c, c->G64.ensure(c->ld64 * nb));
     if (c->ni > 0)  // columns of the internal nodes: zero but the diagonal blocks
@@ -820,7 +849,13 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[3], c->stream));
   {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
-    if (fp64 && c->nn == 2)
+    if (mfree && c->nn == 2)
+      hipLaunchKernelGGL((kelvin_far_kernel<double, 2, false>), grid, dim3(kFarThreads), 0,
+                         c->stream, g, cs, K, nullptr, nullptr, c->hpart.ptr);
+    else if (mfree)
+      hipLaunchKernelGGL((kelvin_far_kernel<double, 3, false>), grid, dim3(kFarThreads), 0,
+                         c->stream, g, cs, K, nullptr, nullptr, c->hpart.ptr);
+    else if (fp64 && c->nn == 2)
       hipLaunchKernelGGL((kelvin_far_kernel<double, 2>), grid, dim3(kFarThreads), 0, c->stream,
                          g, cs, K, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
     else if (fp64)
@@ -836,7 +871,12 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], c->stream));
   const dim3 fgrid((unsigned)((nt + kFinRows - 1) / kFinRows));
-  if (fp64)
+  if (mfree)  // the FP64 diagonal blocks only
+    hipLaunchKernelGGL((kelvin_finalize<double>), fgrid, dim3(256), 0, c->stream, (int)nt, c->nn,
+                       ncb, (size_t)0, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_h.ptr, c->pair_g.ptr, c->diag64.ptr, (double*)nullptr,
+                       (double*)nullptr);
+  else if (fp64)
     hipLaunchKernelGGL((kelvin_finalize<double>), fgrid, dim3(256), 0, c->stream, (int)nt, c->nn,
                        ncb, c->ld64, c->hpart.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                        c->pair_h.ptr, c->pair_g.ptr, c->diag64.ptr, c->H64.ptr, c->G64.ptr);
@@ -911,6 +951,7 @@ int kelvin_apply_bc_impl(bemhip_ctx* c) {
     const int st = kelvin_operator_pass(c, c->wH2.ptr, c->wG2.ptr, c->b64.ptr);
     if (st) return st;
   }
+  if (c->precision == BEMHIP_MATRIX_FREE) return BEMHIP_OK;  // A is applied, never stored
   {
     ScopedTimer t(c, "apply_bc_swap");
     dim3 grid((unsigned)std::min((ns + 255) / 256, 64), (unsigned)std::min(ns, 32768));
@@ -928,8 +969,18 @@ int kelvin_apply_bc_impl(bemhip_ctx* c) {
   return BEMHIP_OK;
 }
 
+// Jacobi diagonal of the current system (2nt values) for GMRES.
+int kelvin_jacobi_diag(bemhip_ctx* c, double* d) {
+  const int ns = c->nsys();
+  hipLaunchKernelGGL(kelvin_jacobi_kernel, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0,
+                     c->stream, c->nt, c->n, c->nn, c->d_bc.ptr, c->diag64.ptr, c->row_ptr.ptr,
+                     c->pair_elem.ptr, c->pair_g.ptr, d);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  return BEMHIP_OK;
+}
+
 // y = A x in FP64: from the stored FP64 H, G (FP64 assembly) or the operator
-// pass (mixed assembly).
+// pass (mixed and matrix-free assemblies).
 int kelvin_operator_apply(bemhip_ctx* c, const double* d_x, double* d_y) {
   const int ns = c->nsys(), nb = c->nbnd();
   if (c->precision != BEMHIP_FP64) {

@@ -157,14 +157,12 @@ extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int r
                                   double* x, int* iters, double* relres) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "solve_gmres: NULL context");
   if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "solve_gmres: apply_bc first");
-  if (c->dof != 1)
-    return fail(c, BEMHIP_ERR_STATE, "solve_gmres: Laplace problems only (use bemhip_solve)");
   if (!(atol >= 0.0) || !(rtol >= 0.0) || (atol == 0.0 && rtol == 0.0))
     return fail(c, BEMHIP_ERR_ARG, "solve_gmres: need atol, rtol >= 0, not both zero");
   if (restart < 1 || restart > 512 || itmax < 1)
     return fail(c, BEMHIP_ERR_ARG, "solve_gmres: restart must be in 1..512 and itmax >= 1");
   hipSetDevice(c->device);
-  const int n = c->nt;
+  const int n = c->nsys();  // unknowns: dof * collocation points
   const int m = restart;
   const size_t ldv = (size_t)n;
   DevBuf<double> V, work, small;
@@ -188,9 +186,14 @@ extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int r
   } ev{t0, t1};
   hipEventRecord(t0, c->stream);
 
-  hipLaunchKernelGGL(jacobi_diag_kernel, dim3(nb), dim3(tb), 0, c->stream, n, c->n, c->nn,
-                     c->d_bc.ptr, c->diag64.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
-                     c->pair_g.ptr, dd);
+  if (c->dof == 2) {
+    int sj = kelvin_jacobi_diag(c, dd);
+    if (sj) return sj;
+  } else {
+    hipLaunchKernelGGL(jacobi_diag_kernel, dim3(nb), dim3(tb), 0, c->stream, n, c->n, c->nn,
+                       c->d_bc.ptr, c->diag64.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_g.ptr, dd);
+  }
   BEMHIP_HIP_TRY(c, hipMemsetAsync(dx, 0, sizeof(double) * n, c->stream));
   int st = 0;
   double bnorm = 0.0;

@@ -220,6 +220,41 @@ def test_kelvin_mixed_storage_operator_and_solve(case):
         B.solve(d, mode="fp64")        # the FP64 LU needs the FP64 assembly
 
 
+@pytest.mark.parametrize("case", list(CASES))
+def test_kelvin_matrix_free_gmres(case):
+    """Matrix-free assembly (near pairs and FP64 diagonal blocks only): the
+    right-hand side and A x from the operator pass against the stored FP64
+    system, H, G and the LU refused, and GMRES (right Jacobi) against the FP64
+    LU solution; GMRES also after the FP64 assembly."""
+    d = CASES[case]()
+    H64, G64 = B.assemble(d, 16)
+    B.apply_bc(d, fetch=True)
+    A64, b64 = d.A.copy(), d.b.copy()
+    B.solve(d, mode="fp64")
+    x64 = d.solve_info["x"].copy()
+    B.solve_Hmat(d, atol=0.0, rtol=1e-12, restart=100)
+    assert d.gmres_stats["solved"]
+    e_stored = np.linalg.norm(d.solve_info["x"] - x64) / np.linalg.norm(x64)
+    B.assemble(d, 16, precision="matrix_free")
+    B.apply_bc(d)
+    terms = np.abs(np.where(d.BC == 0, H64[:, :2 * d.n], G64) * d.BV).sum(axis=1)
+    assert np.all(np.abs(d.b - b64) <= 1e-13 * terms)
+    z = np.random.default_rng(3).standard_normal(2 * d.nt)
+    y = B.apply_operator(d, z)
+    assert np.all(np.abs(y - A64 @ z) <= 1e-13 * (np.abs(A64) @ np.abs(z)))
+    with pytest.raises(Exception):
+        B.get_HG(d)
+    B.solve(d)                         # matrix-free: solve dispatches to solve_Hmat
+    e_mf = np.linalg.norm(d.solve_info["x"] - x64) / np.linalg.norm(x64)
+    print(f"\n[{case}] GMRES after FP64 assembly {e_stored:.2e}; matrix-free "
+          f"{d.gmres_stats['niter']} products, vs FP64 LU {e_mf:.2e}", flush=True)
+    assert d.gmres_stats["solved"]
+    assert e_stored <= 1e-9 and e_mf <= 1e-6
+    B.solve_Hmat(d, atol=0.0, rtol=1e-12, restart=100)
+    e_mf12 = np.linalg.norm(d.solve_info["x"] - x64) / np.linalg.norm(x64)
+    assert e_mf12 <= 1e-9 and d.uint.shape == (2 * d.ni,)
+
+
 def test_kelvin_near_pairs_match_oracle_pairs():
     d = B.format2d(B.quadrado_elasticity(ndiv=9), B.Elasticity(1.0, 0.3), pontointerno=True)
     B.assemble(d, 16, fetch=False)
@@ -291,8 +326,7 @@ def test_kelvin_argument_and_state_errors():
     assert L.bemhip_set_elasticity(ctx.h, 0.4, 0.5) == 1          # nu must be < 0.5
     assert L.bemhip_set_elasticity(ctx.h, -1.0, 0.3) == 1         # mu must be > 0
     assert L.bemhip_set_elasticity(ctx.h, 0.4, 0.3) == 0
-    assert L.bemhip_assemble(ctx.h, 16, 1.5, 2) == 1              # stored H, G only
-    assert L.bemhip_assemble(ctx.h, 16, 1.5, 3) == 1
+    assert L.bemhip_assemble(ctx.h, 16, 1.5, 3) == 1              # no FMM far field
     assert L.bemhip_assemble(ctx.h, 16, 1.5, 0) == 0
     BC = np.zeros(2 * d.n, dtype=np.int64)
     BV = np.zeros(2 * d.n)
@@ -302,11 +336,6 @@ def test_kelvin_argument_and_state_errors():
     BC[1] = 0
     assert L.bemhip_apply_bc(ctx.h, BC.ctypes.data_as(C.POINTER(C.c_int64)),
                              BV.ctypes.data_as(C.POINTER(C.c_double)), None, None) == 0
-    x = np.empty(2 * d.nt)
-    it, rel = C.c_int(0), C.c_double(0.0)
-    assert L.bemhip_solve_gmres(ctx.h, 0.0, 1e-8, 30, 100,
-                                x.ctypes.data_as(C.POINTER(C.c_double)), C.byref(it),
-                                C.byref(rel)) == 3
     pts = np.array([[0.5, 0.5]])
     u = np.empty(1)
     z = np.zeros(d.n)
@@ -316,7 +345,7 @@ def test_kelvin_argument_and_state_errors():
                                      u.ctypes.data_as(dp)) == 3
     ctx.close()
     with pytest.raises(B.ArgumentError):
-        B.assemble(d, 16, precision="matrix_free")
+        B.assemble(d, 16, precision="fmm")
 
 
 def test_laplace_after_elasticity_on_one_context():
