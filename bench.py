@@ -88,7 +88,7 @@ class Ctx:
         a = self._keep
         self.chk(self.L.bemhip_set_geometry(self.h, dad.p, dad.ne, dad.n, dad.ni, dptr(a[0]),
                                             dptr(a[1]), dptr(a[2]), dptr(a[3]), dptr(a[4]),
-                                            dptr(a[5]), float(dad.properties.k)))
+                                            dptr(a[5]), float(getattr(dad.properties, "k", 1.0))))
 
     def pipeline(self, dad, npg, precision, mode, tol=1e-10, maxit=30):
         """assemble -> applyBC -> solve; the mixed solve refines until
@@ -323,34 +323,51 @@ def gmres_large(spec, npg, far="dense", rtol=1e-10, restart=100, itmax=4000):
     elif kind == "star":
         dad = build_problem(int(size))
         workload = f"star N={dad.n} ({dad.ne} quadratic elements), npg={npg}"
+    elif kind == "esquare":
+        if far == "fmm":
+            raise SystemExit("--gmres-large esquare: elasticity has no FMM far field")
+        dad = B.format2d(B.quadrado_elasticity(ndiv=int(size), cells=False),
+                         B.Elasticity(1.0, 0.3, 1.0), pontointerno=False)
+        B.apply_bc_from_physical(dad, B.ana_elasticity_patch(1.0, 0.3, 0.01, -0.004, 0.003))
+        workload = (f"elastic square N={2 * dad.n} dofs ({dad.ne} linear elements), uniform "
+                    f"strain, left edge clamped, traction elsewhere, npg={npg}")
     else:
         raise SystemExit(f"--gmres-large: unknown geometry {kind!r}")
+    elastic = isinstance(dad.properties, B.Elasticity)
+    ns = (2 if elastic else 1) * dad.nt
     ctx = Ctx()
     ctx.set_geometry(dad)
+    if elastic:
+        ctx.chk(ctx.L.bemhip_set_elasticity(ctx.h, float(dad.properties.mu),
+                                            float(dad.properties.effective_nu())))
     BC = np.ascontiguousarray(dad.BC, dtype=np.int64)
     BV = np.ascontiguousarray(dad.BV, dtype=np.float64)
     t = time.perf_counter()
     ctx.chk(ctx.L.bemhip_assemble(ctx.h, npg, 1.5, 3 if far == "fmm" else 2))
     ctx.chk(ctx.L.bemhip_apply_bc(ctx.h, i64ptr(BC), dptr(BV), None, None))
     ms = ctx.counters()["ms"]
-    log(f"[bench] matrix-free ({far} far field) N={dad.nt}: assembly {ms['assemble']:.1f} ms, "
+    log(f"[bench] matrix-free ({far} far field) N={ns}: assembly {ms['assemble']:.1f} ms, "
         f"rhs {ms['apply_bc_rhs']:.1f} ms")
-    x = np.empty(dad.nt)
+    x = np.empty(ns)
     it = C.c_int(0)
     rel = C.c_double(0)
     status = ctx.L.bemhip_solve_gmres(ctx.h, 0.0, rtol, restart, itmax, dptr(x), C.byref(it),
                                       C.byref(rel))
     wall = time.perf_counter() - t
     c = ctx.counters()
-    T, q = B.split_sol(dad, x)
-    dad.set_cache(T=T, q=q)
+    if elastic:
+        u, tr = B.split_sol(dad, x)
+        dad.set_cache(u=u, traction=tr, T=u)
+    else:
+        T, q = B.split_sol(dad, x)
+        dad.set_cache(T=T, q=q)
     out = {"gmres_large": {"workload": workload, "far_field": far, "converged": status == 0,
                            "assemble_ms": c["ms"]["assemble"], "rhs_ms": c["ms"]["apply_bc_rhs"],
                            "gmres_ms": c["ms"]["gmres"], "products": it.value,
                            "relres": rel.value,
                            "ms_per_product": c["ms"]["gmres"] / max(it.value, 1), "wall_s": wall,
                            "rel_error": B.rel_error(dad), "restart": restart, "rtol": rtol,
-                           "dense_fp32_HG_GB": 2 * 4.0 * dad.nt * dad.n / 1e9}}
+                           "dense_fp32_HG_GB": 2 * 4.0 * ns * (ns // dad.nt) * dad.n / 1e9}}
     log(f"[bench] gmres: {it.value} products, {c['ms']['gmres']:.0f} ms, relres {rel.value:.1e}, "
         f"rel_error {out['gmres_large']['rel_error']:.2e}")
     ctx.close()
@@ -394,7 +411,8 @@ def main():
                     help="only the elasticity leg (for profiling)")
     ap.add_argument("--gmres-large", default="", metavar="SPEC",
                     help="only: matrix-free assembly + GMRES; SPEC square:NDIV (linear elements, "
-                         "T = x) or star:NE (quadratic elements)")
+                         "T = x), star:NE (quadratic elements) or esquare:NDIV (elasticity, "
+                         "uniform strain)")
     ap.add_argument("--far", choices=["dense", "fmm"], default="dense",
                     help="far field of the --gmres-large products")
     ap.add_argument("--phase", choices=["all", "assemble"], default="all",

@@ -112,8 +112,10 @@ int bemhip_set_geometry(bemhip_ctx* ctx, int p, int ne, int n, int ni,
  * recomputed from the geometry), bemhip_apply_bc the
  * per-dof applyBC (src/Core/Boundary_conditions.jl:153-175) and bemhip_solve
  * the solve of src/Core/Solver.jl:115-135 (split_sol!, :137-146, is left to
- * the caller). GMRES, the adjoint and interior evaluation are Laplace-only
- * (BEMHIP_ERR_STATE). Invalidates any assembled matrices and factors. */
+ * the caller). BEMHIP_MATRIX_FREE assembly and bemhip_solve_gmres work as for
+ * Laplace; BEMHIP_FMM is Laplace-only (BEMHIP_ERR_ARG); the adjoint and
+ * interior evaluation return BEMHIP_ERR_STATE. Invalidates any assembled
+ * matrices and factors. */
 int bemhip_set_elasticity(bemhip_ctx* ctx, double mu, double nu);
 
 /* Assemble H (nt x nt) and G (nt x n) on the device. Replaces
