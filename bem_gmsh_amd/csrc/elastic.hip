@@ -65,6 +65,7 @@ __device__ __forceinline__ void store_pair(TS* p, double a, double b) {
 // 2 prod1 mu, c34 = 3 - 4 nu, c12 = 1 - 2 nu.
 struct Kelvin {
   double prod1, base, c34, c12;
+  double inv_base, inv_prod1;  // 1 / base, 1 / prod1 (operator passes)
 };
 
 Kelvin make_kelvin(const bemhip_ctx* c) {
@@ -73,6 +74,8 @@ Kelvin make_kelvin(const bemhip_ctx* c) {
   K.base = (2.0 * K.prod1) * c->mu;
   K.c34 = 3.0 - 4.0 * c->nu;
   K.c12 = 1.0 - 2.0 * c->nu;
+  K.inv_base = 1.0 / K.base;
+  K.inv_prod1 = 1.0 / K.prod1;
   return K;
 }
 
@@ -90,6 +93,27 @@ __device__ __forceinline__ void kelvin_UT(const Kelvin& K, double rx, double ry,
   U[2] = U[1];
   U[3] = (prod2 + dr2 * dr2) / K.base;
   const double fat = 1.0 / (K.prod1 * R);
+  T[0] = ((-drdn) * (K.c12 + 2.0 * (dr1 * dr1))) * fat;
+  T[3] = ((-drdn) * (K.c12 + 2.0 * (dr2 * dr2))) * fat;
+  T[1] = (-(((drdn * 2.0) * dr1) * dr2 - K.c12 * (dr1 * ny - dr2 * nx))) * fat;
+  T[2] = (-(((drdn * 2.0) * dr1) * dr2 - K.c12 * (dr2 * nx - dr1 * ny))) * fat;
+}
+
+// The same U, T with one division (1 / R) and the constant reciprocals: for the
+// FP64 operator passes, whose sums are checked to a relative 1e-13, not the
+// stored entries (a few ulps from kelvin_UT's values).
+__device__ __forceinline__ void kelvin_UT_op(const Kelvin& K, double rx, double ry, double nx,
+                                             double ny, double* U, double* T) {
+  const double R = sqrt(rx * rx + ry * ry);
+  const double rinv = 1.0 / R;
+  const double dr1 = rx * rinv, dr2 = ry * rinv;
+  const double drdn = dr1 * nx + dr2 * ny;
+  const double prod2 = -K.c34 * log(R);
+  U[0] = (prod2 + dr1 * dr1) * K.inv_base;
+  U[1] = (dr1 * dr2) * K.inv_base;
+  U[2] = U[1];
+  U[3] = (prod2 + dr2 * dr2) * K.inv_base;
+  const double fat = K.inv_prod1 * rinv;
   T[0] = ((-drdn) * (K.c12 + 2.0 * (dr1 * dr1))) * fat;
   T[3] = ((-drdn) * (K.c12 + 2.0 * (dr2 * dr2))) * fat;
   T[1] = (-(((drdn * 2.0) * dr1) * dr2 - K.c12 * (dr1 * ny - dr2 * nx))) * fat;
@@ -501,8 +525,7 @@ __global__ __launch_bounds__(256) void kelvin_finalize(int nt, int nn, int ncb, 
 
 // ---------------------------------------------------------------------------
 // FP64 operator pass without stored FP64 matrices (mixed storage):
-// y = H wH + G wG with the far blocks recomputed exactly as kelvin_far_kernel's
-// FP64 values, the near blocks from the pair list and the FP64 diagonal
+// y = H wH + G wG with the far blocks recomputed in FP64 (kelvin_UT_op), the near blocks from the pair list and the FP64 diagonal
 // blocks. kelvin_op64_kernel: partial sums of rows 2i, 2i + 1 per 32-element
 // column block, part[a][cb][i]; kelvin_op_finalize adds them in order with the
 // near and diagonal terms.
@@ -547,7 +570,7 @@ __global__ __launch_bounds__(kFarThreads) void kelvin_op64_kernel(Geo g, Consts 
     for (int k = 0; k < NN; ++k) {
       const int t = el * NN + k;
       double U[4], T[4];
-      kelvin_UT(K, rx[k], ry[k], s_nx[t], s_ny[t], U, T);
+      kelvin_UT_op(K, rx[k], ry[k], s_nx[t], s_ny[t], U, T);
       const double w = s_jw[t];
       const double h0 = T[0] * w, h1 = T[1] * w, h2 = T[2] * w, h3 = T[3] * w;
       const double g0 = U[0] * w, g1 = U[1] * w, g2 = U[2] * w, g3 = U[3] * w;
