@@ -9,10 +9,13 @@
 // Kernels, in launch order on the library stream:
 //   near_classify / near_scan / near_fill   the Laplace assembly's near-pair
 //                   CSR (the classification depends on the geometry only).
-//   kelvin_far_kernel<NN>   rows x 32-element column blocks: the pair test of
-//                   near_classify and the far nodal lumping of _far_nodal_vec!
-//                   (:812-822) as 2x2 blocks T * w, U * w; near blocks are
-//                   written 0; FP64 row sums of H per (row, column) component.
+//   kelvin_far_kernel<TS, NN, STORE>   rows x 32-element column blocks: the
+//                   pair test of near_classify and the far nodal lumping of
+//                   _far_nodal_vec! (:812-822) as 2x2 blocks T * w, U * w; near
+//                   blocks are written 0; FP64 row sums of H per (row, column)
+//                   component. FP64 storage keeps the reference's operation
+//                   order; FP32 storage and the matrix-free row sums use one
+//                   division per pair.
 //   kelvin_near_reg_kernel<NN, G> / kelvin_near_sing_kernel<NN>   one lane
 //                   group per near pair: the tan-p3c rule (vectorial
 //                   _accumulate_kernels!, :697-713) and the interpolated
@@ -164,7 +167,12 @@ __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts c
       if (!nr) {
         const int t = el * NN + k;
         double U[4], T[4];
-        kelvin_UT(K, rx[k], ry[k], s_nx[t], s_ny[t], U, T);
+        // FP64 storage: the reference's operation order; FP32 storage and the
+        // row sums alone: one division per pair (entries within FP64 ulps)
+        if constexpr (sizeof(TS) == 8 && STORE)
+          kelvin_UT(K, rx[k], ry[k], s_nx[t], s_ny[t], U, T);
+        else
+          kelvin_UT_op(K, rx[k], ry[k], s_nx[t], s_ny[t], U, T);
         const double w = s_jw[t];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {

@@ -108,7 +108,7 @@ int bemhip_set_geometry(bemhip_ctx* ctx, int p, int ne, int n, int ni,
  * are 2nt x 2nt, G is 2nt x 2n, BC and BV have 2n entries (per dof), x has
  * 2nt. bemhip_assemble then builds H_G_full_direct for a Vectorial problem
  * (src/Core/Assembly_full.jl:120-175; BEMHIP_FP64, or BEMHIP_MIXED: FP32
- * storage of the FP64 values, FP64 right-hand side and refinement operator
+ * storage of FP64 values, FP64 right-hand side and refinement operator
  * recomputed from the geometry), bemhip_apply_bc the
  * per-dof applyBC (src/Core/Boundary_conditions.jl:153-175) and bemhip_solve
  * the solve of src/Core/Solver.jl:115-135 (split_sol!, :137-146, is left to
