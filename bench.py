@@ -54,6 +54,7 @@ PEAK_HBM = 8.0e12          # B/s
 PEAK_MFMA_F32 = 157.3e12   # FLOP/s dense
 PEAK_MFMA_F64 = 78.6e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_star_N32769.json")
+KELVIN_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_kelvin_star_N32766.json")
 
 
 def log(msg):
@@ -144,12 +145,12 @@ def cpu_baseline(dad, npg, rows, lu_n, threads):
     }
 
 
-def load_traffic(nt):
+def load_traffic(nt, path=TRAFFIC_FILE):
     """HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE + WRITE_SIZE,
     scaled by a plain-copy calibration), from the committed summary of the
     same workload; None when absent or for another size."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
@@ -238,10 +239,13 @@ def elastic_leg(ne=5461, npg=16, seed=1, cpu_rows=1024, threads=1, reps=2):
     p64 = dict(phases(c), lu64_ms=c["ms"]["lu64"])
     x64 = d.solve_info["x"]
 
-    def roof(nbytes, ms, name):
+    traffic = load_traffic(nt, KELVIN_TRAFFIC_FILE) or {}
+
+    def roof(nbytes, ms, name, key):
         a = nbytes / (ms * 1e-3)
         return {"kernel": name, "bound": "hbm", "achieved": a / 1e9, "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": a / PEAK_HBM, "traffic": None, "algorithmic_bytes": nbytes}
+                "unit": "GB/s", "frac": a / PEAK_HBM, "traffic": traffic.get(key),
+                "algorithmic_bytes": nbytes}
 
     out = {
         "workload": f"star N={2 * nt} dofs ({d.ne} quadratic elements, {nt} nodes), npg={npg}, "
@@ -259,9 +263,11 @@ def elastic_leg(ne=5461, npg=16, seed=1, cpu_rows=1024, threads=1, reps=2):
         "rel_error_u_mixed": err_u_mixed, "rel_error_traction_mixed": err_t_mixed,
         "rel_error_u_fp64": B.rel_error(d), "rel_error_traction_fp64": B.rel_error_flux(d),
         "roofline_far": roof(32.0 * nt * n, last["far_ms"],
-                             "kelvin_far_kernel<float, 3> (Kelvin H,G far field, FP32 storage)"),
+                             "kelvin_far_kernel<float, 3> (Kelvin H,G far field, FP32 storage)",
+                             "kelvin_far_kernel<float, 3, true>"),
         "roofline_far_fp64": roof(64.0 * nt * n, p64["far_ms"],
-                                  "kelvin_far_kernel<double, 3> (FP64 storage)"),
+                                  "kelvin_far_kernel<double, 3> (FP64 storage)",
+                                  "kelvin_far_kernel<double, 3, true>"),
     }
     ctx.close()
     if cpu_rows:
