@@ -790,8 +790,9 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);
   const bool fp64 = (c->precision == BEMHIP_FP64);
   const bool mfree = (c->precision == BEMHIP_MATRIX_FREE);
-  // the other storage's matrices go; A64 / A32 stay allocated for the next system
-  if (fp64) { c->H32.release(); c->G32.release(); }
+  // the other storage's matrices go; the current storage's A stays allocated
+  // for the next system of the same size
+  if (fp64) { c->H32.release(); c->G32.release(); c->A32.release(); }
   else if (mfree) {
     c->H64.release(); c->G64.release(); c->A64.release();
     c->H32.release(); c->G32.release(); c->A32.release();
