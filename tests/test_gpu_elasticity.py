@@ -102,8 +102,20 @@ def _block_err(A, Ref, nt, ncol):
 @pytest.mark.parametrize("case", list(CASES))
 def test_kelvin_assembly_matches_oracle(case):
     d = CASES[case]()
-    npg = 16
     print(f"\n[{case}] nt={d.nt} n={d.n} p={d.p}", flush=True)
+    _check_kelvin_assembly(d, 16)
+
+
+@pytest.mark.parametrize("npg", [5, 24, 40])
+def test_kelvin_assembly_matches_oracle_other_npg(npg):
+    """The Kelvin near rule at the other lane groups of its pair kernel: npg = 5,
+    24 (32 lanes per pair) and 40 (64 lanes), on the star. Same bars."""
+    d = star_p2()
+    print(f"\n[star_p2 npg={npg}] nt={d.nt} n={d.n}", flush=True)
+    _check_kelvin_assembly(d, npg)
+
+
+def _check_kelvin_assembly(d, npg):
     H, G = B.assemble(d, npg)
     Hr, Gr = O.assemble_kelvin(d, npg)
     near = _near_mask(d, npg)

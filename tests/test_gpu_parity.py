@@ -61,6 +61,26 @@ def test_fp64_assembly_matches_oracle(case):
     assert np.all(ddiff <= off.sum(axis=1) + 1e-12 * np.abs(np.diag(Hr)))
 
 
+@pytest.mark.parametrize("npg", [5, 33, 64])
+def test_fp64_assembly_matches_oracle_other_npg(npg):
+    """The near rule at quadrature orders outside the cases above: npg = 5 (the
+    smallest lane group), 33 (the first with 64 lanes per near pair: a split
+    rule's 2 npg nodes in two rounds) and 64 (the largest the library takes),
+    on the star with its nearly real poles. Same bars as the assembly test."""
+    dad = star(ne=120, seed=5)
+    H, G = B.assemble(dad, npg)
+    Hr, Gr = O.assemble(dad, npg)
+    rep = fp64_parity(dad, H, G, Hr, Gr, npg)
+    print(f"\n[star nt={dad.nt} npg={npg}] " + ", ".join(f"{k}={v}" for k, v in rep.items()),
+          flush=True)
+    assert rep["far"] <= 1e-12 and rep["onel"] <= 1e-12
+    assert not rep["near_violations"]
+    off = np.abs(H - Hr)
+    np.fill_diagonal(off, 0.0)
+    ddiff = np.abs(np.diag(H) - np.diag(Hr))
+    assert np.all(ddiff <= off.sum(axis=1) + 1e-12 * np.abs(np.diag(Hr)))
+
+
 @pytest.mark.parametrize("case", ["square_p1", "ellipse_p2"])
 def test_near_pair_list_matches_classification(case):
     dad = CASES[case]()
