@@ -1,16 +1,18 @@
 // Device code shared by the Laplace and Kelvin (2-D elasticity) assemblies:
 // the near/far pair classification (src/Core/Assembly_full.jl:736-754), the
-// near-pair CSR build, the row sums of column-block partials and the
-// near-singular / singular rule machinery (complex pole, tan-p3c and Mobius
+// near-pair CSR build, the row sums of column-block partials and the entry
+// points of the near-singular rule machinery (complex pole, tan-p3c and Mobius
 // maps, barycentric bases; src/Core/Integration.jl:68-93, :1117-1365,
-// :1597-1845). Everything is in an anonymous namespace: each translation unit
-// that includes this header gets its own copy.
+// :1597-1845), which near_rule.hpp holds as templates shared with the adjoint.
+// Everything is in an anonymous namespace: each translation unit that includes
+// this header gets its own copy.
 //
 // Translation units including this header are compiled with -ffp-contract=off.
 
 #pragma once
 
 #include "common.hpp"
+#include "near_rule.hpp"
 
 #include <cmath>
 
@@ -344,307 +346,33 @@ __global__ __launch_bounds__(kFarThreads) void near_fill(int nt, int nwords, int
 }
 
 // ---------------------------------------------------------------------------
-// Near-singular and singular element integrals: device math.
+// Near-singular and singular element integrals: the pole, tan-p3c and basis
+// steps are the templated rule of near_rule.hpp (one source for the assembly
+// and the adjoint), instantiated here in plain FP64.
 // ---------------------------------------------------------------------------
-struct Cx {
-  double re, im;
-};
-__device__ __forceinline__ Cx cadd(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ Cx csub(Cx a, Cx b) { return {a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ Cx cmul(Cx a, Cx b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-__device__ __forceinline__ Cx cscale(Cx a, double s) { return {a.re * s, a.im * s}; }
-__device__ __forceinline__ Cx cdiv(Cx a, Cx b) {  // Smith's algorithm
-  if (fabs(b.re) >= fabs(b.im)) {
-    double r = b.im / b.re, d = b.re + b.im * r;
-    return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
-  }
-  double r = b.re / b.im, d = b.re * r + b.im;
-  return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
-}
-__device__ __forceinline__ double cabs(Cx a) { return hypot(a.re, a.im); }
-__device__ __forceinline__ Cx csqrt_(Cx z) {  // principal branch
-  if (z.re == 0.0 && z.im == 0.0) return {0.0, z.im};
-  double r = hypot(z.re, z.im);
-  if (z.re >= 0.0) {
-    double t = sqrt(0.5 * (r + z.re));
-    return {t, z.im / (2.0 * t)};
-  }
-  double t = sqrt(0.5 * (r - z.re));
-  return {fabs(z.im) / (2.0 * t), copysign(t, z.im)};
-}
+using TanPlan = rule::TanPlan<double>;
+using rule::tanp3c_count;
+using rule::tanp3c_node;
+using rule::tanp3c_plan;
 
 __device__ __forceinline__ double clampd(double x, double lo, double hi) {
-  return x > hi ? hi : (x < lo ? lo : x);
+  return rule::clampS(x, lo, hi);
 }
 
-// Barycentric interpolation row (src/Core/Interpolation.jl:148-178), NN <= 4.
-template <int NN>
-__device__ __forceinline__ void interp_row(const double* x0, const double* bw, double xx,
-                                           double* M) {
-  double msum = 0.0;
-  int exact = -1;
-#pragma unroll
-  for (int i = 0; i < NN; ++i) {
-    if (xx == x0[i]) exact = i;
-    M[i] = bw[i] / (xx - x0[i]);
-    msum += M[i];
-  }
-  if (msum == 0.0) {
-#pragma unroll
-    for (int i = 0; i < NN; ++i) M[i] = 0.0;
-  } else if (exact >= 0) {
-#pragma unroll
-    for (int i = 0; i < NN; ++i) M[i] = (i == exact) ? 1.0 : 0.0;
-  } else {
-#pragma unroll
-    for (int i = 0; i < NN; ++i) M[i] = M[i] / msum;
-  }
-}
-
-// shapefun: N and dN = N * D (src/Core/Interpolation.jl:215-218).
+// Barycentric basis and its derivative (src/Core/Interpolation.jl:148-178, :215-218).
 template <int NN>
 __device__ __forceinline__ void shapefun(const double* x0, const double* bw, const double* D,
                                          double xx, double* N, double* dN) {
-  interp_row<NN>(x0, bw, xx, N);
-#pragma unroll
-  for (int j = 0; j < NN; ++j) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < NN; ++i) s += N[i] * D[i * NN + j];
-    dN[j] = s;
-  }
+  rule::shapefun<double, NN>(x0, bw, D, xx, N, dN);
 }
 
 // Complex pole zeta + i eta of z(xi) through the collocation nodes
 // (src/Core/Integration.jl:1117-1160) for p = 1, 2.
 template <int NN>
-__device__ void complex_pole(const double* lx, const double* cx, const double* cy, double px,
-                             double py, double& zeta, double& eta) {
-  const Cx y = {px, py};
-  if (NN == 2) {
-    const double d = lx[1] - lx[0] + 1e-30;
-    const Cx dz = {cx[1] - cx[0], cy[1] - cy[0]};
-    const Cx B = {dz.re / d, dz.im / d};
-    const Cx C = csub(Cx{cx[0], cy[0]}, cscale(B, lx[0]));
-    if (cabs(B) < 1e-30) { zeta = 0.0; eta = 1.0; return; }
-    const Cx r = cdiv(csub(y, C), B);
-    zeta = r.re;
-    eta = fabs(r.im);
-    return;
-  }
-  const Cx Z0 = {cx[0], cy[0]}, Z1 = {cx[1], cy[1]}, Z2 = {cx[2], cy[2]};
-  const double a11 = lx[0] * lx[0], a12 = lx[0], a21 = lx[1] * lx[1], a22 = lx[1],
-               a31 = lx[2] * lx[2], a32 = lx[2];
-  const double det = a11 * (a22 - a32) - a12 * (a21 - a31) + (a21 * a32 - a22 * a31);
-  // Cramer's rule for [xi^2 xi 1] c = Z (_fit_quad_complex, :1153-1160)
-  auto cdivr = [](Cx z, double d) { return Cx{z.re / d, z.im / d}; };
-  const Cx A = cdivr(cadd(csub(cscale(Z0, a22 - a32), cscale(csub(Z1, Z2), a12)),
-                          csub(cscale(Z1, a32), cscale(Z2, a22))), det);
-  const Cx B = cdivr(cadd(csub(cscale(csub(Z1, Z2), a11), cscale(Z0, a21 - a31)),
-                          csub(cscale(Z2, a21), cscale(Z1, a31))), det);
-  const Cx C = cdivr(cadd(csub(cscale(csub(cscale(Z2, a22), cscale(Z1, a32)), a11),
-                               cscale(csub(cscale(Z2, a21), cscale(Z1, a31)), a12)),
-                          cscale(Z0, a21 * a32 - a22 * a31)), det);
-  if (cabs(A) < 1e-14) {
-    if (cabs(B) < 1e-30) { zeta = 0.0; eta = 1.0; return; }
-    const Cx r = cdiv(csub(y, C), B);
-    zeta = r.re;
-    eta = fabs(r.im);
-    return;
-  }
-  const Cx disc = csqrt_(csub(cmul(B, B), cmul(cscale(A, 4.0), csub(C, y))));
-  const Cx twoA = cscale(A, 2.0);
-  const Cx mB = {-B.re, -B.im};
-  const Cx r1 = cdiv(cadd(mB, disc), twoA);
-  const Cx r2 = cdiv(csub(mB, disc), twoA);
-  const Cx r = (fabs(r1.im) <= fabs(r2.im)) ? r1 : r2;
-  zeta = r.re;
-  eta = fabs(r.im);
-}
-
-// p3c inverse pole (src/Core/Integration.jl:1220-1266).
-__device__ void p3c_inv_pole(double zeta, double eta, double& zt_out, double& et_out) {
-  if (fabs(eta) < 1e-14) {
-    double x = zeta;
-    for (int it = 0; it < 25; ++it) {
-      const double g = (((x * x * x) - (3.0 * zeta) * (x * x)) + 3.0 * x) - zeta;
-      const double gp = ((3.0 * (x * x)) - (6.0 * zeta) * x) + 3.0;
-      if (fabs(gp) < 1e-30) break;
-      x -= g / gp;
-      if (fabs(g) < 1e-14) break;
-    }
-    zt_out = x;
-    et_out = 0.0;
-    return;
-  }
-  const double sgn = eta > 0.0 ? 1.0 : (eta < 0.0 ? -1.0 : 0.0);
-  double et = cbrt(fabs(eta)) * sgn;
-  for (int it = 0; it < 20; ++it) {
-    const double g = ((2.0 * (et * et * et)) - (3.0 * eta) * (et * et)) - eta;
-    const double gp = (6.0 * (et * et)) - (6.0 * eta) * et;
-    if (fabs(gp) < 1e-30) break;
-    et -= g / gp;
-  }
-  et += cbrt(eta) * sqrt(zeta * zeta + eta * eta);
-  double zt = zeta;
-  for (int it = 0; it < 30; ++it) {
-    const double den = (1.0 + 3.0 * (zt * zt)) + 3.0 * (et * et);
-    const double nz = ((3.0 + zt * zt) + 3.0 * (et * et)) * zt;
-    const double ne = 2.0 * (et * et * et);
-    const double f1 = nz / den - zeta;
-    const double f2 = ne / den - eta;
-    if (fabs(f1) + fabs(f2) < 1e-14) break;
-    const double denz = 6.0 * zt, dene = 6.0 * et;
-    const double nz_z = (3.0 + 3.0 * (zt * zt)) + 3.0 * (et * et);
-    const double nz_e = (6.0 * et) * zt;
-    const double ne_e = 6.0 * (et * et);
-    const double den2 = den * den;
-    const double j11 = (nz_z * den - nz * denz) / den2;
-    const double j12 = (nz_e * den - nz * dene) / den2;
-    const double j21 = (0.0 * den - ne * denz) / den2;
-    const double j22 = (ne_e * den - ne * dene) / den2;
-    const double detJ = j11 * j22 - j12 * j21;
-    if (fabs(detJ) < 1e-30) break;
-    zt -= (j22 * f1 - j12 * f2) / detJ;
-    et -= (-j21 * f1 + j11 * f2) / detJ;
-  }
-  zt_out = zt;
-  et_out = et;
-}
-
-struct P3C {
-  double zt, et, A, B;
-};
-__device__ __forceinline__ P3C p3c_make(double zeta, double eta) {
-  P3C c;
-  p3c_inv_pole(zeta, eta, c.zt, c.et);
-  const double den = (1.0 + 3.0 * (c.zt * c.zt)) + 3.0 * (c.et * c.et);
-  c.A = 3.0 / den;
-  c.B = ((3.0 + c.zt * c.zt) * c.zt) / den;
-  return c;
-}
-__device__ __forceinline__ void p3c_node(const P3C& c, double s, double w, double& x,
-                                         double& wo) {
-  const double ds = s - c.zt;
-  x = c.A * ((ds * ds * ds) / 3.0 + (c.et * c.et) * s) + c.B;
-  wo = w * (c.A * (ds * ds + c.et * c.et));
-}
-
-__device__ __forceinline__ bool eq41_collinear(double zeta, double eta, bool has_den,
-                                               double den) {
-  if (fabs(zeta) <= 1.0) return false;
-  if (fabs(eta) < 1e-8) return true;
-  if (has_den && fabs(den) < 1e-10) return true;
-  return false;
-}
-
-__device__ __forceinline__ void mobius_node(double zeta, double u, double w, double& x,
-                                            double& wo) {
-  if (fabs(zeta) <= 1.0) { x = u; wo = w; return; }
-  const double A = zeta * zeta - 1.0, B = -zeta;
-  const double d = u - B;
-  x = zeta - A / d;
-  wo = w * (A / (d * d));
-}
-
-__device__ __forceinline__ void tan_back(double zeta, double eta, double At, double Bt,
-                                         double ut, double wt, double& x, double& wo) {
-  const double halfpi = kPi / 2.0 - 1e-8;
-  const double th = clampd((ut - Bt) / At, -halfpi, halfpi);
-  const double tn = tan(th);
-  x = zeta + eta * tn;
-  wo = wt * ((eta / At) * (1.0 + tn * tn));
-}
-
-// _tangenttrans for one node (src/Core/Integration.jl:1177-1200).
-__device__ void tangent_node(double zeta, double eta, double u, double w, double& x,
-                             double& wo) {
-  if (eq41_collinear(zeta, eta, false, 0.0)) { mobius_node(zeta, u, w, x, wo); return; }
-  eta = fmax(eta, 1e-14);
-  const double tp = atan((1.0 - zeta) / eta);
-  const double tm = atan((-1.0 - zeta) / eta);
-  const double den = tp - tm;
-  if (eq41_collinear(zeta, eta, true, den)) { mobius_node(zeta, u, w, x, wo); return; }
-  const double A = (2.0 * eta) / den;
-  const double B = -(tp + tm) / den;
-  const double halfpi = kPi / 2.0 - 1e-8;
-  const double phi = clampd((eta * (u - B)) / A, -halfpi, halfpi);
-  const double tn = tan(phi);
-  x = zeta + eta * tn;
-  wo = w * (((eta * eta) / A) * (1.0 + tn * tn));
-}
-
-// Plan of the tan-p3c rule (src/Core/Integration.jl:1291-1365), uniform per pair.
-struct TanPlan {
-  int kind;       // 0 Mobius, 1 p3c only, 2 compose, 3 split
-  double zeta, eta, At, Bt;
-  P3C c1, c2;     // compose: c1; split: c1 left, c2 right
-  double mid1, h1, mid2, h2;
-  bool tan1, tan2;  // split half degenerated to the tangent map
-};
-
-__device__ void tanp3c_plan(double zeta, double eta, TanPlan& P) {
-  P.zeta = zeta;
-  P.eta = eta;
-  P.tan1 = P.tan2 = false;
-  if (eq41_collinear(zeta, eta, false, 0.0)) { P.kind = 0; return; }
-  eta = fmax(eta, 1e-14);
-  P.eta = eta;
-  const double ap = atan((1.0 - zeta) / eta);
-  const double am = atan((1.0 + zeta) / eta);
-  const double den = ap + am;
-  if (eq41_collinear(zeta, eta, true, den)) { P.kind = 0; return; }
-  if (fabs(den) < 1e-30) { P.kind = 1; P.c1 = p3c_make(zeta, eta); return; }
-  const double At = 2.0 / den;
-  const double Bt = ((-At) * (ap - am)) / 2.0;
-  const double uL = Bt - (At * kPi) / 2.0;
-  const double uR = Bt + (At * kPi) / 2.0;
-  P.At = At;
-  P.Bt = Bt;
-  if (fabs(zeta) < 1.0 && -1.0 < Bt && Bt < 1.0) {
-    P.kind = 3;
-    P.mid1 = (Bt + -1.0) / 2.0; P.h1 = (Bt - -1.0) / 2.0;
-    P.mid2 = (1.0 + Bt) / 2.0;  P.h2 = (1.0 - Bt) / 2.0;
-    P.tan1 = P.h1 <= 1e-14;
-    P.tan2 = P.h2 <= 1e-14;
-    if (!P.tan1) P.c1 = p3c_make((uL - P.mid1) / P.h1, 1e-14);
-    if (!P.tan2) P.c2 = p3c_make((uR - P.mid2) / P.h2, 1e-14);
-    return;
-  }
-  P.kind = 2;
-  P.c1 = p3c_make(zeta >= 1.0 ? uL : uR, 1e-14);
-}
-
-__device__ __forceinline__ int tanp3c_count(const TanPlan& P, int npg) {
-  return P.kind == 3 ? 2 * npg : npg;
-}
-
-// Node q of the tan-p3c rule built from the Gauss rule (u, w).
-__device__ void tanp3c_node(const TanPlan& P, int q, int npg, const double* u, const double* w,
-                            double& x, double& wo) {
-  switch (P.kind) {
-    case 0: mobius_node(P.zeta, u[q], w[q], x, wo); return;
-    case 1: p3c_node(P.c1, u[q], w[q], x, wo); return;
-    case 2: {
-      double ut, wt;
-      p3c_node(P.c1, u[q], w[q], ut, wt);
-      tan_back(P.zeta, P.eta, P.At, P.Bt, ut, wt, x, wo);
-      return;
-    }
-    default: {
-      const bool left = q < npg;
-      const int qq = left ? q : q - npg;
-      if (left ? P.tan1 : P.tan2) { tangent_node(P.zeta, P.eta, u[qq], w[qq], x, wo); return; }
-      const P3C& c = left ? P.c1 : P.c2;
-      const double mid = left ? P.mid1 : P.mid2, h = left ? P.h1 : P.h2;
-      double s, ws;
-      p3c_node(c, u[qq], w[qq], s, ws);
-      tan_back(P.zeta, P.eta, P.At, P.Bt, mid + h * s, ws * h, x, wo);
-      return;
-    }
-  }
+__device__ __forceinline__ void complex_pole(const double* lx, const double* cx,
+                                             const double* cy, double px, double py,
+                                             double& zeta, double& eta) {
+  rule::complex_pole<double, NN>(lx, cx, cy, px, py, zeta, eta);
 }
 
 // Sum over the G lanes of a group (xor offsets < G stay in the group).

@@ -204,6 +204,17 @@ __host__ __device__ inline Dual<D> copysign(const Dual<D>& a, const Dual<D>& b) 
   const Dual<D> m = fabs(a);
   return neg ? -m : m;
 }
+// arg(x + i y): d = (x dy - y dx) / (x^2 + y^2)
+template <int D>
+__host__ __device__ inline Dual<D> atan2(const Dual<D>& y, const Dual<D>& x) {
+  Dual<D> r(::atan2(y.v, x.v));
+  const double r2 = x.v * x.v + y.v * y.v;
+  if (r2 > 0.0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) r.d[k] = (x.v * y.d[k] - y.v * x.d[k]) / r2;
+  }
+  return r;
+}
 
 }  // namespace ad
 
@@ -212,6 +223,27 @@ __host__ __device__ inline double value(double x) { return x; }
 template <int D>
 __host__ __device__ inline double value(const ad::Dual<D>& x) {
   return x.v;
+}
+
+// true for the dual types (derivative-carrying scalars)
+template <typename S>
+struct is_dual {
+  static constexpr bool value = false;
+};
+template <int D>
+struct is_dual<ad::Dual<D>> {
+  static constexpr bool value = true;
+};
+
+// The derivatives of `deriv` with the value v: a quantity whose value comes
+// from one (reference) formula and whose derivatives come from another,
+// mathematically identical one that does not cancel.
+__host__ __device__ inline double with_value(double, double v) { return v; }
+template <int D>
+__host__ __device__ inline ad::Dual<D> with_value(const ad::Dual<D>& deriv, double v) {
+  ad::Dual<D> r = deriv;
+  r.v = v;
+  return r;
 }
 
 }  // namespace bemhip

@@ -296,15 +296,6 @@ __host__ __device__ inline void mobius_node(const S& zeta, double u, double w, S
   wo = w * (A / (d * d));
 }
 
-template <typename S>
-__host__ __device__ inline void tan_back(const S& zeta, const S& eta, const S& At, const S& Bt,
-                                         const S& ut, const S& wt, S& x, S& wo) {
-  const double halfpi = kPi / 2.0 - 1e-8;
-  const S th = clampS((ut - Bt) / At, -halfpi, halfpi);
-  const S tn = tan(th);
-  x = zeta + eta * tn;
-  wo = wt * ((eta / At) * (1.0 + tn * tn));
-}
 
 template <typename S>
 __host__ __device__ void tangent_node(const S& zeta0, const S& eta0, double u, double w, S& x,
@@ -329,16 +320,75 @@ template <typename S>
 struct TanPlan {
   int kind;  // 0 Mobius, 1 p3c only, 2 compose, 3 split
   S zeta, eta, At, Bt;
+  S up;      // compose: the tangent map's pole u (uL or uR), where p3c clusters
+  S r;       // pole form: den / eta, with den = 2 / At (see tanp3c_plan)
   P3C<S> c1, c2;
   S mid1, h1, mid2, h2;
   bool tan1, tan2;
+  bool pole_form;  // compose with |zeta| > 1 and eta^2 < (zeta^2 - 1) / 2
 };
+
+// atan(z) / z and tan(z) / z, without the cancellation of the quotient at
+// small z (series to the z^6 term below |z| = 1e-3, where it is exact in FP64).
+template <typename S>
+__host__ __device__ inline S atanc(const S& z) {
+  if (fabs(z) < 1e-3) {
+    const S z2 = z * z;
+    return 1.0 + z2 * (-1.0 / 3.0 + z2 * (1.0 / 5.0 + z2 * (-1.0 / 7.0)));
+  }
+  return atan(z) / z;
+}
+template <typename S>
+__host__ __device__ inline S tanc(const S& z) {
+  if (fabs(z) < 1e-3) {
+    const S z2 = z * z;
+    return 1.0 + z2 * (1.0 / 3.0 + z2 * (2.0 / 15.0 + z2 * (17.0 / 315.0)));
+  }
+  return tan(z) / z;
+}
+
+// Tangent map back from u to the element: x = zeta + eta tan((u - Bt) / At),
+// weight (eta / At) (1 + tan^2).
+//
+// Beyond the element (|zeta| > 1) with a small eta, At and Bt grow like
+// 1/eta and theta = (u - Bt) / At stays within ~eta of -+pi/2: the value of
+// tan(theta) carries the rounding of u - Bt magnified by 1/eta, and x is the
+// difference of terms ~1/eta. That is the reference's arithmetic and stays
+// the value; a dual evaluation takes its derivatives from the same map with
+// the 1/eta factors cancelled analytically. With up the pole (uL or uR),
+// phi = (u - up) / At = (u - up) eta r / 2 (r = den / eta) and
+// theta = phi -+ pi/2, so tan(theta) = -1 / tan(phi) and
+//   x = zeta - 2 / ((u - up) r tanc(phi)),
+//   w = w_u 2 (1 + tan^2 phi) / ((u - up)^2 r tanc(phi)^2).
+template <typename S>
+__host__ __device__ inline void tan_back(const TanPlan<S>& P, const S& ut, const S& wt, S& x,
+                                         S& wo) {
+  const double halfpi = kPi / 2.0 - 1e-8;
+  const S raw = (ut - P.Bt) / P.At;
+  const S th = clampS(raw, -halfpi, halfpi);
+  const S tn = tan(th);
+  x = P.zeta + P.eta * tn;
+  wo = wt * ((P.eta / P.At) * (1.0 + tn * tn));
+  if constexpr (is_dual<S>::value) {
+    if (P.pole_form && raw >= -halfpi && raw <= halfpi) {
+      const S du = ut - P.up;
+      const S phi = (du * P.eta) * (P.r * 0.5);
+      const S tc = tanc(phi);
+      const S tp = tan(phi);
+      const S xs = P.zeta - 2.0 / ((du * P.r) * tc);
+      const S ws = wt * ((2.0 * (1.0 + tp * tp)) / (((du * du) * P.r) * (tc * tc)));
+      x = with_value(xs, value(x));
+      wo = with_value(ws, value(wo));
+    }
+  }
+}
 
 template <typename S>
 __host__ __device__ void tanp3c_plan(const S& zeta, const S& eta0, TanPlan<S>& P) {
   P.zeta = zeta;
   P.eta = eta0;
   P.tan1 = P.tan2 = false;
+  P.pole_form = false;
   if (eq41_collinear(zeta, eta0, false, S(0.0))) { P.kind = 0; return; }
   const S eta = fmax(eta0, 1e-14);
   P.eta = eta;
@@ -364,7 +414,27 @@ __host__ __device__ void tanp3c_plan(const S& zeta, const S& eta0, TanPlan<S>& P
     return;
   }
   P.kind = 2;
-  P.c1 = p3c_make(zeta >= 1.0 ? uL : uR, S(1e-14));
+  P.up = zeta >= 1.0 ? uL : uR;
+  if constexpr (is_dual<S>::value) {
+    // uL = Bt - At pi/2 (uR = Bt + At pi/2) is O(1) while At, Bt ~ 1/eta when
+    // |zeta| > 1: with a = atan(eta/(|zeta|-1)), b = atan(eta/(|zeta|+1)),
+    // den = a - b = atan(z1), z1 = 2 eta / (zeta^2 - 1 + eta^2), and
+    // a + b = atan(z2), z2 = 2 eta |zeta| / (zeta^2 - 1 - eta^2), so the pole
+    // -sign(zeta) (a + b) / (a - b) and r = den / eta are, with no 1/eta,
+    //   up = -zeta (zeta^2 - 1 + eta^2) / (zeta^2 - 1 - eta^2) atanc(z2) / atanc(z1),
+    //   r  = 2 atanc(z1) / (zeta^2 - 1 + eta^2).
+    // Their derivatives come from these forms (the values stay the reference's).
+    const S z2m1 = zeta * zeta - 1.0, e2 = eta * eta;
+    if (fabs(zeta) > 1.0 && e2 < 0.5 * z2m1) {
+      const S qp = z2m1 + e2, qm = z2m1 - e2;
+      const S a1 = atanc((2.0 * eta) / qp);
+      const S a2 = atanc(((2.0 * eta) * fabs(zeta)) / qm);
+      P.up = with_value(((-zeta) * (qp / qm)) * (a2 / a1), value(P.up));
+      P.r = (2.0 * a1) / qp;
+      P.pole_form = true;
+    }
+  }
+  P.c1 = p3c_make(P.up, S(1e-14));
 }
 
 template <typename S>
@@ -381,7 +451,7 @@ __host__ __device__ void tanp3c_node(const TanPlan<S>& P, int q, int npg, const 
     case 2: {
       S ut, wt;
       p3c_node(P.c1, S(u[q]), S(w[q]), ut, wt);
-      tan_back(P.zeta, P.eta, P.At, P.Bt, ut, wt, x, wo);
+      tan_back(P, ut, wt, x, wo);
       return;
     }
     default: {
@@ -392,7 +462,7 @@ __host__ __device__ void tanp3c_node(const TanPlan<S>& P, int q, int npg, const 
       const S mid = left ? P.mid1 : P.mid2, h = left ? P.h1 : P.h2;
       S s, ws;
       p3c_node(c, S(u[qq]), S(w[qq]), s, ws);
-      tan_back(P.zeta, P.eta, P.At, P.Bt, mid + h * s, ws * h, x, wo);
+      tan_back(P, mid + h * s, ws * h, x, wo);
       return;
     }
   }
