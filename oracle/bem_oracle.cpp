@@ -398,6 +398,11 @@ Rule eq41_real_pole(const Rule& u, double zeta) {
   return r;
 }
 
+// Mapped nodes whose tangent angle hit the +-(pi/2 - 1e-8) clamp of
+// _tangenttrans / _tanp3c_compose / _tanp3c_half since the last reset (a
+// diagnostic of oracle_pair_plan; the arithmetic does not depend on it).
+thread_local long g_clamped = 0;
+
 // _tangenttrans (:1177-1200)
 Rule tangenttrans(const Rule& u, double zeta, double eta) {
   if (eq41_collinear(zeta, eta, false, 0.0)) return eq41_real_pole(u, zeta);
@@ -412,7 +417,9 @@ Rule tangenttrans(const Rule& u, double zeta, double eta) {
   Rule r;
   r.x.resize(u.x.size()); r.w.resize(u.x.size());
   for (size_t i = 0; i < u.x.size(); ++i) {
-    double phi = std::min(std::max((eta * (u.x[i] - B)) / A, -halfpi), halfpi);
+    const double raw = (eta * (u.x[i] - B)) / A;
+    double phi = std::min(std::max(raw, -halfpi), halfpi);
+    if (phi != raw) ++g_clamped;
     double tn = std::tan(phi);
     r.x[i] = zeta + eta * tn;
     r.w[i] = u.w[i] * (((eta * eta) / A) * (1.0 + tn * tn));
@@ -495,7 +502,9 @@ void tan_back(const Rule& ut, double zeta, double eta, double At, double Bt, Rul
   double halfpi = kPi / 2.0 - 1e-8;
   out->x.resize(ut.x.size()); out->w.resize(ut.x.size());
   for (size_t i = 0; i < ut.x.size(); ++i) {
-    double th = std::min(std::max((ut.x[i] - Bt) / At, -halfpi), halfpi);
+    const double raw = (ut.x[i] - Bt) / At;
+    double th = std::min(std::max(raw, -halfpi), halfpi);
+    if (th != raw) ++g_clamped;
     double tn = std::tan(th);
     out->x[i] = zeta + eta * tn;
     out->w[i] = ut.w[i] * ((eta / At) * (1.0 + tn * tn));
@@ -1133,6 +1142,49 @@ int oracle_pair(int p, int ne, int n, int ni, const double* coll, const double* 
   gl.x.resize(npg); gl.w.resize(npg);
   gausslegendre(npg, gl.x.data(), gl.w.data());
   integrate_element(M, gl, e, i, h, g);
+  return 0;
+}
+
+// Diagnostics of the rule a pair (i, e) takes (integrate_element, :262-313):
+// out = {zeta0, eta0, branch, clamped}. branch: -1 on-element (interpolated
+// Guiggiani), 0 Mobius (_eq41_real_pole), 1 p3c only, 2 compose
+// (_tanp3c_compose), 3 split (_tanp3c_half twice); clamped = how many of the
+// rule's nodes had their tangent angle clamped to +-(pi/2 - 1e-8)
+// (src/Core/Integration.jl:1177-1200, :1321-1365).
+int oracle_pair_plan(int p, int ne, int n, int ni, const double* coll, int npg, int i, int e,
+                     double* out) {
+  const int nn = p + 1;
+  if (e < 0 || e >= ne || i < 0 || i >= n + ni) return 1;
+  out[0] = out[1] = 0.0;
+  out[3] = 0.0;
+  if (i >= e * nn && i < (e + 1) * nn) { out[2] = -1.0; return 0; }
+  const Poly field = legendre_poly(p);
+  double zeta, eta;
+  complex_pole_1d(field, coll + 2 * (size_t)e * nn, nn, coll[2 * (size_t)i],
+                  coll[2 * (size_t)i + 1], &zeta, &eta);
+  out[0] = zeta;
+  out[1] = eta;
+  int branch;
+  if (eq41_collinear(zeta, eta, false, 0.0)) {
+    branch = 0;
+  } else {
+    const double et = std::max(eta, 1e-14);
+    const double ap = std::atan((1.0 - zeta) / et), am = std::atan((1.0 + zeta) / et);
+    const double den = ap + am;
+    if (eq41_collinear(zeta, et, true, den)) branch = 0;
+    else if (std::fabs(den) < 1e-30) branch = 1;
+    else {
+      const double At = 2.0 / den, Bt = ((-At) * (ap - am)) / 2.0;
+      branch = (std::fabs(zeta) < 1.0 && -1.0 < Bt && Bt < 1.0) ? 3 : 2;
+    }
+  }
+  out[2] = branch;
+  Rule gl;
+  gl.x.resize(npg); gl.w.resize(npg);
+  gausslegendre(npg, gl.x.data(), gl.w.data());
+  g_clamped = 0;
+  tanp3ctrans(gl, zeta, eta);
+  out[3] = (double)g_clamped;
   return 0;
 }
 

@@ -51,6 +51,7 @@ def lib():
                                                                  C.c_int, dp, dp]
         L.oracle_pair.argtypes = [C.c_int] * 4 + [dp] * 6 + [C.c_double, C.c_int, C.c_int,
                                                              C.c_int, dp, dp]
+        L.oracle_pair_plan.argtypes = [C.c_int] * 4 + [dp, C.c_int, C.c_int, C.c_int, dp]
         L.oracle_apply_bc.argtypes = [C.c_int, C.c_int, dp, dp, ip64, dp, dp, dp]
         L.oracle_kelvin.argtypes = [C.c_double] * 6 + [dp, dp]
         L.oracle_kelvin.restype = None
@@ -216,6 +217,17 @@ def pair(dad, i, e, npg=20):
     lib().oracle_pair(dad.p, dad.ne, dad.n, dad.ni, _p(coll), _p(nrm), _p(J), _p(ew), _p(geo),
                       _p(L), float(dad.properties.k), int(npg), int(i), int(e), _p(h), _p(g))
     return h, g
+
+
+def pair_plan(dad, i, e, npg=20):
+    """(zeta0, eta0, branch, clamped) of pair (i, e): branch -1 on-element,
+    0 Mobius, 1 p3c only, 2 compose, 3 split; clamped = rule nodes whose
+    tangent angle hit the +-(pi/2 - 1e-8) clamp."""
+    coll = np.ascontiguousarray(dad.collocation, dtype=np.float64)
+    out = np.zeros(4)
+    lib().oracle_pair_plan(dad.p, dad.ne, dad.n, dad.ni, _p(coll), int(npg), int(i), int(e),
+                           _p(out))
+    return float(out[0]), float(out[1]), int(out[2]), int(out[3])
 
 
 def apply_bc(dad, H, G):
