@@ -17,6 +17,7 @@ from .analytical import (AnalyticalSolution, ana_elasticity_patch, ana_exp_cos, 
 from .api import (ArgumentError, HipContext, adjoint, apply_bc, apply_operator, assemble,
                   get_HG, lu_factor, lu_solve, near_pairs, solve, solve_Hmat, split_sol,
                   unique_node_gradient)
+from .autodiff import FluxMisfit, objective, rrule_flux_misfit
 from .interior import (gera_p_in, interior_laplace, internal_grid, internal_grid_, internal_layer,
                        internal_layer_, point_in_domain, set_internal_nodes)
 
@@ -31,5 +32,5 @@ __all__ = [
     "adjoint", "apply_bc", "apply_operator", "assemble", "get_HG", "lu_factor", "lu_solve",
     "near_pairs", "solve", "solve_Hmat", "split_sol", "unique_node_gradient", "gera_p_in",
     "interior_laplace", "internal_grid", "internal_grid_", "internal_layer", "internal_layer_",
-    "point_in_domain", "set_internal_nodes",
+    "point_in_domain", "set_internal_nodes", "FluxMisfit", "objective", "rrule_flux_misfit",
 ]

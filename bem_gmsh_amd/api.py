@@ -26,8 +26,10 @@ unless ``fetch=False``.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import json
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -37,7 +39,22 @@ from ._lib import check, dptr, fptr, i64ptr, iptr, lib
 from .problem import BEMData, Elasticity
 
 PRECISIONS = {"fp64": 0, "mixed": 1, "matrix_free": 2, "fmm": 3}
-MODES = {"fp64": 0, "mixed": 1}
+MODES = {"fp64": 0, "mixed": 1, "mixed_gmres": 2}
+
+
+# Every open context, closed by an atexit hook: Python runs atexit callbacks
+# before it tears modules down and before the C runtime's own exit handlers
+# (HIP's among them), so no bemhip_destroy is left to a finalizer that runs in
+# an arbitrary order at shutdown.
+_LIVE = weakref.WeakSet()
+
+
+def _close_all_contexts():
+    for ctx in list(_LIVE):
+        ctx.close()
+
+
+atexit.register(_close_all_contexts)
 
 
 class HipContext:
@@ -52,6 +69,7 @@ class HipContext:
         check(L.bemhip_create(C.byref(h), int(device)))
         self.h = h
         self.device = device
+        _LIVE.add(self)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -205,8 +223,10 @@ def split_sol(dad: BEMData, x: np.ndarray):
 
 
 def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: int = 30):
-    """Apply BCs and solve on the GPU. ``mode`` 'fp64' (FP64 LU) or 'mixed'
-    (FP32 LU + FP64 refinement); default follows the assembly precision."""
+    """Apply BCs and solve on the GPU. ``mode`` 'fp64' (FP64 LU), 'mixed' (FP32
+    LU + FP64 refinement, GMRES-IR when refinement stops contracting) or
+    'mixed_gmres' (FP32 LU + GMRES-IR from the start); default follows the
+    assembly precision."""
     if dad.hip is None or "assembly_info" not in dad.cache:
         raise RuntimeError("cache.H is not set — assemble/solve first?")
     if dad.assembly_info["precision"] in ("matrix_free", "fmm") and mode in (None, "gmres"):
@@ -216,7 +236,7 @@ def solve(dad: BEMData, mode: Optional[str] = None, tol: float = 1e-12, maxit: i
     if mode is None:
         mode = "fp64" if dad.assembly_info["precision"] == "fp64" else "mixed"
     if mode not in MODES:
-        raise ArgumentError(f"mode must be 'fp64' or 'mixed'; got {mode!r}")
+        raise ArgumentError(f"mode must be 'fp64', 'mixed' or 'mixed_gmres'; got {mode!r}")
     apply_bc(dad)
     ctx = dad.hip
     x = np.empty(dad.dof * dad.nt, dtype=np.float64)

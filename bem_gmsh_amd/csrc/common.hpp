@@ -184,6 +184,8 @@ struct bemhip_ctx {
   const void* dinv_of = nullptr;      // factor the inverses belong to
   int dinv_n = 0;
   bemhip::DevBuf<double> vec64a, vec64b, vec64c;
+  bemhip::DevBuf<double> scal;          // device scalars of the norm reductions
+  bemhip::DevBuf<double> ir_d;          // GMRES-IR correction (lu_solve_t owns vec64c)
   bemhip::DevBuf<unsigned> solve_sync;  // block-chain solve flags, tickets, abort word
   unsigned solve_epoch = 0;
   unsigned* h_flag = nullptr;           // pinned host word: solve abort flag
@@ -246,6 +248,25 @@ int fmm_far(bemhip_ctx* c, hipStream_t s, const double* wH, const double* wG, do
 int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_py,
                   const double* d_w, const QuadConst* d_qc, int npg, double near_factor,
                   double* d_out);
+
+// Right preconditioner of the GMRES core (gmres.hip): x = P^-1 u.
+// With trans, GMRES solves A^T x = rhs (operator_apply_trans, the FP32 factors
+// applied transposed); JACOBI is for A only.
+struct Precond {
+  enum Kind { JACOBI, LU32 } kind = JACOBI;
+  const double* d = nullptr;  // JACOBI: diag A
+  bool trans = false;
+};
+int precond_apply(bemhip_ctx* c, const Precond& P, int n, const double* in, double* out);
+int gmres_core(bemhip_ctx* c, int n, const double* rhs, double* dx, bool x_zero, double atol,
+               double rtol, int m, int itmax, const Precond& P, int* iters, int* cycles,
+               double* beta_out);
+// GMRES-IR from the FP32-LU solution, for A x = b64 (trans: A^T x = rhs):
+// see solve.hip.
+int gmres_ir(bemhip_ctx* c, int nt, bool trans, const double* rhs, double* dx, double* dr,
+             double* dy, double tol, int maxit, int* outer, int* inner, bool* converged);
+// max |a_i| (and max |b_i|) on the device, read back.
+int absmax2(bemhip_ctx* c, int n, const double* a, const double* b, double* ha, double* hb);
 
 // Small host helpers
 void gauss_legendre_host(int n, double* x, double* w);

@@ -18,6 +18,10 @@
 // Stop (as Krylov.gmres, which solve_Hmat calls): ||b - A x||_2 <= atol +
 // rtol ||b||_2, tested on the Arnoldi residual estimate inside a cycle and on
 // a fresh residual at every restart.
+//
+// The same GMRES core (gmres_core) also solves the correction equations of
+// GMRES-based iterative refinement in the mixed solve (solve.hip), with the
+// FP32 LU factors as the right preconditioner instead of Jacobi.
 
 #include "common.hpp"
 
@@ -87,15 +91,20 @@ __global__ void divide_kernel(int n, const double* __restrict__ x, const double*
   if (i < n) z[i] = x[i] / d[i];
 }
 
-// x += (V[:, 0..k) y) / d
-__global__ void update_kernel(int n, int k, const double* __restrict__ V, size_t ldv,
-                              const double* __restrict__ y, const double* __restrict__ d,
-                              double* __restrict__ x) {
+// t = V[:, 0..k) y
+__global__ void combine_kernel(int n, int k, const double* __restrict__ V, size_t ldv,
+                               const double* __restrict__ y, double* __restrict__ t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
   for (int j = 0; j < k; ++j) s += V[(size_t)j * ldv + i] * y[j];
-  x[i] += s / d[i];
+  t[i] = s;
+}
+
+// x += z
+__global__ void add_kernel(int n, const double* __restrict__ z, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += z[i];
 }
 
 // r = b - y
@@ -149,60 +158,56 @@ static int norm2(bemhip_ctx* c, int n, const double* w, double* part, double* d_
   return BEMHIP_OK;
 }
 
-}  // namespace bemhip
+// out = P^-1 in: Jacobi (divide by diag A) or the FP32 LU factors of A.
+int precond_apply(bemhip_ctx* c, const Precond& P, int n, const double* in, double* out) {
+  const int tb = 256, nb = (n + tb - 1) / tb;
+  if (P.kind == Precond::JACOBI) {
+    hipLaunchKernelGGL(divide_kernel, dim3(nb), dim3(tb), 0, c->stream, n, in, P.d, out);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+    return BEMHIP_OK;
+  }
+  if (!c->factored32) return fail(c, BEMHIP_ERR_STATE, "gmres: no FP32 factors to precondition with");
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(out, in, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                   c->stream));
+  if (P.trans) return lu_solve_trans_device(c, n, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, out);
+  return lu_solve_device(c, n, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, out);
+}
 
-using namespace bemhip;
+static int apply_op(bemhip_ctx* c, bool trans, const double* x, double* y) {
+  return trans ? operator_apply_trans(c, x, y) : operator_apply(c, x, y);
+}
 
-extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int restart, int itmax,
-                                  double* x, int* iters, double* relres) {
-  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "solve_gmres: NULL context");
-  if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "solve_gmres: apply_bc first");
-  if (!(atol >= 0.0) || !(rtol >= 0.0) || (atol == 0.0 && rtol == 0.0))
-    return fail(c, BEMHIP_ERR_ARG, "solve_gmres: need atol, rtol >= 0, not both zero");
-  if (restart < 1 || restart > 512 || itmax < 1)
-    return fail(c, BEMHIP_ERR_ARG, "solve_gmres: restart must be in 1..512 and itmax >= 1");
-  hipSetDevice(c->device);
-  const int n = c->nsys();  // unknowns: dof * collocation points
-  const int m = restart;
+// GMRES(m) on A x = rhs with A P^-1 u = rhs, x = P^-1 u, continuing from the x
+// in dx (x_zero: dx holds zeros, the first residual is rhs itself). Stops when
+// ||rhs - A x||_2 <= atol + rtol ||r0||_2, r0 the first residual (Arnoldi
+// estimate inside a cycle, a fresh residual at each restart), or after itmax
+// products. beta_out: the last fresh residual norm.
+int gmres_core(bemhip_ctx* c, int n, const double* rhs, double* dx, bool x_zero, double atol,
+               double rtol, int m, int itmax, const Precond& P, int* iters, int* cycles_out,
+               double* beta_out) {
   const size_t ldv = (size_t)n;
   DevBuf<double> V, work, small;
   BEMHIP_HIP_TRY(c, V.ensure(ldv * (m + 1)));
-  BEMHIP_HIP_TRY(c, work.ensure(4 * (size_t)n));  // x, w (also r), z, D
+  BEMHIP_HIP_TRY(c, work.ensure(3 * (size_t)n));  // w (also r), z, t
   BEMHIP_HIP_TRY(c, small.ensure((size_t)kDotBlocks * (m + 1) + 2 * (size_t)(m + 1)));
-  double* dx = work.ptr;
-  double* dw = work.ptr + n;
-  double* dz = work.ptr + 2 * (size_t)n;
-  double* dd = work.ptr + 3 * (size_t)n;
+  double* dw = work.ptr;
+  double* dz = work.ptr + n;
+  double* dt = work.ptr + 2 * (size_t)n;
   double* part = small.ptr;
   double* d_h = small.ptr + (size_t)kDotBlocks * (m + 1);
   double* d_y = d_h + (m + 1);
   const int tb = 256, nb = (n + tb - 1) / tb;
-  hipEvent_t t0, t1;
-  BEMHIP_HIP_TRY(c, hipEventCreate(&t0));
-  BEMHIP_HIP_TRY(c, hipEventCreate(&t1));
-  struct Ev {
-    hipEvent_t a, b;
-    ~Ev() { hipEventDestroy(a); hipEventDestroy(b); }
-  } ev{t0, t1};
-  hipEventRecord(t0, c->stream);
-
-  if (c->dof == 2) {
-    int sj = kelvin_jacobi_diag(c, dd);
-    if (sj) return sj;
-  } else {
-    hipLaunchKernelGGL(jacobi_diag_kernel, dim3(nb), dim3(tb), 0, c->stream, n, c->n, c->nn,
-                       c->d_bc.ptr, c->diag64.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
-                       c->pair_g.ptr, dd);
-  }
-  BEMHIP_HIP_TRY(c, hipMemsetAsync(dx, 0, sizeof(double) * n, c->stream));
   int st = 0;
-  double bnorm = 0.0;
-  if ((st = norm2(c, n, c->b64.ptr, part, d_h, &bnorm))) return st;
-  const double tol = atol + rtol * bnorm;
-  // r = b (x = 0)
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(dw, c->b64.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                   c->stream));
-  double beta = bnorm;
+  if (x_zero) {
+    BEMHIP_HIP_TRY(c, hipMemcpyAsync(dw, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                     c->stream));
+  } else {
+    if ((st = apply_op(c, P.trans, dx, dz))) return st;
+    hipLaunchKernelGGL(resid_kernel, dim3(nb), dim3(tb), 0, c->stream, n, rhs, dz, dw);
+  }
+  double beta = 0.0;
+  if ((st = norm2(c, n, dw, part, d_h, &beta))) return st;
+  const double tol = atol + rtol * beta;
   int it = 0, cycles = 0;
   bool converged = beta <= tol;
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1), h2(m + 1), y(m);
@@ -215,8 +220,8 @@ extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int r
     for (; k < m && it < itmax; ++k) {
       double* vk = V.ptr + (size_t)k * ldv;
       double* vk1 = V.ptr + (size_t)(k + 1) * ldv;
-      hipLaunchKernelGGL(divide_kernel, dim3(nb), dim3(tb), 0, c->stream, n, vk, dd, dz);
-      if ((st = operator_apply(c, dz, dw))) return st;
+      if ((st = precond_apply(c, P, n, vk, dz))) return st;
+      if ((st = apply_op(c, P.trans, dz, dw))) return st;
       ++it;
       // CGS2
       if ((st = dots(c, n, k + 1, V.ptr, ldv, dw, part, d_h, h.data()))) return st;
@@ -248,7 +253,7 @@ extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int r
       g[k] = cs[k] * g[k];
       if (std::fabs(g[k + 1]) <= tol || hn == 0.0) { ++k; break; }
     }
-    // y = R^-1 g (upper triangular k x k), x += D^-1 V y
+    // y = R^-1 g (upper triangular k x k), x += P^-1 V y
     for (int i = k - 1; i >= 0; --i) {
       double s = g[i];
       for (int j = i + 1; j < k; ++j) s -= H[(size_t)j * (m + 1) + i] * y[j];
@@ -256,15 +261,73 @@ extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int r
     }
     BEMHIP_HIP_TRY(c, hipMemcpyAsync(d_y, y.data(), sizeof(double) * k, hipMemcpyHostToDevice,
                                      c->stream));
-    hipLaunchKernelGGL(update_kernel, dim3(nb), dim3(tb), 0, c->stream, n, k, V.ptr, ldv, d_y, dd,
-                       dx);
+    hipLaunchKernelGGL(combine_kernel, dim3(nb), dim3(tb), 0, c->stream, n, k, V.ptr, ldv, d_y,
+                       dt);
+    if ((st = precond_apply(c, P, n, dt, dz))) return st;
+    hipLaunchKernelGGL(add_kernel, dim3(nb), dim3(tb), 0, c->stream, n, dz, dx);
     // fresh residual
-    if ((st = operator_apply(c, dx, dz))) return st;
-    hipLaunchKernelGGL(resid_kernel, dim3(nb), dim3(tb), 0, c->stream, n, c->b64.ptr, dz, dw);
+    if ((st = apply_op(c, P.trans, dx, dz))) return st;
+    hipLaunchKernelGGL(resid_kernel, dim3(nb), dim3(tb), 0, c->stream, n, rhs, dz, dw);
     if ((st = norm2(c, n, dw, part, d_h, &beta))) return st;
     if (!std::isfinite(beta)) break;
     converged = beta <= tol;
   }
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  if (iters) *iters = it;
+  if (cycles_out) *cycles_out = cycles;
+  if (beta_out) *beta_out = beta;
+  return converged ? BEMHIP_OK : BEMHIP_ERR_NOCONV;
+}
+
+}  // namespace bemhip
+
+using namespace bemhip;
+
+extern "C" int bemhip_solve_gmres(bemhip_ctx* c, double atol, double rtol, int restart, int itmax,
+                                  double* x, int* iters, double* relres) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "solve_gmres: NULL context");
+  if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "solve_gmres: apply_bc first");
+  if (!(atol >= 0.0) || !(rtol >= 0.0) || (atol == 0.0 && rtol == 0.0))
+    return fail(c, BEMHIP_ERR_ARG, "solve_gmres: need atol, rtol >= 0, not both zero");
+  if (restart < 1 || restart > 512 || itmax < 1)
+    return fail(c, BEMHIP_ERR_ARG, "solve_gmres: restart must be in 1..512 and itmax >= 1");
+  hipSetDevice(c->device);
+  const int n = c->nsys();  // unknowns: dof * collocation points
+  DevBuf<double> work, small;
+  BEMHIP_HIP_TRY(c, work.ensure(2 * (size_t)n));  // x, D
+  BEMHIP_HIP_TRY(c, small.ensure((size_t)kDotBlocks + 2));
+  double* dx = work.ptr;
+  double* dd = work.ptr + n;
+  const int tb = 256, nb = (n + tb - 1) / tb;
+  hipEvent_t t0, t1;
+  BEMHIP_HIP_TRY(c, hipEventCreate(&t0));
+  BEMHIP_HIP_TRY(c, hipEventCreate(&t1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() { hipEventDestroy(a); hipEventDestroy(b); }
+  } ev{t0, t1};
+  hipEventRecord(t0, c->stream);
+
+  if (c->dof == 2) {
+    int sj = kelvin_jacobi_diag(c, dd);
+    if (sj) return sj;
+  } else {
+    hipLaunchKernelGGL(jacobi_diag_kernel, dim3(nb), dim3(tb), 0, c->stream, n, c->n, c->nn,
+                       c->d_bc.ptr, c->diag64.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
+                       c->pair_g.ptr, dd);
+  }
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(dx, 0, sizeof(double) * n, c->stream));
+  int st = 0;
+  double bnorm = 0.0;
+  if ((st = norm2(c, n, c->b64.ptr, small.ptr, small.ptr + kDotBlocks, &bnorm))) return st;
+  Precond P;
+  P.kind = Precond::JACOBI;
+  P.d = dd;
+  int it = 0, cycles = 0;
+  double beta = bnorm;
+  st = gmres_core(c, n, c->b64.ptr, dx, true, atol, rtol, restart, itmax, P, &it, &cycles, &beta);
+  if (st && st != BEMHIP_ERR_NOCONV) return st;
+  const bool converged = st == BEMHIP_OK;
   hipEventRecord(t1, c->stream);
   hipEventSynchronize(t1);
   float ms = 0.f;

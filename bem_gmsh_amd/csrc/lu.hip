@@ -1043,42 +1043,125 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restr
   if (tid == 0) __hip_atomic_store(&flags[rb], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Transposed solves A^T x = b with A = P^T L U: U^T w = b (forward), L^T z = w
-// (backward), x = P^T z. Left-looking dot-product form: block bi needs, for
-// each of its columns j, s_j = sum_r F[r, j] v[r] over the already solved
-// rows r, which reads column j of the factor contiguously (one workgroup per
-// column), then v_b = Dinv_b^T (b_b - s).
-template <typename T>
-__global__ __launch_bounds__(256) void dot_cols_kernel(const T* __restrict__ F, int lda, int c0,
-                                                       int r0, int r1,
-                                                       const double* __restrict__ v,
-                                                       double* __restrict__ s) {
-  __shared__ double red[4];
-  const int j = c0 + blockIdx.x;
-  const T* Fc = F + (size_t)j * lda;
-  double acc = 0.0;
-  for (int r = r0 + threadIdx.x; r < r1; r += 256) acc += (double)Fc[r] * v[r];
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+// Transposed triangle as one launch (A^T x = b with A = P^T L U: U^T w = b in
+// forward block order, UT = true; L^T z = w in backward order, UT = false), the
+// block-chain scheme of trsv_chain_kernel: workgroups take 128-column blocks in
+// dependency order from a ticket counter, add the contributions of the blocks
+// published before them (epoch flags), apply the transposed inverse of their
+// diagonal block and publish. Block cb needs, for each of its columns j,
+// s_j = sum over solved rows r of F[r, j] v[r]. Tiles are read as in the
+// forward solve (lanes over rows, coalesced): wave q4 takes the tile's columns
+// [32 q4, 32 q4 + 32) and lane l rows l and l + 64, and accumulates per column
+// across tiles; one transpose-reduction across the wave at the end (63
+// shuffles) leaves lane l with the sum for column 32 q4 + (l & 31).
+template <typename T, bool UT>
+__global__ __launch_bounds__(256) void trsv_chain_trans_kernel(int n, const T* __restrict__ F,
+                                                               int lda,
+                                                               const T* __restrict__ Dinv,
+                                                               const double* __restrict__ y,
+                                                               unsigned long long* __restrict__ xw,
+                                                               unsigned* __restrict__ flags,
+                                                               unsigned* __restrict__ ticket,
+                                                               unsigned epoch,
+                                                               unsigned* __restrict__ abort_flag) {
+  constexpr int NB = kSolveNB;
+  constexpr int QC = NB / 4;  // columns per wave
+  constexpr int HB = NB / 2;  // rows per lane pair (l, l + 64) span / k-range per half
+  __shared__ T sD[NB][NB];    // sD[k][c] = Dinv_cb(k, c)
+  __shared__ double s_x[NB], s_v[NB], s_z[2][NB];
+  __shared__ int s_blk, s_abort;
+  const int tid = threadIdx.x, lane = tid & 63, q4 = tid >> 6;
+  const int nblk = (n + NB - 1) / NB;
+  if (tid == 0) {
+    s_blk = (int)atomicAdd(ticket, 1u);
+    s_abort = 0;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) s[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
-}
-
-// v[s0 + i] = sum_k D[k + i*NB] (b[s0 + k] - s[k]), i < nb (D^T applied).
-template <typename T>
-__global__ __launch_bounds__(kSolveNB) void diag_t_kernel(int s0, int nb, const T* __restrict__ D,
-                                                          const double* __restrict__ b,
-                                                          const double* __restrict__ s,
-                                                          double* __restrict__ v) {
-  __shared__ double t[kSolveNB];
-  const int i = threadIdx.x;
-  if (i < nb) t[i] = b[s0 + i] - s[i];
+  const int t = s_blk;
+  if (t >= nblk) return;
+  const int cb = UT ? t : nblk - 1 - t;
+  const int s0 = cb * NB, nb = min(NB, n - s0);
+  {
+    const T* Db = Dinv + (size_t)cb * NB * NB;
+    for (int idx = tid; idx < NB * NB; idx += 256) sD[idx % NB][idx / NB] = Db[idx];
+  }
+  double acc[QC];
+#pragma unroll
+  for (int jj = 0; jj < QC; ++jj) acc[jj] = 0.0;
+  T tv[2 * QC];  // [0, QC): row lane, [QC, 2 QC): row lane + 64
+  auto load_tile = [&](int q) {
+    const int row = (UT ? q : nblk - 1 - q) * NB + lane;
+    const bool ra = row < n, rb2 = row + 64 < n;
+    const T* Fr = F + (size_t)row + (size_t)(s0 + q4 * QC) * lda;
+#pragma unroll
+    for (int jj = 0; jj < QC; ++jj) {
+      const bool cok = q4 * QC + jj < nb;
+      tv[jj] = (ra && cok) ? Fr[(size_t)jj * lda] : T(0);
+      tv[QC + jj] = (rb2 && cok) ? Fr[(size_t)jj * lda + 64] : T(0);
+    }
+  };
+  if (t > 0) load_tile(0);
+  for (int q = 0; q < t; ++q) {
+    const int rbk = UT ? q : nblk - 1 - q;
+    if (tid == 0) {
+      unsigned spin = 0;
+      while (ld_ag32(&flags[rbk]) != epoch) {
+        if (++spin > kSpinLimit || ld_ag32(abort_flag)) {
+          atomicExch(abort_flag, 1u);
+          s_abort = 1;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_abort) return;
+    if (tid < NB) {
+      const int j = rbk * NB + tid;
+      s_x[tid] = (j < n) ? __longlong_as_double((long long)ld_ag(&xw[j])) : 0.0;
+    }
+    __syncthreads();
+    const double xa = s_x[lane], xb = s_x[lane + 64];
+#pragma unroll
+    for (int jj = 0; jj < QC; ++jj) {
+      acc[jj] += (double)tv[jj] * xa;
+      acc[jj] += (double)tv[QC + jj] * xb;
+    }
+    if (q + 1 < t) load_tile(q + 1);
+    __syncthreads();
+  }
+  // lanes l and l ^ 32 hold the same columns for other rows: add them, then a
+  // transpose-reduction over the remaining offsets leaves lane l with column
+  // (l & 31): at offset s, lane l keeps the half of the indices its bit s picks
+#pragma unroll
+  for (int jj = 0; jj < QC; ++jj) acc[jj] += __shfl_xor(acc[jj], 32, 64);
+#pragma unroll
+  for (int sft = 16; sft >= 1; sft >>= 1) {
+    const bool up = (lane & sft) != 0;
+#pragma unroll
+    for (int k = 0; k < sft; ++k) {
+      const double send = up ? acc[k] : acc[k + sft];
+      const double keep = up ? acc[k + sft] : acc[k];
+      acc[k] = keep + __shfl_xor(send, sft, 64);
+    }
+  }
+  if (lane < QC) {
+    const int c = q4 * QC + lane;
+    s_v[c] = (c < nb) ? y[s0 + c] - acc[0] : 0.0;
+  }
   __syncthreads();
-  if (i >= nb) return;
-  const T* Dc = D + (size_t)i * kSolveNB;
-  double acc = 0.0;
-  for (int k = 0; k < nb; ++k) acc += (double)Dc[k] * t[k];
-  v[s0 + i] = acc;
+  // z_c = sum_k Dinv(k, c) v_k; thread (c, h) takes k in half h (rows k >= nb
+  // of a partial block's inverse are not written by tri_inv_kernel)
+  const int cc = tid & (NB - 1), h = tid >> 7;
+  double zp = 0.0;
+  const int k1 = min(h * HB + HB, nb);
+  for (int k = h * HB; k < k1; ++k) zp += (double)sD[k][cc] * s_v[k];
+  s_z[h][cc] = zp;
+  __syncthreads();
+  if (h == 0 && cc < nb)
+    st_ag(&xw[s0 + cc], (unsigned long long)__double_as_longlong(s_z[0][cc] + s_z[1][cc]));
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&flags[cb], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void scatter_kernel(int n, const int* __restrict__ perm, const double* __restrict__ z,
@@ -1478,7 +1561,8 @@ int ensure_dinv(bemhip_ctx* c, int n, const T* dLU, int lda) {
   return BEMHIP_OK;
 }
 
-// A^T x = b in place of d_b, with P A = L U (perm: (P v)[i] = v[perm[i]]).
+// A^T x = b in place of d_b, with P A = L U (perm: (P v)[i] = v[perm[i]]):
+// U^T w = b, L^T z = w as one block-chain launch each, x = P^T z.
 template <typename T>
 int lu_solve_trans_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dperm,
                      double* d_b) {
@@ -1488,28 +1572,35 @@ int lu_solve_trans_t(bemhip_ctx* c, int n, const T* dLU, int lda, const int* dpe
   const size_t blk_elems = (size_t)kSolveNB * kSolveNB;
   const T* DL = (const T*)c->dinv.ptr;
   const T* DU = DL + nblk * blk_elems;
-  BEMHIP_HIP_TRY(c, c->vec64c.ensure(2 * (size_t)n + kSolveNB));
-  double* w = c->vec64c.ptr;
-  double* z = c->vec64c.ptr + n;
-  double* s = c->vec64c.ptr + 2 * (size_t)n;
-  for (int bi = 0; bi < nblk; ++bi) {  // U^T w = b: rows [0, s0) of U's columns
-    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
-    hipLaunchKernelGGL((dot_cols_kernel<T>), dim3(nb), dim3(256), 0, c->stream, dLU, lda, s0, 0,
-                       s0, w, s);
-    hipLaunchKernelGGL((diag_t_kernel<T>), dim3(1), dim3(kSolveNB), 0, c->stream, s0, nb,
-                       DU + bi * blk_elems, d_b, s, w);
+  BEMHIP_HIP_TRY(c, c->vec64c.ensure(3 * (size_t)n));
+  unsigned long long* w = (unsigned long long*)c->vec64c.ptr;
+  unsigned long long* z = (unsigned long long*)(c->vec64c.ptr + n);
+  const size_t nsync = 2 * (size_t)nblk + 4;
+  if (c->solve_sync.count < nsync) {
+    BEMHIP_HIP_TRY(c, c->solve_sync.ensure(nsync));
+    BEMHIP_HIP_TRY(c, hipMemsetAsync(c->solve_sync.ptr, 0, sizeof(unsigned) * nsync, c->stream));
   }
-  for (int bi = nblk - 1; bi >= 0; --bi) {  // L^T z = w: rows [s0+nb, n) of L's columns
-    const int s0 = bi * kSolveNB, nb = std::min(kSolveNB, n - s0);
-    hipLaunchKernelGGL((dot_cols_kernel<T>), dim3(nb), dim3(256), 0, c->stream, dLU, lda, s0,
-                       s0 + nb, n, z, s);
-    hipLaunchKernelGGL((diag_t_kernel<T>), dim3(1), dim3(kSolveNB), 0, c->stream, s0, nb,
-                       DL + bi * blk_elems, w, s, z);
-  }
+  unsigned* flags = c->solve_sync.ptr;
+  unsigned* tickets = flags + 2 * (size_t)nblk;
+  unsigned* abort_flag = tickets + 2;
+  if (!c->h_flag) BEMHIP_HIP_TRY(c, hipHostMalloc((void**)&c->h_flag, sizeof(unsigned)));
+  const unsigned epoch = ++c->solve_epoch;
+  if (epoch == 0) return fail(c, BEMHIP_ERR_STATE, "lu_solve: epoch counter wrapped");
+  BEMHIP_HIP_TRY(c, hipMemsetAsync(tickets, 0, sizeof(unsigned) * 3, c->stream));  // + abort
+  hipLaunchKernelGGL((trsv_chain_trans_kernel<T, true>), dim3(nblk), dim3(256), 0, c->stream, n,
+                     dLU, lda, DU, (const double*)d_b, w, flags, tickets, epoch, abort_flag);
+  hipLaunchKernelGGL((trsv_chain_trans_kernel<T, false>), dim3(nblk), dim3(256), 0, c->stream, n,
+                     dLU, lda, DL, (const double*)w, z, flags + nblk, tickets + 1, epoch,
+                     abort_flag);
   const int tb = 256;
   hipLaunchKernelGGL(scatter_kernel, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n, dperm,
-                     z, d_b);
+                     (const double*)z, d_b);
   BEMHIP_HIP_TRY(c, hipGetLastError());
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(c->h_flag, abort_flag, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                   c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (*c->h_flag)
+    return fail(c, BEMHIP_ERR_HIP, "lu_solve: block-chain wait ran out (solve abandoned)");
   return BEMHIP_OK;
 }
 

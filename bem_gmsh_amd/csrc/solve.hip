@@ -7,7 +7,11 @@
 //                       iterative refinement: r = b - A64 x with A64 applied in
 //                       FP64 (recomputed far field + FP64 near/singular entries +
 //                       FP64 diagonal), d = (LU32)^-1 r, x += d, until
-//                       ||d||_inf <= tol ||x||_inf.
+//                       ||d||_inf <= tol ||x||_inf. When the corrections stop
+//                       contracting above the FP64 noise floor, GMRES-IR with
+//                       the FP32 factors as preconditioner takes over (the
+//                       reference's solve falls through to an iterative solve
+//                       when its factorization fails, src/Core/Solver.jl:39-50).
 
 #include "common.hpp"
 
@@ -70,20 +74,71 @@ __global__ __launch_bounds__(1024) void absmax_kernel(int n, const double* __res
   }
 }
 
+}  // namespace
+
 int absmax2(bemhip_ctx* c, int n, const double* a, const double* b, double* ha, double* hb) {
-  BEMHIP_HIP_TRY(c, c->vec64b.ensure(std::max<size_t>(c->vec64b.count, 4)));
-  double* slots = c->vec64b.ptr;  // first entries reused as scalars
+  BEMHIP_HIP_TRY(c, c->scal.ensure(2));
+  double* slots = c->scal.ptr;
   hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(1024), 0, c->stream, n, a, slots, 0);
   if (b) hipLaunchKernelGGL(absmax_kernel, dim3(1), dim3(1024), 0, c->stream, n, b, slots, 1);
   double h[2] = {0, 0};
-  BEMHIP_HIP_TRY(c, hipMemcpyAsync(h, slots, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(h, slots, sizeof(double) * (b ? 2 : 1), hipMemcpyDeviceToHost,
+                                   c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   *ha = h[0];
   if (hb) *hb = h[1];
   return BEMHIP_OK;
 }
 
-}  // namespace
+// GMRES-based iterative refinement (Carson and Higham, SIAM J. Sci. Comput.
+// 40 (2018) A817-A847): x = (LU32)^-1 rhs, then r = rhs - A x in FP64 and the
+// correction A d = r solved by GMRES right-preconditioned by the FP32 factors
+// (one cycle of at most 30 products, to ||r - A d||_2 <= 1e-6 ||r||_2), x += d,
+// until ||d||_inf <= tol ||x||_inf, or until d stops contracting at the FP64
+// noise floor (kappa eps64 <= 1e-6). It converges where classical refinement
+// needs kappa u32 < 1 and does not. trans: the same for A^T x = rhs.
+int gmres_ir(bemhip_ctx* c, int nt, bool trans, const double* rhs, double* dx, double* dr,
+             double* dy, double tol, int maxit, int* outer, int* inner, bool* converged) {
+  const int tb = 256, nb = (nt + tb - 1) / tb;
+  constexpr int kInner = 30;
+  int st = 0;
+  BEMHIP_HIP_TRY(c, c->ir_d.ensure(nt));
+  double* dd = c->ir_d.ptr;
+  Precond P;
+  P.kind = Precond::LU32;
+  P.trans = trans;
+  if ((st = precond_apply(c, P, nt, rhs, dx))) return st;
+  double dn_prev = INFINITY;
+  int k = 0, total = 0;
+  *converged = false;
+  for (k = 0; k < maxit; ++k) {
+    if ((st = trans ? operator_apply_trans(c, dx, dy) : operator_apply(c, dx, dy))) return st;
+    hipLaunchKernelGGL(sub_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, rhs, dy, dr);
+    BEMHIP_HIP_TRY(c, hipMemsetAsync(dd, 0, sizeof(double) * nt, c->stream));
+    int its = 0;
+    st = gmres_core(c, nt, dr, dd, true, 0.0, 1e-6, kInner, kInner, P, &its, nullptr, nullptr);
+    total += its;
+    if (st && st != BEMHIP_ERR_NOCONV) return st;
+    double dn = 0, xn = 0;
+    if ((st = absmax2(c, nt, dd, dx, &dn, &xn))) return st;
+    if (k < 40) {
+      char key[40];
+      std::snprintf(key, sizeof(key), "refine_gmres_dx_%02d", k);
+      c->stats[key] = xn > 0 ? dn / xn : dn;
+      std::snprintf(key, sizeof(key), "refine_gmres_its_%02d", k);
+      c->stats[key] = its;
+    }
+    if (!std::isfinite(dn)) break;
+    hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dd, dx);
+    if (dn <= tol * xn) { *converged = true; ++k; break; }
+    if (k >= 1 && dn > 0.5 * dn_prev && dn <= 1e-6 * xn) { *converged = true; ++k; break; }
+    dn_prev = dn;
+  }
+  *outer = k;
+  *inner = total;
+  return BEMHIP_OK;
+}
+
 }  // namespace bemhip
 
 using namespace bemhip;
@@ -92,13 +147,14 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
                             double* relres) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "solve: NULL context");
   if (!c->has_bc) return fail(c, BEMHIP_ERR_STATE, "solve: apply_bc first");
-  if (mode != BEMHIP_SOLVE_FP64 && mode != BEMHIP_SOLVE_MIXED)
-    return fail(c, BEMHIP_ERR_ARG, "solve: mode must be BEMHIP_SOLVE_FP64 or BEMHIP_SOLVE_MIXED");
+  if (mode != BEMHIP_SOLVE_FP64 && mode != BEMHIP_SOLVE_MIXED && mode != BEMHIP_SOLVE_MIXED_GMRES)
+    return fail(c, BEMHIP_ERR_ARG,
+                "solve: mode must be BEMHIP_SOLVE_FP64, BEMHIP_SOLVE_MIXED or BEMHIP_SOLVE_MIXED_GMRES");
   if (c->precision == BEMHIP_MATRIX_FREE || c->precision == BEMHIP_FMM)
     return fail(c, BEMHIP_ERR_STATE, "solve: LU solves need stored matrices; use bemhip_solve_gmres");
   if (mode == BEMHIP_SOLVE_FP64 && c->precision != BEMHIP_FP64)
     return fail(c, BEMHIP_ERR_STATE, "solve: FP64 LU needs an FP64 assembly (precision 0)");
-  if (mode == BEMHIP_SOLVE_MIXED && (!(tol > 0.0) || maxit < 0))
+  if (mode != BEMHIP_SOLVE_FP64 && (!(tol > 0.0) || maxit < 0))
     return fail(c, BEMHIP_ERR_ARG, "solve: tol must be > 0 and maxit >= 0");
   hipSetDevice(c->device);
   const int nt = c->nsys();  // rows of A: dof * collocation points
@@ -180,23 +236,28 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, dx))) return st;
     bool converged = false;
     double dn_prev = INFINITY;
-    c->stats.erase("refine_stagnated");
+    for (const char* key : {"refine_stagnated", "refine_gmres_ir", "refine_gmres_outer",
+                            "refine_gmres_products"})
+      c->stats.erase(key);
     for (auto it2 = c->stats.begin(); it2 != c->stats.end();)
-      it2 = (it2->first.rfind("refine_dx_", 0) == 0) ? c->stats.erase(it2) : std::next(it2);
-    for (it = 0; it < maxit; ++it) {
+      it2 = (it2->first.rfind("refine_dx_", 0) == 0 || it2->first.rfind("refine_gmres_", 0) == 0)
+                ? c->stats.erase(it2)
+                : std::next(it2);
+    const int classical = (mode == BEMHIP_SOLVE_MIXED_GMRES) ? 0 : maxit;
+    for (it = 0; it < classical; ++it) {
       if ((st = operator_apply(c, dx, dy))) return st;
       hipLaunchKernelGGL(sub_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, c->b64.ptr, dy, dr);
       if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, dr)))
         return st;
       double dn = 0, xn = 0;
       if ((st = absmax2(c, nt, dr, dx, &dn, &xn))) return st;
-      hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dr, dx);
       if (it < 40) {
         char key[32];
         std::snprintf(key, sizeof(key), "refine_dx_%02d", it);
         c->stats[key] = xn > 0 ? dn / xn : dn;
       }
       if (!std::isfinite(dn)) break;
+      hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dr, dx);
       if (dn <= tol * xn) { converged = true; ++it; break; }
       // Stagnation at the FP64 noise floor (kappa * eps64): the correction no
       // longer halves and is already below 1e-10 relative.
@@ -206,7 +267,21 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
         ++it;
         break;
       }
+      // Contracting by less than half above that floor (kappa u32 near or
+      // above 1): classical refinement would need tens of steps or diverge;
+      // GMRES-IR below takes over.
+      if (it >= 1 && dn > 0.5 * dn_prev) { ++it; break; }
       dn_prev = dn;
+    }
+    if (!converged && maxit > 0) {
+      int outer = 0, inner = 0;
+      if ((st = gmres_ir(c, nt, false, c->b64.ptr, dx, dr, dy, tol, maxit, &outer, &inner,
+                         &converged)))
+        return st;
+      c->stats["refine_gmres_ir"] = 1.0;
+      c->stats["refine_gmres_outer"] = outer;
+      c->stats["refine_gmres_products"] = inner;
+      it += outer;
     }
     hipEventRecord(r1, c->stream);
     hipEventSynchronize(r1);
@@ -219,7 +294,8 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     if (!converged && maxit > 0) {
       hipEventDestroy(t0);
       hipEventDestroy(t1);
-      return fail(c, BEMHIP_ERR_NOCONV, "solve: iterative refinement did not converge");
+      return fail(c, BEMHIP_ERR_NOCONV,
+                  "solve: iterative refinement did not converge, nor did GMRES-IR");
     }
   }
   hipEventRecord(t1, c->stream);

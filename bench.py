@@ -12,7 +12,7 @@ T = e^x cos y, Dirichlet/Neumann arcs of 64 elements.
 A step = one pass of the hot path with the geometry resident in HBM:
 assemble H, G (far field FP32, near/singular/diagonal FP64) -> mixed-BC
 column swap and FP64 right-hand side -> FP32 LU (MFMA trailing updates) ->
-FP64 iterative refinement until ||dx|| <= 1e-12 ||x||. The all-FP64 pipeline
+FP64 iterative refinement until ||dx|| <= 1e-10 ||x||. The all-FP64 pipeline
 (FP64 assembly + FP64 LU) runs once after the timed steps for the speed-up and
 the accuracy check (mixed vs FP64, per field).
 
@@ -28,8 +28,11 @@ roofline and an oracle CPU sample; the
 adjoint shape gradient on the
 N = 4095 star (1365 elements, npg = 12, target fluxes from the FP64 solve on
 the shape with a_m shifted by U(-0.01, 0.01)) after an FP64 and after a mixed
-solve, and the CPU baseline (the FP64 restatement of the reference algorithm
-on a bounded sample, host cores).
+solve; the 64-domain sweep (stars with N ~ U[2048, 16384], npg = 16): forward
+solve and adjoint gradient per domain with the factors resident, mixed and
+FP64, as entries/s, solves/s and gradients/s; and the CPU baseline (the FP64
+restatement of the reference algorithm on a bounded sample, in a child process
+whose OpenMP and BLAS thread counts are set before anything loads them).
 """
 
 from __future__ import annotations
@@ -38,6 +41,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -68,19 +72,16 @@ def build_problem(ne: int, seed: int = 1):
     return dad
 
 
-class Ctx:
+class Ctx(B.HipContext):
+    """A library context driven through the raw C ABI (closed at exit like every
+    HipContext)."""
+
     def __init__(self):
+        super().__init__(0)
         self.L = _lib.lib()
-        self.h = C.c_void_p()
-        _lib.check(self.L.bemhip_create(C.byref(self.h), 0))
 
     def chk(self, st):
         _lib.check(st, self.h)
-
-    def counters(self):
-        buf = C.create_string_buffer(1 << 16)
-        self.chk(self.L.bemhip_counters(self.h, buf, len(buf)))
-        return json.loads(buf.value.decode())
 
     def set_geometry(self, dad):
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (
@@ -115,34 +116,66 @@ class Ctx:
                                                 npg, near_factor, dptr(u)))
         return u
 
-    def close(self):
-        self.L.bemhip_destroy(self.h)
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
-def cpu_baseline(dad, npg, rows, lu_n, threads):
-    """Reference-algorithm CPU restatement (the FP64 oracle, OpenMP over rows)
-    on a bounded sample: `rows` assembly rows of this workload, and an LAPACK LU
-    (scipy/OpenBLAS) of size lu_n, extrapolated to this workload as n^3."""
+def host_threads():
+    """Threads for the CPU leg: the cores this process may run on, capped by
+    OMP_NUM_THREADS when the environment sets it (the job's CPU share)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(avail, env) if env > 0 else avail
+
+
+def cpu_leg(ne, npg, rows, lu_n):
+    """Child process: reference-algorithm CPU restatement (the FP64 oracle,
+    OpenMP over rows) on a bounded sample, i.e. `rows` assembly rows of this
+    workload and an LAPACK LU (scipy/OpenBLAS) of size lu_n, extrapolated to the
+    workload as n^3. OMP_NUM_THREADS / OPENBLAS_NUM_THREADS are set by the
+    parent before this process imports scipy; threadpoolctl reports what the
+    libraries actually use."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import scipy.linalg as sl
-    os.environ.setdefault("OPENBLAS_NUM_THREADS", str(threads))
+    from threadpoolctl import threadpool_info
+    dad = build_problem(ne)
+    threads = int(os.environ["OMP_NUM_THREADS"])
     t = time.perf_counter()
     O.assemble(dad, npg, nthreads=threads, rows=(0, rows))
     t_asm = time.perf_counter() - t
-    entries = 2.0 * rows * dad.n
     rng = np.random.default_rng(0)
     A = rng.standard_normal((lu_n, lu_n)) + lu_n * np.eye(lu_n)
+    sl.lu_factor(A[:256, :256].copy())           # load the BLAS before timing
     t = time.perf_counter()
     sl.lu_factor(A, overwrite_a=True, check_finite=False)
     t_lu = time.perf_counter() - t
-    return {
-        "entries_per_s": entries / t_asm,
-        "assembly_sample_s": t_asm,
-        "lu_sample_n": lu_n,
-        "lu_sample_s": t_lu,
-        "lu_extrapolated_s": t_lu * (dad.nt / lu_n) ** 3,
-    }
+    pools = [{"api": p.get("internal_api"), "threads": p.get("num_threads"),
+              "version": p.get("version")} for p in threadpool_info()]
+    print(json.dumps({"entries_per_s": 2.0 * rows * dad.n / t_asm, "assembly_sample_s": t_asm,
+                      "rows": rows, "lu_sample_n": lu_n, "lu_sample_s": t_lu,
+                      "lu_extrapolated_s": t_lu * (dad.nt / lu_n) ** 3,
+                      "threadpools": pools}))
+
+
+def cpu_baseline(ne, npg, rows, lu_n, threads):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OPENBLAS_NUM_THREADS=str(threads),
+               MKL_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", "--ne", str(ne),
+                        "--npg", str(npg), "--cpu-rows", str(rows), "--cpu-lu-n", str(lu_n)],
+                       env=env, capture_output=True, text=True, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def load_traffic(nt, path=TRAFFIC_FILE):
@@ -289,6 +322,72 @@ def elastic_leg(ne=5461, npg=16, seed=1, cpu_rows=1024, threads=1, reps=2):
     return out
 
 
+def sweep_sizes(nseeds=64):
+    """The 64-domain sweep: N ~ U[2048, 16384] (SplitMix64 seed 1000 + s), rounded
+    to the nearest multiple of 3; domain s is the seed-(s + 1) star."""
+    out = []
+    for s in range(nseeds):
+        rng = B.SplitMix64(1000 + s)
+        out.append((s, int(round(rng.uniform(2048.0, 16384.0) / 3.0)) * 3))
+    return out
+
+
+def sweep_leg(npg=16, nseeds=64):
+    """Per domain, in one context (buffers reused, factors resident from the
+    solve to the adjoint): mixed pipeline (FP32 far field, FP32 LU, FP64
+    refinement to 1e-10) + adjoint gradient, then the FP64 pipeline + adjoint.
+    Objective J = 1/2 sum_j w_j (q_j - q_exact_j)^2, w_j = J_j w_k. Rates from the
+    library's HIP-event times: entries/s = sum 2 nt n / sum assembly time,
+    solves/s = domains / sum solve time (LU + solves / refinement), gradients/s =
+    domains / sum adjoint time (transposed solve + gradient pass)."""
+    ctx = B.HipContext(0)
+    acc = {m: {"entries": 0.0, "assemble_ms": 0.0, "solve_ms": 0.0, "adjoint_ms": 0.0}
+           for m in ("mixed", "fp64")}
+    worst = {"T": 0.0, "q": 0.0, "grad": 0.0}
+    sizes = sweep_sizes(nseeds)
+    t0 = time.perf_counter()
+    for s, N in sizes:
+        d = build_problem(N // 3, seed=s + 1)
+        d.hip = ctx
+        qt = B.analytical_flux(d)[:d.n]
+        res = {}
+        for mode in ("mixed", "fp64"):
+            B.assemble(d, npg, precision=mode, fetch=False)
+            B.solve(d, mode=mode, tol=1e-10)
+            J, g = B.adjoint(d, qt)
+            ms = ctx.counters()["ms"]
+            a = acc[mode]
+            a["entries"] += 2.0 * d.nt * d.n
+            a["assemble_ms"] += ms["assemble"]
+            a["solve_ms"] += ms["solve_mixed" if mode == "mixed" else "solve64"]
+            a["adjoint_ms"] += ms["adjoint"]
+            res[mode] = (d.T.copy(), d.q.copy(), g)
+        (Tm, qm, gm), (T6, q6, g6) = res["mixed"], res["fp64"]
+        worst["T"] = max(worst["T"], float(np.linalg.norm(Tm - T6) / np.linalg.norm(T6)))
+        worst["q"] = max(worst["q"], float(np.linalg.norm(qm - q6) / np.linalg.norm(q6)))
+        worst["grad"] = max(worst["grad"], float(np.abs(gm - g6).max() / np.abs(g6).max()))
+    ctx.close()
+    out = {"workload": f"{nseeds} seeded stars, N ~ U[2048, 16384] "
+                       f"({min(n for _, n in sizes)}..{max(n for _, n in sizes)}), npg={npg}, "
+                       "forward solve + adjoint gradient per domain",
+           "wall_s": time.perf_counter() - t0,
+           "mixed_vs_fp64_max": worst}
+    for mode, a in acc.items():
+        out[mode] = {"entries_per_s": a["entries"] / (a["assemble_ms"] * 1e-3),
+                     "solves_per_s": nseeds / (a["solve_ms"] * 1e-3),
+                     "gradients_per_s": nseeds / (a["adjoint_ms"] * 1e-3),
+                     "assemble_s": a["assemble_ms"] * 1e-3, "solve_s": a["solve_ms"] * 1e-3,
+                     "adjoint_s": a["adjoint_ms"] * 1e-3}
+    m, f = out["mixed"], out["fp64"]
+    out["speedup_mixed_vs_fp64"] = {"assembly": m["entries_per_s"] / f["entries_per_s"],
+                                    "solve": m["solves_per_s"] / f["solves_per_s"],
+                                    "gradient": m["gradients_per_s"] / f["gradients_per_s"]}
+    log(f"[bench] sweep {nseeds} domains: mixed {m['solves_per_s']:.1f} solves/s, "
+        f"{m['gradients_per_s']:.1f} gradients/s; fp64 {f['solves_per_s']:.1f} solves/s, "
+        f"{f['gradients_per_s']:.1f} gradients/s; worst mixed vs fp64 {worst}")
+    return out
+
+
 def interior_leg(ctx, dad, npg, T, q, m_far=65536, m_near=2048, seed=3):
     """Potential at random points inside the star from the boundary solution:
     m_far points with the assembly's rules (far lumping beyond 1.5 L) and m_near
@@ -380,27 +479,6 @@ def gmres_large(spec, npg, far="dense", rtol=1e-10, restart=100, itmax=4000):
     print(json.dumps(out))
 
 
-def write_ceiling(ne):
-    """Practical HBM write bandwidth: a write-only fill (torch's fill kernel) of
-    the bytes the far-field kernel writes, 2 nt n FP32 entries."""
-    import torch
-    nt = 3 * ne
-    nbytes = 2 * nt * nt * 4
-    buf = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    best = None
-    for r in range(5):
-        a.record()
-        buf.fill_(1.0)
-        b.record()
-        b.synchronize()
-        ms = a.elapsed_time(b)
-        if r and (best is None or ms < best):
-            best = ms
-    print(json.dumps({"write_ceiling": {"bytes": nbytes, "ms": best,
-                                        "GB_per_s": nbytes / (best * 1e-3) / 1e9}}))
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -423,16 +501,18 @@ def main():
                     help="far field of the --gmres-large products")
     ap.add_argument("--phase", choices=["all", "assemble"], default="all",
                     help="'assemble': only assembly + BC passes (for counter runs)")
-    ap.add_argument("--write-ceiling", action="store_true",
-                    help="only time a write-only fill of the FP32 H+G size (HBM write ceiling)")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-rows", type=int, default=16384, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-lu-n", type=int, default=10240, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the path is single-GPU; "
               "see the sharding notes); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    if args.write_ceiling:
-        write_ceiling(args.ne)
+    if args.cpu_leg:
+        cpu_leg(args.ne, args.npg, args.cpu_rows, args.cpu_lu_n)
         return
     if args.gmres_large:
         gmres_large(args.gmres_large, args.npg, far=args.far)
@@ -463,7 +543,7 @@ def main():
     for w in range(args.warmup):
         ctx.pipeline(dad, args.npg, 1, 1)
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
-    asm_ms, lu_ms, ref_ms, iters, far_ms = [], [], [], [], []
+    asm_ms, lu_ms, ref_ms, solve_ms, iters, far_ms = [], [], [], [], [], []
     big_ms_l, big_n_l, big_flops_l, big_bytes_l = [], [], [], []
     t_start = time.perf_counter()
     x_mixed = None
@@ -479,6 +559,7 @@ def main():
         big_flops_l.append(st["lu32_big_gemm_flops"])
         big_bytes_l.append(st["lu32_big_gemm_bytes"])
         ref_ms.append(cnt.get("refine", 0.0))
+        solve_ms.append(cnt["solve_mixed"])
         iters.append(it)
         log(f"[bench] step {s + 1}/{args.steps}: assemble {cnt['assemble']:.2f} ms, "
             f"lu32 {cnt['lu32']:.1f} ms, refine {cnt.get('refine', 0):.1f} ms ({it} it), "
@@ -489,7 +570,7 @@ def main():
     asm_avg = float(np.mean(asm_ms))
     entries = 2.0 * nt * n
     value = entries / (asm_avg * 1e-3)
-    solve_mixed_s = float(np.mean(np.array(lu_ms) + np.array(ref_ms))) * 1e-3
+    solve_mixed_s = float(np.mean(solve_ms)) * 1e-3    # FP32 LU + FP64 refinement
 
     detail = {"assemble_ms": asm_avg, "far_kernel_ms": float(np.mean(far_ms)),
               "lu32_ms": float(np.mean(lu_ms)), "refine_ms": float(np.mean(ref_ms)),
@@ -504,7 +585,7 @@ def main():
         detail.update({
             "fp64_pipeline_s": time.perf_counter() - t,
             "fp64_assemble_ms": c64["assemble"], "fp64_far_kernel_ms": c64["assemble_far"],
-            "lu64_ms": c64["lu64"], "solve_fp64_s": c64["lu64"] * 1e-3 + c64.get("solve64", 0) * 0,
+            "lu64_ms": c64["lu64"], "solve_fp64_s": c64["solve64"] * 1e-3,  # LU + solves
             "fp64_relres": rel64,
             "fp64_entries_per_s": entries / (c64["assemble"] * 1e-3),
         })
@@ -516,7 +597,7 @@ def main():
         detail.update({"err_T_mixed_vs_fp64": eT, "err_q_mixed_vs_fp64": eq,
                        "rel_error_fp64": B.rel_error(dad),
                        "rel_error_flux_fp64": B.rel_error_flux(dad),
-                       "solve_speedup_mixed_vs_fp64": (c64["lu64"] * 1e-3) / solve_mixed_s,
+                       "solve_speedup_mixed_vs_fp64": (c64["solve64"] * 1e-3) / solve_mixed_s,
                        "assembly_speedup_mixed_vs_fp64": c64["assemble"] / asm_avg,
                        "lu64_gemm_ms": c64.get("lu64_gemm")})
         log(f"[bench] fp64 pipeline: assemble {c64['assemble']:.2f} ms, lu64 {c64['lu64']:.1f} ms;"
@@ -528,10 +609,11 @@ def main():
 
     if not args.no_adjoint:
         detail["adjoint"] = adjoint_leg()
+    if not args.no_sweep:
+        detail["sweep"] = sweep_leg(npg=args.npg)
     if not args.no_elastic:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         detail["elastic"] = elastic_leg(npg=args.npg, cpu_rows=0 if args.no_cpu else 1024,
-                                        threads=threads)
+                                        threads=host_threads())
 
     # roofline of the dominant kernel: the LU's FP32 trailing updates on
     # 256x256 tiles, each launch timed with HIP events on its stream
@@ -558,17 +640,25 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        threads = host_threads()
+        rows, lu_n = min(nt, 16384), min(nt, 10240)
         log(f"[bench] cpu baseline on {threads} threads ...")
-        cb = cpu_baseline(dad, args.npg, rows=min(nt, 16384), lu_n=min(nt, 10240),
-                          threads=threads)
+        cb = cpu_baseline(dad.ne, args.npg, rows, lu_n, threads)
+        asm_cpu_s = entries / cb["entries_per_s"]
         cpu = {"value": cb["entries_per_s"], "unit": "entries/s", "cores": threads,
                "kind": "port",
-               "sample": f"oracle assembly of rows 0..{min(nt, 16384)} of this workload "
-                         f"({cb['assembly_sample_s']:.2f} s); LAPACK LU n={cb['lu_sample_n']} "
+               "sample": f"oracle assembly of rows 0..{rows} of this workload "
+                         f"({cb['assembly_sample_s']:.2f} s); LAPACK LU n={lu_n} "
                          f"({cb['lu_sample_s']:.2f} s) -> {cb['lu_extrapolated_s']:.1f} s at "
                          f"n={nt} (n^3)",
-               "lu_extrapolated_s": cb["lu_extrapolated_s"]}
+               "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+                        "OMP_NUM_THREADS_env": os.environ.get("OMP_NUM_THREADS"),
+                        "cpu_model": cpu_model(), "threadpools": cb["threadpools"]},
+               "assembly_extrapolated_s": asm_cpu_s,
+               "lu_extrapolated_s": cb["lu_extrapolated_s"],
+               "gpu_vs_cpu": {"assembly": asm_cpu_s / (asm_avg * 1e-3),
+                              "solve_mixed": cb["lu_extrapolated_s"] / solve_mixed_s,
+                              "target": 50.0}}
     out = {
         "metric": "H,G entries/s; solve s to 1e-10 rel, mixed vs FP64, N=32768, 1 GPU",
         "value": value, "unit": "entries/s", "n_gpus": 1, "steps": args.steps,

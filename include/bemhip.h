@@ -56,6 +56,10 @@ typedef struct bemhip_ctx bemhip_ctx;
 /* Solve modes. */
 #define BEMHIP_SOLVE_FP64 0     /* FP64 LU (MFMA trailing updates)                */
 #define BEMHIP_SOLVE_MIXED 1    /* FP32 LU (MFMA) + FP64 iterative refinement     */
+                                /* (GMRES-IR when refinement stops contracting)   */
+#define BEMHIP_SOLVE_MIXED_GMRES 2  /* FP32 LU + GMRES-IR directly (no classical  */
+                                /* refinement): for systems known to be ill-     */
+                                /* conditioned (kappa * 6e-8 near or above 1)    */
 
 /* Library version (major*10000 + minor*100 + patch). */
 int bemhip_version(void);
@@ -159,10 +163,18 @@ int bemhip_apply_bc(bemhip_ctx* ctx, const int64_t* BC, const double* BV, double
 /* Solve A x = b on the device and return x (nt). Replaces bem_linsolve
  * (src/Core/LinearSolveUtils.jl:44-49) as called by solve
  * (src/Core/Solver.jl:24-64); split_sol! (:88-98) is left to the caller.
- *   mode     BEMHIP_SOLVE_FP64 (needs FP64 assembly) or BEMHIP_SOLVE_MIXED
- *   tol      refinement stop: ||dx||_inf <= tol * ||x||_inf (MIXED only)
- *   maxit    refinement iteration cap (MIXED only)
- *   iters    (out, may be NULL) refinement iterations taken
+ *   mode     BEMHIP_SOLVE_FP64 (needs FP64 assembly), BEMHIP_SOLVE_MIXED or
+ *            BEMHIP_SOLVE_MIXED_GMRES. MIXED refines classically (d = (LU32)^-1 r,
+ *            r = b - A x in FP64) and, once the correction stops halving above
+ *            1e-10 ||x||, switches to GMRES-IR: A d = r by GMRES with the FP32
+ *            factors as right preconditioner (Carson and Higham 2018); this
+ *            stands where the reference falls through to an iterative solve
+ *            (src/Core/Solver.jl:39-50). MIXED_GMRES starts with GMRES-IR.
+ *   tol      refinement stop: ||dx||_inf <= tol * ||x||_inf (MIXED modes); GMRES-IR
+ *            also stops when its corrections stagnate at the FP64 noise floor
+ *            (kappa * eps64 <= 1e-6)
+ *   maxit    cap on the classical steps and, separately, the GMRES-IR steps
+ *   iters    (out, may be NULL) refinement steps taken (classical + GMRES-IR)
  *   relres   (out, may be NULL) final ||b - A x||_inf / ||b||_inf in FP64
  * The factors stay resident (re-used by later solves on the same system). */
 int bemhip_solve(bemhip_ctx* ctx, int mode, double tol, int maxit, double* x, int* iters,

@@ -5,7 +5,11 @@ npg = 16).
 
 Every seed: the mixed pipeline (FP32 far field, FP32 LU, FP64 refinement)
 against the FP64 pipeline on the GPU, per field (T, q) within 1e-10, the
-relative accuracy the benchmark metric names. Every 8th seed also: the FP64
+relative accuracy the benchmark metric names; and the adjoint shape gradient of
+J = 1/2 sum_j w_j (q_j - q_exact_j)^2 after each solve, with that solve's
+factors still resident (mixed: FP32 factors + FP64-refined transposed solve;
+FP64: FP64 factors), mixed vs FP64 within 1e-8 of the largest component.
+Every 8th seed also: the FP64
 solve against LAPACK's solve of the oracle's applyBC of the same (GPU) H, G
 (both with normwise backward error <= N eps; forward difference within max(1e-10,
 4 kappa_1 eps), kappa_1 from LAPACK's dgecon), and two blocks
@@ -54,20 +58,25 @@ def test_sweep_sizes_are_the_configured_range():
 
 @pytest.mark.parametrize("block", range(8))
 def test_sweep_mixed_vs_fp64_and_oracle(block):
-    worst_T = worst_q = 0.0
+    worst_T = worst_q = worst_g = 0.0
     for s, N in sweep_sizes()[block * 8:(block + 1) * 8]:
         d = star(N // 3, seed=s + 1)
         assert d.nt == N
+        qt = B.analytical_flux(d)[:d.n]
         B.assemble(d, 16, fetch=False)
         B.solve(d, mode="fp64")
         T64, q64, x64 = d.T.copy(), d.q.copy(), d.solve_info["x"].copy()
+        J64, g64 = B.adjoint(d, qt)
         B.assemble(d, 16, precision="mixed", fetch=False)
         B.solve(d, mode="mixed", tol=1e-12)
+        Jm, gm = B.adjoint(d, qt)
         eT, eq = _rel(d.T, T64), _rel(d.q, q64)
-        worst_T, worst_q = max(worst_T, eT), max(worst_q, eq)
-        print(f"\n[seed {s}] N={N}: mixed vs fp64 T {eT:.2e} q {eq:.2e}, refine "
-              f"{d.solve_info['iters']}", flush=True)
+        eg = float(np.abs(gm - g64).max() / np.abs(g64).max())
+        worst_T, worst_q, worst_g = max(worst_T, eT), max(worst_q, eq), max(worst_g, eg)
+        print(f"\n[seed {s}] N={N}: mixed vs fp64 T {eT:.2e} q {eq:.2e} gradient {eg:.2e} "
+              f"(J {J64:.4e}), refine {d.solve_info['iters']}", flush=True)
         assert eT <= 1e-10 and eq <= 1e-10
+        assert abs(Jm - J64) <= 1e-9 * abs(J64) and eg <= 1e-8
         if s % 8 == 0:
             H, G = B.assemble(d, 16)          # FP64, fetched
             A, b = O.apply_bc(d, H, G)
@@ -97,7 +106,8 @@ def test_sweep_mixed_vs_fp64_and_oracle(block):
                   f"kappa_1 {1 / rcond:.1e}), sampled rows {er:.2e}", flush=True)
             assert eo <= bar
             assert er <= 1e-7
-    print(f"\n[block {block}] worst mixed vs fp64: T {worst_T:.2e} q {worst_q:.2e}")
+    print(f"\n[block {block}] worst mixed vs fp64: T {worst_T:.2e} q {worst_q:.2e} "
+          f"gradient {worst_g:.2e}")
 
 
 def test_ellipse_N8193_matches_oracle():
