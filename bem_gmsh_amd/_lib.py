@@ -48,6 +48,7 @@ SIGNATURES = [
     ("bemhip_lu_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p,
                                   c_double_p, C.c_int]),
     ("bemhip_adjoint", C.c_int, [C.c_void_p, c_double_p, c_double_p, c_double_p, c_double_p]),
+    ("bemhip_solve_rhs_noalloc", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     ("bemhip_solve_gmres", C.c_int,
      [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, c_double_p, c_int_p, c_double_p]),
     ("bemhip_interior_laplace", C.c_int,

@@ -23,6 +23,8 @@
 #include "common.hpp"
 
 #include <hip/hip_cooperative_groups.h>
+#include <mutex>
+#include <string>
 
 #include <algorithm>
 #include <type_traits>
@@ -1686,3 +1688,102 @@ int build_perm(bemhip_ctx* c, int n, const int* dipiv, int* dperm) {
 }
 
 }  // namespace bemhip
+
+// ---------------------------------------------------------------------------
+// Re-solve with resident FP64 factors, allocating nothing
+// (bemhip_solve_rhs_noalloc). Its device scratch lives in this file-scope
+// array, one copy per GPU; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device. Layout, in doubles:
+// [0, N) right-hand side, [N, 2N) gathered rhs, [2N, 3N) forward result words,
+// [3N, 4N) solution words, then the chain's flags [2][kNoAllocBlocks],
+// tickets [2] and abort word (unsigned).
+// ---------------------------------------------------------------------------
+namespace bemhip {
+namespace {
+
+constexpr int kNoAllocMaxN = 65536;
+constexpr int kNoAllocBlocks = kNoAllocMaxN / kSolveNB;
+constexpr size_t kNoAllocWords = 2 * (size_t)kNoAllocBlocks + 4;
+constexpr size_t kNoAllocBytes = 4 * sizeof(double) * (size_t)kNoAllocMaxN +
+                                 sizeof(unsigned) * kNoAllocWords;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[256] = {};
+unsigned g_scratch_epoch[256] = {};
+
+__global__ void zero_words_kernel(unsigned* __restrict__ w, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = 0u;
+}
+
+}  // namespace
+
+}  // namespace bemhip
+
+extern "C" int bemhip_solve_rhs_noalloc(bemhip_ctx* c, const double* b, double* x) {
+  using namespace bemhip;
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "solve_rhs_noalloc: NULL context");
+  if (!b || !x) return fail(c, BEMHIP_ERR_ARG, "solve_rhs_noalloc: NULL b or x");
+  const int n = c->nsys();
+  if (n > kNoAllocMaxN)
+    return fail(c, BEMHIP_ERR_ARG, "solve_rhs_noalloc: the system has " + std::to_string(n) +
+                                       " unknowns, above the maximum of " +
+                                       std::to_string(kNoAllocMaxN));
+  if (!c->factored64 || c->dinv_of != (const void*)c->A64.ptr || c->dinv_n != n)
+    return fail(c, BEMHIP_ERR_STATE,
+                "solve_rhs_noalloc: needs the FP64 factors of a bemhip_solve(BEMHIP_SOLVE_FP64) "
+                "on this system as the last solve");
+  if (c->device < 0 || c->device >= 256)
+    return fail(c, BEMHIP_ERR_ARG, "solve_rhs_noalloc: device index out of range");
+  BEMHIP_HIP_TRY(c, hipSetDevice(c->device));
+  unsigned char* base = nullptr;
+  unsigned epoch = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return fail(c, BEMHIP_ERR_HIP, std::string("solve_rhs_noalloc: hipGetSymbolAddress: ") +
+                                           hipGetErrorString(e));
+      g_scratch_ptr[c->device] = p;
+    }
+    base = (unsigned char*)g_scratch_ptr[c->device];
+    epoch = ++g_scratch_epoch[c->device];
+  }
+  if (epoch == 0) return fail(c, BEMHIP_ERR_STATE, "solve_rhs_noalloc: epoch counter wrapped");
+  const int NM = kNoAllocMaxN;
+  double* d_b = (double*)base;
+  double* d_y = d_b + NM;
+  unsigned long long* d_w = (unsigned long long*)(d_b + 2 * (size_t)NM);
+  unsigned long long* d_x = (unsigned long long*)(d_b + 3 * (size_t)NM);
+  unsigned* flags = (unsigned*)(d_b + 4 * (size_t)NM);
+  unsigned* tickets = flags + 2 * (size_t)kNoAllocBlocks;
+  unsigned* abort_flag = tickets + 2;
+  const int nblk = (n + kSolveNB - 1) / kSolveNB;
+  const size_t blk_elems = (size_t)kSolveNB * kSolveNB;
+  const double* DL = (const double*)c->dinv.ptr;
+  const double* DU = DL + nblk * blk_elems;
+  const int tb = 256;
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(d_b, b, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, c->stream, tickets, 3);
+  hipLaunchKernelGGL(gather_kernel, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n,
+                     c->perm.ptr, (const double*)d_b, d_y);
+  hipLaunchKernelGGL((trsv_chain_kernel<double, true>), dim3(nblk), dim3(256), 0, c->stream, n,
+                     c->A64.ptr, (int)c->ld64, DL, (const double*)d_y, d_w, flags, tickets, epoch,
+                     abort_flag);
+  hipLaunchKernelGGL((trsv_chain_kernel<double, false>), dim3(nblk), dim3(256), 0, c->stream, n,
+                     c->A64.ptr, (int)c->ld64, DU, (const double*)d_w, d_x,
+                     flags + kNoAllocBlocks, tickets + 1, epoch, abort_flag);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
+  unsigned aborted = 0;
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(x, d_x, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  BEMHIP_HIP_TRY(c, hipMemcpyAsync(&aborted, abort_flag, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                   c->stream));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (aborted)
+    return fail(c, BEMHIP_ERR_HIP, "solve_rhs_noalloc: block-chain wait ran out (solve abandoned)");
+  return BEMHIP_OK;
+}

@@ -180,6 +180,19 @@ int bemhip_apply_bc(bemhip_ctx* ctx, const int64_t* BC, const double* BV, double
 int bemhip_solve(bemhip_ctx* ctx, int mode, double tol, int maxit, double* x, int* iters,
                  double* relres);
 
+/* Solve A x = b for a new right-hand side b (host, nsys entries) with the FP64
+ * factors of the last bemhip_solve(BEMHIP_SOLVE_FP64) on this context, and
+ * allocate nothing: no hipMalloc, hipFree or allocating library call. No
+ * counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, tight serving or time-stepping loops re-solving one
+ * factored system). Its device scratch is a fixed 2.1 MB array per GPU, so
+ * the system may have at most 65536 unknowns (BEMHIP_ERR_ARG beyond);
+ * BEMHIP_ERR_STATE if the last solve was not an FP64 one of this system.
+ * Every call on a GPU shares that GPU's scratch: calls that overlap in time
+ * on one GPU (other threads, other contexts) race. Same arithmetic as the
+ * triangular solves of bemhip_solve (no refinement, no residual report). */
+int bemhip_solve_rhs_noalloc(bemhip_ctx* ctx, const double* b, double* x);
+
 /* Restarted GMRES on the device for the current system, the iterative path of
  * solve_Hmat (src/Core/Solver.jl:66-86) for operators that are not stored:
  * every product y = A x is the FP64 operator of bemhip_apply_operator (far
