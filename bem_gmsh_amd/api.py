@@ -304,12 +304,27 @@ def apply_operator(dad: BEMData, x: np.ndarray) -> np.ndarray:
     return y
 
 
-def lu_factor(A: np.ndarray, ctx: Optional[HipContext] = None):
-    """Dense LU with partial pivoting on the GPU of a host matrix (float64/float32).
-    Returns (LU, ipiv) in the LAPACK convention (0-based ipiv)."""
+PIVOTING = {"partial": 0, "tournament": 1}
+
+
+def set_lu_pivoting(ctx: HipContext, pivoting: str) -> None:
+    """FP32 LU pivoting of a context: 'partial' (default; GEPP) or 'tournament'
+    (TSLU panels). The FP64 LU always pivots partially."""
+    if pivoting not in PIVOTING:
+        raise ArgumentError(f"pivoting must be 'tournament' or 'partial'; got {pivoting!r}")
+    ctx.check(lib().bemhip_set_lu_pivoting(ctx.h, PIVOTING[pivoting]))
+
+
+def lu_factor(A: np.ndarray, ctx: Optional[HipContext] = None, pivoting: Optional[str] = None):
+    """Dense LU on the GPU of a host matrix (float64/float32): partial pivoting in
+    FP64; FP32 pivots by ``pivoting`` ('tournament' or 'partial'; default: the
+    context's setting, partial for a new one). Returns (LU, ipiv) in the
+    LAPACK convention (0-based ipiv)."""
     ctx = ctx or HipContext(0)
     if A.ndim != 2 or A.shape[0] != A.shape[1]:
         raise ArgumentError("lu_factor: square matrix required")
+    if pivoting is not None:
+        set_lu_pivoting(ctx, pivoting)
     dtype = 0 if A.dtype == np.float64 else 1
     LU = np.array(A, dtype=np.float64 if dtype == 0 else np.float32, order="F", copy=True)
     ipiv = np.empty(A.shape[0], dtype=np.int32)

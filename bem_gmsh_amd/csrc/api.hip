@@ -311,6 +311,16 @@ int bemhip_set_elasticity(bemhip_ctx* c, double mu, double nu) {
   return BEMHIP_OK;
 }
 
+int bemhip_set_lu_pivoting(bemhip_ctx* c, int mode) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "set_lu_pivoting: NULL context");
+  if (mode != BEMHIP_PIVOT_TOURNAMENT && mode != BEMHIP_PIVOT_PARTIAL)
+    return fail(c, BEMHIP_ERR_ARG,
+                "set_lu_pivoting: mode must be BEMHIP_PIVOT_TOURNAMENT or BEMHIP_PIVOT_PARTIAL");
+  c->lu32_pivot = mode;
+  c->factored32 = false;
+  return BEMHIP_OK;
+}
+
 int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "assemble: NULL context");
   if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "assemble: set_geometry first");

@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <type_traits>
 #include <climits>
+#include <limits>
 #include <cmath>
 
 namespace cg = cooperative_groups;
@@ -102,6 +103,9 @@ __device__ __forceinline__ void st_ag(unsigned long long* p, unsigned long long 
 __device__ __forceinline__ unsigned long long ld_ag(const unsigned long long* p) {
   return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_ag32(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned ld_ag32(const unsigned* p) {
   return __hip_atomic_load(const_cast<unsigned*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -227,11 +231,21 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
       // 1. wave candidate over rows >= c (column c is register u)
       T bv = (valid && r >= c) ? fabs(v[u]) : T(-1);
       int br = (valid && r >= c) ? r : INT_MAX;
+      if constexpr (sizeof(T) == 4) {
+        // |a| bits order like |a| (+1: an eligible zero beats no row); one
+        // DPP wave reduction, then the lowest lane (= lowest row) holding it
+        const unsigned key = br != INT_MAX ? __float_as_uint(bv) + 1u : 0u;
+        const unsigned kmax = __ockl_wfred_max_u32(key);
+        const unsigned long long hit = __ballot(key == kmax && kmax != 0u);
+        bv = hit ? __uint_as_float(kmax - 1u) : T(-1);
+        br = hit ? rbase + w * 64 + (int)__ffsll((long long)hit) - 1 : INT_MAX;
+      } else {
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const T ov = __shfl_xor(bv, off, 64);
-        const int orr = __shfl_xor(br, off, 64);
-        if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+        for (int off = 32; off >= 1; off >>= 1) {
+          const T ov = __shfl_xor(bv, off, 64);
+          const int orr = __shfl_xor(br, off, 64);
+          if (better(ov, orr, bv, br)) { bv = ov; br = orr; }
+        }
       }
       if (lane == 0) { s_wval[w] = bv; s_wrowi[w] = br; }
       if (br != INT_MAX && r == br) {
@@ -384,7 +398,10 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
         }
         const T piv = s_pr[u];
         const bool below = valid && r > c;
-        const T l = below ? v[u] / piv : T(0);
+        // LAPACK's scaling: by the reciprocal, or by division below the safe minimum
+        const bool recip = fabs(piv) >= std::numeric_limits<T>::min();
+        const T rp = T(1) / piv;
+        const T l = below ? (recip ? v[u] * rp : v[u] / piv) : T(0);
         if (below) v[u] = l;
         // later columns: registers u+1 .. W-c0-1, in groups of kPU
 #pragma unroll
@@ -415,6 +432,451 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
   for (int k = 0; k < W; ++k) {
     const int j = (rot + k) % W;
     if (valid && j < kb) Ap[(size_t)r + (size_t)j * lda] = v[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 panels by tournament pivoting: TSLU, the panel factorization of CALU
+// (Grigori, Demmel and Xiang, SIAM J. Matrix Anal. Appl. 32(4), 2011).
+//
+// The panel's m rows are cut into leaves of kPR = 512 rows. A leaf workgroup
+// runs GEPP on its rows in registers, with no exchange between workgroups, and
+// keeps the kb rows it pivoted on as candidates. Candidate sets meet in a tree
+// of fan-in kTF = 4 (4 x 128 rows: a leaf's size again): a node runs GEPP on
+// the ORIGINAL values of its children's candidates and passes its kb pivot
+// rows up. The root's GEPP fixes the panel's kb pivot rows in order; its registers
+// then hold their factors L11\U11. The tree needs no grid barrier: a workgroup
+// that finishes a node writes the candidate set, waits for its stores to be
+// acknowledged and counts itself in at the parent; the last child to arrive
+// goes on as the parent's worker and the others leave, so the launch works at
+// any residency and nothing waits. Candidate words are written and read with
+// relaxed agent-scope atomics (coherent across XCDs); the writer's stores are
+// acknowledged (s_waitcnt vmcnt(0)) before it counts itself in, so the worker
+// that reads the final count reads complete sets.
+//
+// Pivot choice: the largest |a| among the rows that have not pivoted yet,
+// lowest position on ties (a leaf's positions are its rows in order, an inner
+// node's its children's candidates in order), so a panel of one leaf
+// (m <= 512) is plain GEPP. The interchanges (ipiv, LAPACK order) reach the
+// outer block's columns through perm_build / perm_apply, and tslu_apply then
+// forms L21 = A21 U11^-1 row by row with U11 in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kTF = 4;                 // fan-in of the candidate tree
+constexpr int kTW = 128;               // FP32 panel width
+constexpr unsigned kNoRow = 0xffffffffu;
+
+struct TsluArgs {
+  float* A;
+  int lda;
+  int m;                      // panel rows (global rows k0 .. k0+m-1)
+  int kb;                     // panel width (<= kTW)
+  int k0;                     // global index of the panel's first row/column
+  int G;                      // leaves
+  unsigned* cand;             // [node][kTW][kTW]: word j*kTW + i = column j of candidate i
+  unsigned* cid;              // [node][kTW]: panel row of candidate i (kNoRow: none)
+  unsigned* count;            // [node] arrivals at an inner node (zero between launches)
+  float* top;                 // [kTW][kTW]: row c = L11\U11 row of the step-c pivot
+  int* ipiv;                  // ipiv[k0 + c] = global row swapped with row k0 + c
+  int* info;                  // first zero pivot: global column + 1 (0 if none)
+  unsigned* abort_flag;       // raised if a candidate set reads back inconsistent
+};
+
+// Tree level l: n_0 = G nodes, n_{l+1} = ceil(n_l / kTF); its first node slot.
+__device__ __forceinline__ void tslu_level(int G, int level, int& size, int& off) {
+  int s = G, o = 0;
+  for (int l = 0; l < level; ++l) {
+    o += s;
+    s = (s + kTF - 1) / kTF;
+  }
+  size = s;
+  off = o;
+}
+
+// Layout of a node's 512 x 128 block in a workgroup: wave w holds columns
+// 16w .. 16w+15, lane l rows 8l .. 8l+7 (a[i][j] = row 8l+i, column 16w+j).
+// A step of GEPP then needs, per thread, 8 multipliers and 16 pivot-row
+// values: the wave owning the step's column picks the pivot (one wave
+// reduction) and publishes the multipliers in LDS (one barrier per step,
+// double-buffered); each wave reads its 16 pivot-row values from the lane
+// holding the pivot row into scalar registers (v_readlane), where the finished
+// columns are zeroed by scalar selects.
+constexpr int kTR = 8;    // rows per thread
+constexpr int kTC = 16;   // columns per thread
+
+struct TsluShared {
+  float mult[2][kPR];          // multipliers of the step, by parity
+  int piv_pos[2];              // pivot position of the step (-1: no row left)
+  int zero[kTW];               // step c had a zero pivot
+  unsigned rid[kTW];           // panel row of step c's pivot
+  float stage[kTW][kTW + 1];   // originals of the candidates, row = pivot step
+  int flag;
+};
+
+// a[i][j] of lane `lane` for wave-uniform i < kTR and lane (a scalar branch
+// picks the register; no run-time register index). j: a compile-time index
+// after unrolling.
+__device__ __forceinline__ float tslu_lane_value(const float (&a)[kTR][kTC], int i, int j,
+                                                 int lane) {
+  int x;
+  switch (i) {
+    case 0: x = __builtin_amdgcn_readlane(__float_as_int(a[0][j]), lane); break;
+    case 1: x = __builtin_amdgcn_readlane(__float_as_int(a[1][j]), lane); break;
+    case 2: x = __builtin_amdgcn_readlane(__float_as_int(a[2][j]), lane); break;
+    case 3: x = __builtin_amdgcn_readlane(__float_as_int(a[3][j]), lane); break;
+    case 4: x = __builtin_amdgcn_readlane(__float_as_int(a[4][j]), lane); break;
+    case 5: x = __builtin_amdgcn_readlane(__float_as_int(a[5][j]), lane); break;
+    case 6: x = __builtin_amdgcn_readlane(__float_as_int(a[6][j]), lane); break;
+    default: x = __builtin_amdgcn_readlane(__float_as_int(a[7][j]), lane); break;
+  }
+  return __int_as_float(x);
+}
+
+// The 16 values of row i of lane `lane` (wave-uniform i, lane) as scalars.
+__device__ __forceinline__ void tslu_lane_row(const float (&a)[kTR][kTC], int i, int lane,
+                                              float (&pr)[kTC]) {
+#define TSLU_ROW(I)                                                                   \
+  case I:                                                                             \
+    _Pragma("unroll") for (int j = 0; j < kTC; ++j) pr[j] =                           \
+        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[I][j]), lane));      \
+    break;
+  switch (i) {
+    TSLU_ROW(0) TSLU_ROW(1) TSLU_ROW(2) TSLU_ROW(3) TSLU_ROW(4) TSLU_ROW(5) TSLU_ROW(6)
+    default:
+      TSLU_ROW(7)
+  }
+#undef TSLU_ROW
+}
+
+// GEPP for kb steps over the workgroup's rows (live: bit i = row i of the
+// thread still in play). stp[i] = step at which row i became a pivot (-1:
+// never). On return a[][] holds, for the pivot rows, their L\U factors.
+// Returns the number of steps that found a row.
+__device__ __forceinline__ int tslu_gepp(float (&a)[kTR][kTC], unsigned& live, int (&stp)[kTR],
+                                         int kb, TsluShared& S) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int nsel = 0;
+  int cb = 0;
+  for (; cb * kTC < kb; ++cb) {
+    if (w < cb) break;  // wave-uniform: this wave's columns are all finished
+#pragma unroll
+    for (int jc = 0; jc < kTC; ++jc) {
+      const int c = cb * kTC + jc;
+      const int par = c & 1;
+      if (w == cb) {
+        // the wave owning column c: pivot = largest |a| over the live rows,
+        // lowest row on ties (|a| bits order like |a|, +1 so a zero beats none)
+        unsigned kbest = 0u;
+        int ibest = 0;
+#pragma unroll
+        for (int i = 0; i < kTR; ++i) {
+          const unsigned k =
+              (((live >> i) & 1u) && c < kb) ? __float_as_uint(fabsf(a[i][jc])) + 1u : 0u;
+          if (k > kbest) { kbest = k; ibest = i; }
+        }
+        const unsigned kmax = __ockl_wfred_max_u32(kbest);
+        const unsigned long long hit = __ballot(kbest == kmax && kmax != 0u);
+        const int lw = hit ? (int)__ffsll((long long)hit) - 1 : 0;
+        const int iw = __builtin_amdgcn_readlane(ibest, lw);
+        const float piv = tslu_lane_value(a, iw, jc, lw);
+        const bool nz = kmax != 0u && piv != 0.f;
+        const float rp = nz ? 1.f / piv : 0.f;
+        float mi[kTR];
+#pragma unroll
+        for (int i = 0; i < kTR; ++i) {
+          const bool upd = nz && ((live >> i) & 1u) && !(lane == lw && i == iw);
+          mi[i] = upd ? a[i][jc] * rp : 0.f;
+        }
+        if (nz && !(fabsf(piv) >= 1.17549435e-38f)) {
+          // LAPACK divides instead below the safe minimum
+#pragma unroll
+          for (int i = 0; i < kTR; ++i) mi[i] = mi[i] != 0.f ? a[i][jc] / piv : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < kTR; ++i) a[i][jc] = mi[i] != 0.f ? mi[i] : a[i][jc];
+        float4* md = reinterpret_cast<float4*>(&S.mult[par][kTR * lane]);
+        md[0] = make_float4(mi[0], mi[1], mi[2], mi[3]);
+        md[1] = make_float4(mi[4], mi[5], mi[6], mi[7]);
+        if (lane == 0) {
+          S.piv_pos[par] = kmax ? kTR * lw + iw : -1;
+          if (c < kb) S.zero[c] = nz ? 0 : 1;
+        }
+      }
+      __syncthreads();
+      const int p = __builtin_amdgcn_readfirstlane(S.piv_pos[par]);
+      const float4* md = reinterpret_cast<const float4*>(&S.mult[par][kTR * lane]);
+      const float4 m0 = md[0], m1 = md[1];
+      const float mi[kTR] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};
+      float pr[kTC];
+      if (p >= 0) {
+        const int lw = p >> 3, iw = p & 7;
+        nsel += 1;
+#pragma unroll
+        for (int i = 0; i < kTR; ++i) stp[i] = (lane == lw && i == iw) ? c : stp[i];
+        live &= (lane == lw) ? ~(1u << iw) : ~0u;
+        tslu_lane_row(a, iw, lw, pr);
+      } else {
+#pragma unroll
+        for (int j = 0; j < kTC; ++j) pr[j] = 0.f;
+      }
+      // rank-1 update; the owner's columns up to c take pivot-row value 0
+      // (scalar selects; waves left of the owner have left the loop)
+      const bool owner = w == cb;
+#pragma unroll
+      for (int j = 0; j < kTC; ++j) {
+        const float prj = (owner && j <= jc) ? 0.f : pr[j];
+#pragma unroll
+        for (int i = 0; i < kTR; ++i) a[i][j] -= mi[i] * prj;
+      }
+    }
+  }
+  // A wave whose columns are all finished only keeps step with the others:
+  // one barrier per remaining step and the pivot bookkeeping.
+  const int nsteps = (kb + kTC - 1) / kTC * kTC;
+  for (int c = cb * kTC; c < nsteps; ++c) {
+    __syncthreads();
+    const int p = __builtin_amdgcn_readfirstlane(S.piv_pos[c & 1]);
+    if (p >= 0) {
+      const int lw = p >> 3, iw = p & 7;
+      nsel += 1;
+#pragma unroll
+      for (int i = 0; i < kTR; ++i) stp[i] = (lane == lw && i == iw) ? c : stp[i];
+      live &= (lane == lw) ? ~(1u << iw) : ~0u;
+    }
+  }
+  __syncthreads();
+  return nsel;
+}
+
+// Candidate ci of the set at `slot`: its panel row, and its values in this
+// wave's columns. The set's words were acknowledged before its writer counted
+// itself in at the parent, and this workgroup read that count, so plain
+// agent-scope loads see them.
+__device__ __forceinline__ unsigned tslu_read_cand(const TsluArgs& a, size_t slot, int ci, int w,
+                                                   float (&r)[kTC]) {
+  unsigned rid = ld_ag32(a.cid + slot * kTW + ci);
+  if (rid != kNoRow && rid >= (unsigned)a.m) {  // never for a consistent set
+    atomicExch(a.abort_flag, 1u);
+    rid = kNoRow;
+  }
+  const unsigned* cand = a.cand;
+  asm volatile("" : "+s"(cand));  // no addresses kept live across calls
+  const unsigned* dw = cand + slot * (size_t)kTW * kTW + ci;
+#pragma unroll
+  for (int j = 0; j < kTC; ++j) {
+    const int col = kTC * w + j;
+    r[j] = (rid != kNoRow && col < a.kb) ? __uint_as_float(ld_ag32(dw + (size_t)col * kTW)) : 0.f;
+  }
+  return rid;
+}
+
+// Row i of the thread (position 8 lane + i of node `node` at `level`) in this
+// wave's columns: a leaf's row node * 512 + position of the panel Ap, an inner
+// node's candidate (position & 127) of child (position >> 7). Returns its
+// panel row (kNoRow: none).
+__device__ __forceinline__ unsigned tslu_load_row(const TsluArgs& a, const float* Ap, int level,
+                                                  int node, int i, float (&r)[kTC]) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int pos = kTR * lane + i;
+  asm volatile("" : "+s"(Ap));  // a fresh base per call: no addresses kept live
+  if (level == 0) {
+    const int row = node * kPR + pos;
+    const unsigned rr = row < a.m ? (unsigned)row : kNoRow;
+    const size_t lda = (size_t)a.lda;
+#pragma unroll
+    for (int j = 0; j < kTC; ++j) {
+      const int col = kTC * w + j;
+      r[j] = (rr != kNoRow && col < a.kb) ? Ap[(size_t)row + (size_t)col * lda] : 0.f;
+    }
+    return rr;
+  }
+  int np, op;
+  tslu_level(a.G, level - 1, np, op);
+  const int q = pos >> 7;
+  if (kTF * node + q < np)
+    return tslu_read_cand(a, (size_t)op + (size_t)kTF * node + q, pos & (kTW - 1), w, r);
+#pragma unroll
+  for (int j = 0; j < kTC; ++j) r[j] = 0.f;
+  return kNoRow;
+}
+
+__global__ __launch_bounds__(kPR, 1) void tslu_kernel(TsluArgs a) {
+  __shared__ __align__(16) TsluShared S;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const float* Ap = a.A + (size_t)a.k0 + (size_t)a.k0 * (size_t)a.lda;
+  float v[kTR][kTC];
+  unsigned rid[kTR];
+
+  int level = 0, node = blockIdx.x;
+  for (;;) {
+    int nlev, off;
+    tslu_level(a.G, level, nlev, off);
+    unsigned live = 0u;
+#pragma unroll
+    for (int i = 0; i < kTR; ++i) {
+      rid[i] = tslu_load_row(a, Ap, level, node, i, v[i]);
+      live |= (rid[i] != kNoRow) ? (1u << i) : 0u;
+    }
+    int stp[kTR];
+#pragma unroll
+    for (int i = 0; i < kTR; ++i) stp[i] = -1;
+    const int nsel = tslu_gepp(v, live, stp, a.kb, S);
+    if (w == 0) {
+#pragma unroll
+      for (int i = 0; i < kTR; ++i)
+        if (stp[i] >= 0) S.rid[stp[i]] = rid[i];
+    }
+    if (nlev == 1) {
+      // root: the pivot rows' L11\U11 (row-major, row c = step c)
+#pragma unroll
+      for (int i = 0; i < kTR; ++i) {
+        if (stp[i] >= 0) {
+#pragma unroll
+          for (int j = 0; j < kTC; ++j) {
+            const int col = kTC * w + j;
+            if (col < a.kb) a.top[(size_t)stp[i] * kTW + col] = v[i][j];
+          }
+        }
+      }
+      __syncthreads();
+      if (tid < 64) {
+        // LAPACK-order interchanges: position c takes the step-c pivot row.
+        // Positions >= c of the first kb hold original top rows; lane l keeps
+        // where[x] (current position of top row x) and topc[p] (top row at
+        // position p) for x, p in {l, l + 64}.
+        int wh0 = lane, wh1 = lane + 64, tc0 = lane, tc1 = lane + 64;
+        if (nsel < a.kb && lane == 0) atomicExch(a.abort_flag, 1u);  // m >= kb rows
+        for (int c = 0; c < a.kb; ++c) {
+          const int wr = c < nsel ? (int)S.rid[c] : c;
+          int pos;
+          if (wr >= a.kb) {
+            pos = wr;
+          } else {
+            const int t0 = __shfl(wh0, wr & 63, 64), t1 = __shfl(wh1, wr & 63, 64);
+            pos = wr < 64 ? t0 : t1;
+          }
+          const int x0 = __shfl(tc0, c & 63, 64), x1 = __shfl(tc1, c & 63, 64);
+          const int x = c < 64 ? x0 : x1;
+          if (lane == 0) a.ipiv[a.k0 + c] = a.k0 + pos;
+          if (x < 64) { if (lane == x) wh0 = pos; } else { if (lane == x - 64) wh1 = pos; }
+          if (pos < a.kb) {
+            if (pos < 64) { if (lane == pos) tc0 = x; } else { if (lane == pos - 64) tc1 = x; }
+          }
+        }
+        if (lane == 0) {
+          for (int c = 0; c < a.kb; ++c)
+            if (S.zero[c]) {
+              atomicCAS(a.info, 0, a.k0 + c + 1);
+              break;
+            }
+        }
+      }
+      return;
+    }
+    // the originals of this node's candidates, in pivot order, into LDS
+#pragma unroll
+    for (int i = 0; i < kTR; ++i) {
+      if (stp[i] >= 0) {
+        float r[kTC];
+        tslu_load_row(a, Ap, level, node, i, r);
+#pragma unroll
+        for (int j = 0; j < kTC; ++j) S.stage[stp[i]][kTC * w + j] = r[j];
+      }
+    }
+    __syncthreads();
+    const size_t slot = (size_t)off + node;
+    unsigned* cw = a.cand + slot * (size_t)kTW * kTW;
+    for (int wd = tid; wd < kTW * kTW; wd += kPR) {
+      const int j = wd >> 7, ci = wd & (kTW - 1);
+      if (ci < nsel && j < a.kb) st_ag32(cw + wd, __float_as_uint(S.stage[ci][j]));
+    }
+    if (tid < kTW) st_ag32(a.cid + slot * kTW + tid, tid < nsel ? S.rid[tid] : kNoRow);
+    // every word acknowledged before the arrival is counted
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    int nnext, onext;
+    tslu_level(a.G, level + 1, nnext, onext);
+    const int parent = node / kTF;
+    const int nchild = min(kTF, nlev - kTF * parent);
+    unsigned* cnt = a.count + onext + parent;
+    if (tid == 0)
+      S.flag = (int)__hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (S.flag != nchild - 1) return;  // a sibling is still running: it goes on
+    if (tid == 0) __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // go on as the parent (its children's candidate sets)
+    level += 1;
+    node = parent;
+  }
+}
+
+// Panel rows [kb, m) after the interchanges: L21 = A21 U11^-1 (thread = row,
+// U11 and the reciprocals of its diagonal in LDS); workgroup 0 also writes
+// L11\U11 into rows [0, kb). Multipliers take LAPACK's scaling (by the
+// reciprocal, or by division below the safe minimum); a zero pivot leaves its
+// column as is (the factorization reports it through info).
+constexpr int kApplyThreads = 256;
+__global__ __launch_bounds__(kApplyThreads) void tslu_apply(float* __restrict__ A, int lda, int m,
+                                                            int kb, int k0,
+                                                            const float* __restrict__ top) {
+  __shared__ __align__(16) float sU[kTW][kTW];
+  __shared__ float sR[kTW];
+  const int tid = threadIdx.x;
+  const size_t ld = (size_t)lda;
+  float* Ap = A + (size_t)k0 + (size_t)k0 * ld;
+  for (int idx = tid; idx < kb * kTW; idx += kApplyThreads) sU[idx / kTW][idx % kTW] = top[idx];
+  __syncthreads();
+  for (int c = tid; c < kb; c += kApplyThreads) {
+    const float piv = sU[c][c];
+    sR[c] = fabsf(piv) >= 1.17549435e-38f ? 1.f / piv : 0.f;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int idx = tid; idx < kb * kb; idx += kApplyThreads) {
+      const int col = idx / kb, row = idx - col * kb;
+      Ap[(size_t)row + (size_t)col * ld] = sU[row][col];
+    }
+  }
+  const int r = kb + blockIdx.x * kApplyThreads + tid;
+  if (r >= m) return;
+  float v[kTW];
+#pragma unroll
+  for (int j = 0; j < kTW; ++j) v[j] = (j < kb) ? Ap[(size_t)r + j * ld] : 0.f;
+  int rot = 0;  // register k holds column (rot + k) mod kTW
+  for (int c0 = 0; c0 < kb; c0 += kPU) {
+#pragma unroll
+    for (int u = 0; u < kPU; ++u) {
+      const int c = c0 + u;
+      if (c >= kb) break;
+      const float piv = sU[c][c];
+      if (piv != 0.f) {
+        const float rp = sR[c];
+        const float l = rp != 0.f ? v[u] * rp : v[u] / piv;
+        v[u] = l;
+#pragma unroll
+        for (int g0 = 0; g0 < kTW; g0 += kPU) {
+          if (g0 + kPU <= u) continue;
+          if (g0 < kTW - c0) {
+#pragma unroll
+            for (int k = g0; k < g0 + kPU; ++k)
+              if (k > u) v[k] -= l * sU[c][(c0 + k) & (kTW - 1)];
+          }
+        }
+      }
+    }
+    float t[kPU];
+#pragma unroll
+    for (int k = 0; k < kPU; ++k) t[k] = v[k];
+#pragma unroll
+    for (int k = 0; k < kTW - kPU; ++k) v[k] = v[k + kPU];
+#pragma unroll
+    for (int k = 0; k < kPU; ++k) v[kTW - kPU + k] = t[k];
+    rot += kPU;
+  }
+#pragma unroll
+  for (int k = 0; k < kTW; ++k) {
+    const int j = (rot + k) & (kTW - 1);
+    if (j < kb) Ap[(size_t)r + (size_t)j * ld] = v[k];
   }
 }
 
@@ -1264,10 +1726,21 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   constexpr int OB = kOuterBlocks * kSolveNB;  // outer block: K of the trailing update
   constexpr int PPB = OB / KB;                   // panels per outer block
   constexpr int LSLOT = 4 * kMaxPanel + 16;      // ints per interchange-list slot
+  // FP32 panels by tournament pivoting when asked for
+  const bool tslu = sizeof(T) == 4 && c->lu32_pivot == BEMHIP_PIVOT_TOURNAMENT;
+  size_t tnodes = 0;  // candidate-tree nodes of the largest panel
+  for (int s = Gmax;; s = (s + kTF - 1) / kTF) {
+    tnodes += (size_t)s;
+    if (s == 1) break;
+  }
+  const size_t tslu_bytes =
+      tslu ? 4 * tnodes * kTW * (size_t)kTW + 4 * tnodes * kTW + 4 * tnodes + 4 * (size_t)kTW * kTW +
+                 4 * 256
+           : 0;
   // work buffers
   const size_t need = 8 * 4 * (size_t)Gmax + 8 * 2 * (size_t)Gmax * RW + 8 * 2 * (size_t)RW +
                       sizeof(int) * ((size_t)PPB * LSLOT + 16) +
-                      sizeof(T) * 2 * kOuterBlocks * kSolveNB * kSolveNB + 8 * 256;
+                      sizeof(T) * 2 * kOuterBlocks * kSolveNB * kSolveNB + 8 * 256 + tslu_bytes;
   BEMHIP_HIP_TRY(c, c->lu_work.ensure(need));
   char* wp = c->lu_work.ptr;
   auto carve = [&](size_t bytes) {
@@ -1286,6 +1759,16 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   auto linv_of = [&](int k0) {
     return linv2 + (size_t)((k0 / OB) & 1) * kOuterBlocks * kSolveNB * kSolveNB;
   };
+  unsigned* t_cand = nullptr;
+  unsigned* t_cid = nullptr;
+  unsigned* t_count = nullptr;
+  float* t_top = nullptr;
+  if (tslu) {
+    t_cand = (unsigned*)carve(4 * tnodes * kTW * (size_t)kTW);
+    t_cid = (unsigned*)carve(4 * tnodes * kTW);
+    t_count = (unsigned*)carve(4 * tnodes);
+    t_top = (float*)carve(4 * (size_t)kTW * kTW);
+  }
   int* info = flags + 4;
   unsigned* abort_flag = (unsigned*)(flags + 8);
   auto l_dst = [&](int q) { return lists + (size_t)q * LSLOT; };
@@ -1296,6 +1779,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   hipStream_t sA = c->stream, sB = c->stream2;
   BEMHIP_HIP_TRY(c, hipMemsetAsync(flags, 0, sizeof(int) * 16, sA));
   BEMHIP_HIP_TRY(c, hipMemsetAsync(keys, 0, 8 * 4 * (size_t)Gmax, sA));
+  if (tslu) BEMHIP_HIP_TRY(c, hipMemsetAsync(t_count, 0, 4 * tnodes, sA));
   int key_buf = 0;  // key buffer of the next panel launch
   std::vector<GemmTiming> big_gemms;
   c->gemm_log = &big_gemms;
@@ -1355,29 +1839,56 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       const int m = n - kc;
       const int G = (m + kPR - 1) / kPR;
       hipEvent_t e0 = ev_now(st);
-      PanelArgs<T> pa;
-      pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = kc; pa.G = G; pa.gstride = Gmax;
-      pa.epoch = (++c->panel_epoch) & 0xffffffu;
-      pa.ipiv = dipiv; pa.info = info; pa.recs = recs; pa.rowc = rowc;
-      pa.keys = keys + (size_t)key_buf * 2 * Gmax;
-      pa.keys_next = keys + (size_t)(key_buf ^ 1) * 2 * Gmax;
-      key_buf ^= 1;
-      pa.abort_flag = abort_flag;
-      void* args[] = {&pa};
-      BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, KB>, dim3(G),
-                                                   dim3(kPR), args, 0, st));
-      seg_end(st, e0, kPanel);
-      e0 = ev_now(st);
-      hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, st, dipiv, kc, kb, l_dst(q), l_src(q),
-                         l_len(q));
-      const int ncols_blk = ob - kb;
-      if (ncols_blk > 0)
-        hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_blk + 3) / 4), dim3(256), 0, st, dA, lda,
-                           k0, kc, kc + kb, k0 + ob, (const int*)l_dst(q), (const int*)l_src(q),
-                           (const int*)l_len(q));
-      BEMHIP_HIP_TRY(c, hipGetLastError());
-      seg_end(st, e0, kSwap);
-      e0 = ev_now(st);
+      bool tournament = false;
+      if constexpr (sizeof(T) == 4) {
+        if (tslu) {
+          tournament = true;
+          // tournament: pivots and the winners' L11\U11, the interchanges on
+          // every column of the block (the panel's too), then L21
+          TsluArgs ta;
+          ta.A = (float*)dA; ta.lda = lda; ta.m = m; ta.kb = kb; ta.k0 = kc; ta.G = G;
+          ta.cand = t_cand; ta.cid = t_cid; ta.count = t_count; ta.top = t_top;
+          ta.ipiv = dipiv; ta.info = info; ta.abort_flag = abort_flag;
+          hipLaunchKernelGGL(tslu_kernel, dim3(G), dim3(kPR), 0, st, ta);
+          hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, st, dipiv, kc, kb, l_dst(q),
+                             l_src(q), l_len(q));
+          hipLaunchKernelGGL((perm_apply<T>), dim3((ob + 3) / 4), dim3(256), 0, st, dA, lda, k0,
+                             k0 + ob, 0, 0, (const int*)l_dst(q), (const int*)l_src(q),
+                             (const int*)l_len(q));
+          hipLaunchKernelGGL(tslu_apply,
+                             dim3(std::max(1, (m - kb + kApplyThreads - 1) / kApplyThreads)),
+                             dim3(kApplyThreads), 0, st, (float*)dA, lda, m, kb, kc,
+                             (const float*)t_top);
+          BEMHIP_HIP_TRY(c, hipGetLastError());
+          seg_end(st, e0, kPanel);
+          e0 = ev_now(st);
+        }
+      }
+      if (!tournament) {
+        PanelArgs<T> pa;
+        pa.A = dA; pa.lda = lda; pa.m = m; pa.kb = kb; pa.k0 = kc; pa.G = G; pa.gstride = Gmax;
+        pa.epoch = (++c->panel_epoch) & 0xffffffu;
+        pa.ipiv = dipiv; pa.info = info; pa.recs = recs; pa.rowc = rowc;
+        pa.keys = keys + (size_t)key_buf * 2 * Gmax;
+        pa.keys_next = keys + (size_t)(key_buf ^ 1) * 2 * Gmax;
+        key_buf ^= 1;
+        pa.abort_flag = abort_flag;
+        void* args[] = {&pa};
+        BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, KB>, dim3(G),
+                                                     dim3(kPR), args, 0, st));
+        seg_end(st, e0, kPanel);
+        e0 = ev_now(st);
+        hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, st, dipiv, kc, kb, l_dst(q), l_src(q),
+                           l_len(q));
+        const int ncols_blk = ob - kb;
+        if (ncols_blk > 0)
+          hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_blk + 3) / 4), dim3(256), 0, st, dA, lda,
+                             k0, kc, kc + kb, k0 + ob, (const int*)l_dst(q), (const int*)l_src(q),
+                             (const int*)l_len(q));
+        BEMHIP_HIP_TRY(c, hipGetLastError());
+        seg_end(st, e0, kSwap);
+        e0 = ev_now(st);
+      }
       // inverse of each completed 128-block of this outer block's L11
       if (aligned && ((s + kb) % kSolveNB == 0 || s + kb == ob)) {
         const int jb = (s + kb - 1) / kSolveNB;

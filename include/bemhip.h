@@ -216,8 +216,23 @@ int bemhip_solve_gmres(bemhip_ctx* ctx, double atol, double rtol, int restart, i
  * refinement residual; exposed for verification. */
 int bemhip_apply_operator(bemhip_ctx* ctx, const double* x, double* y);
 
-/* Standalone dense LU with partial pivoting of a host matrix (column-major
- * n x n), the LU seam of bem_linsolve (src/Core/LinearSolveUtils.jl:44-49).
+/* Pivoting of the FP32 LU (the factorization of BEMHIP_SOLVE_MIXED /
+ * MIXED_GMRES and of bemhip_lu_factor with dtype 1). No counterpart in the
+ * reference, whose LU is LAPACK's FP64 getrf (src/Core/LinearSolveUtils.jl:44-49).
+ *   BEMHIP_PIVOT_PARTIAL (default): GEPP, LAPACK's pivots (ties aside)
+ *   BEMHIP_PIVOT_TOURNAMENT: each 128-column panel chooses its pivot rows by
+ *     tournament pivoting (TSLU of CALU: GEPP on 512-row leaves, the leaves'
+ *     pivot rows reduced in a tree of GEPPs, no exchange between workgroups
+ *     per column); |L| entries may exceed 1
+ * The FP64 LU always pivots partially. Discards FP32 factors held by the
+ * context. */
+#define BEMHIP_PIVOT_PARTIAL 0
+#define BEMHIP_PIVOT_TOURNAMENT 1
+int bemhip_set_lu_pivoting(bemhip_ctx* ctx, int mode);
+
+/* Standalone dense LU of a host matrix (column-major n x n), the LU seam of
+ * bem_linsolve (src/Core/LinearSolveUtils.jl:44-49): partial pivoting in FP64,
+ * FP32 pivoting as set by bemhip_set_lu_pivoting.
  *   dtype 0: A is double*, 1: A is float*. A is overwritten by L\U (unit L)
  *   and ipiv (n, 0-based: row i was swapped with row ipiv[i]).
  * bemhip_lu_solve solves A x = b (trans = 0) or A^T x = b (trans = 1) with

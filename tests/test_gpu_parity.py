@@ -168,28 +168,94 @@ def test_reference_square_T_equals_x():
     assert B.rel_error(dad) < 0.05
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def _perm_of(piv):
+    perm = np.arange(len(piv))
+    for i in range(len(piv)):
+        perm[i], perm[piv[i]] = perm[piv[i]], perm[i]
+    return perm
+
+
+def _check_lu(A, LU, piv, dtype, lbound):
+    """P A = L U within FP backward error, |L| <= lbound, and the factors solve A x = b."""
+    n = A.shape[0]
+    assert np.all((piv >= np.arange(n)) & (piv < n))
+    L = np.tril(LU.astype(np.float64), -1) + np.eye(n)
+    U = np.triu(LU.astype(np.float64))
+    eps = np.finfo(dtype).eps
+    rec = np.linalg.norm(A.astype(np.float64)[_perm_of(piv)] - L @ U) / np.linalg.norm(A)
+    lmax = np.abs(np.tril(LU, -1)).max() if n > 1 else 0.0
+    assert rec < 50 * n ** 0.5 * eps, rec
+    assert lmax <= lbound, lmax
+    rng = np.random.default_rng(n + 1)
+    b = rng.standard_normal(n)
+    x = B.lu_solve(LU, piv, b)
+    assert np.linalg.norm(A.astype(np.float64) @ x - b) / np.linalg.norm(b) < 1e3 * n * eps
+    return rec, lmax
+
+
+@pytest.mark.parametrize("dtype,pivoting", [(np.float64, "partial"), (np.float32, "partial"),
+                                            (np.float32, "tournament")])
 # 6149: trailing updates large enough for the 256-wide GEMM tiles (>= 2 tiles per CU)
 @pytest.mark.parametrize("n", [1, 7, 130, 517, 1000, 6149])
-def test_lu_matches_lapack(n, dtype):
+def test_lu_matches_lapack(n, dtype, pivoting):
     import scipy.linalg as sl
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n)).astype(dtype)
-    LU, piv = B.lu_factor(A)
+    LU, piv = B.lu_factor(A, pivoting=pivoting)
     LUr, pivr = sl.lu_factor(A.astype(np.float64))
-    # same factorization up to rounding (pivot ties aside): reconstruct P A = L U
-    L = np.tril(LU.astype(np.float64), -1) + np.eye(n)
-    U = np.triu(LU.astype(np.float64))
-    perm = np.arange(n)
-    for i in range(n):
-        perm[i], perm[piv[i]] = perm[piv[i]], perm[i]
-    eps = np.finfo(dtype).eps
-    rec = np.linalg.norm(A.astype(np.float64)[perm] - L @ U) / np.linalg.norm(A)
-    print(f"\n[lu n={n} {dtype.__name__}] |PA-LU|/|A| = {rec:.2e}  pivots==LAPACK "
-          f"{np.array_equal(piv, pivr)}")
-    assert rec < 50 * n ** 0.5 * eps
-    assert np.abs(np.tril(LU, -1)).max() <= 1.0 + 1e-6
-    b = rng.standard_normal(n)
-    x = B.lu_solve(LU, piv, b)
-    xr = np.linalg.solve(A.astype(np.float64), b)
-    assert np.linalg.norm(A.astype(np.float64) @ x - b) / np.linalg.norm(b) < 1e3 * n * eps
+    # partial pivoting: |L| <= 1 (LAPACK's pivots, ties aside); tournament
+    # pivoting bounds |L| less tightly (a few units in practice)
+    rec, lmax = _check_lu(A, LU, piv, dtype, 1.0 + 1e-6 if pivoting == "partial" else 16.0)
+    print(f"\n[lu n={n} {dtype.__name__} {pivoting}] |PA-LU|/|A| = {rec:.2e} max|L| {lmax:.2f} "
+          f"pivots==LAPACK {np.array_equal(piv, pivr)}")
+
+
+@pytest.mark.parametrize("n", [100, 300, 512])
+def test_tournament_single_leaf_is_gepp(n):
+    """Panels of at most 512 rows are one leaf: tournament pivoting is then GEPP,
+    with the same arithmetic as the partial-pivoting panel, so pivots and
+    factors are identical."""
+    rng = np.random.default_rng(7 * n)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    ctx = B.HipContext(0)
+    LUt, pivt = B.lu_factor(A, ctx=ctx, pivoting="tournament")
+    LUp, pivp = B.lu_factor(A, ctx=ctx, pivoting="partial")
+    assert np.array_equal(pivt, pivp)
+    assert np.array_equal(LUt, LUp)
+
+
+@pytest.mark.parametrize("n", [2600, 8200])
+def test_tournament_tree(n):
+    """Several tree levels (2600 rows: 6 leaves, 2 levels; 8200: 17 leaves, 3
+    levels). The first panel's pivot rows are checked against a CPU replay of
+    the tournament (GEPP on each leaf's original rows, then on the stacked
+    candidates), and the factorization as a whole like LAPACK's."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    LU, piv = B.lu_factor(A, pivoting="tournament")
+    rec, lmax = _check_lu(A, LU, piv, np.float32, 16.0)
+    print(f"\n[tslu n={n}] |PA-LU|/|A| = {rec:.2e} max|L| {lmax:.2f}")
+    # CPU replay of the first panel's tournament in FP64 (the choices are
+    # decided by clear margins, so FP32 vs FP64 rounding does not move them)
+    P = A[:, :128].astype(np.float64)
+
+    def gepp_rows(rows):
+        M = P[rows].copy()
+        chosen, alive = [], np.ones(len(rows), bool)
+        for c in range(128):
+            a = np.where(alive, np.abs(M[:, c]), -1.0)
+            t = int(np.argmax(a))
+            chosen.append(rows[t])
+            alive[t] = False
+            if M[t, c] != 0:
+                l = M[:, c] / M[t, c]
+                l[~alive] = 0.0
+                M -= np.outer(l, M[t])
+        return chosen
+
+    sets = [gepp_rows(list(range(s, min(s + 512, n)))) for s in range(0, n, 512)]
+    while len(sets) > 1:
+        sets = [gepp_rows(sum(sets[i:i + 4], [])) for i in range(0, len(sets), 4)]
+    winners = sets[0]
+    perm = _perm_of(piv)
+    assert list(perm[:128]) == winners
