@@ -725,16 +725,14 @@ int assemble_impl(bemhip_ctx* c) {
                        (int)nt, 1.0, c->wH.ptr, 0.0, c->wG.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
   } else if (fp64) {
-    c->H32.release(); c->G32.release();
-    BEMHIP_HIP_TRY(c, c->H64.ensure(c->ld64 * nt));
-    BEMHIP_HIP_TRY(c, c->G64.ensure(c->ld64 * (size_t)c->n));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->H64, c->ld64 * nt));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->G64, c->ld64 * (size_t)c->n, &c->H64));
     if (c->ni > 0)  // columns of the internal nodes: zero but the diagonal
       BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + (size_t)c->n * c->ld64, 0,
                                        sizeof(double) * c->ld64 * c->ni, c->stream));
   } else {
-    c->H64.release(); c->G64.release();
-    BEMHIP_HIP_TRY(c, c->H32.ensure(c->ld32 * nt));
-    BEMHIP_HIP_TRY(c, c->G32.ensure(c->ld32 * (size_t)c->n));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->H32, c->ld32 * nt));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->G32, c->ld32 * (size_t)c->n, &c->H32));
     if (c->ni > 0)
       BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + (size_t)c->n * c->ld32, 0,
                                        sizeof(float) * c->ld32 * c->ni, c->stream));
@@ -995,13 +993,11 @@ int apply_bc_impl(bemhip_ctx* c) {
     ScopedTimer t(c, "apply_bc_swap");
     dim3 grid((unsigned)std::min<size_t>((nt + 255) / 256, 64), (unsigned)nt);
     if (c->precision == BEMHIP_FP64) {
-      c->A32.release();
-      BEMHIP_HIP_TRY(c, c->A64.ensure(c->ld64 * nt));
+      BEMHIP_HIP_TRY(c, ensure_matrix(c, c->A64, c->ld64 * nt, &c->H64, &c->G64));
       hipLaunchKernelGGL((bc_swap_kernel<double>), grid, dim3(256), 0, c->stream, c->nt, c->n,
                          c->ld64, c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A64.ptr);
     } else {
-      c->A64.release();
-      BEMHIP_HIP_TRY(c, c->A32.ensure(c->ld32 * nt));
+      BEMHIP_HIP_TRY(c, ensure_matrix(c, c->A32, c->ld32 * nt, &c->H32, &c->G32));
       hipLaunchKernelGGL((bc_swap_kernel<float>), grid, dim3(256), 0, c->stream, c->nt, c->n,
                          c->ld32, c->d_bc.ptr, c->H32.ptr, c->G32.ptr, c->A32.ptr);
     }

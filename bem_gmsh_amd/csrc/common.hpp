@@ -201,6 +201,31 @@ struct bemhip_ctx {
 
 namespace bemhip {
 
+// System matrices (H, G, A in either precision) stay allocated between calls,
+// so a caller that alternates precisions or problem sizes does not re-allocate
+// them every time. ensure_matrix(c, buf, n, keep...) ensures one of them; on an
+// out-of-memory error it frees the others (except `buf` and the `keep` ones)
+// and tries again.
+inline void release_spare_matrices(bemhip_ctx* c, const void* k0, const void* k1, const void* k2,
+                                   const void* k3) {
+  auto rel = [&](auto& b) {
+    const void* p = &b;
+    if (p != k0 && p != k1 && p != k2 && p != k3) b.release();
+  };
+  rel(c->H64); rel(c->G64); rel(c->H32); rel(c->G32); rel(c->A64); rel(c->A32);
+}
+template <typename T>
+hipError_t ensure_matrix(bemhip_ctx* c, DevBuf<T>& buf, size_t n, const void* keep1 = nullptr,
+                         const void* keep2 = nullptr, const void* keep3 = nullptr) {
+  hipError_t e = buf.ensure(n);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    release_spare_matrices(c, &buf, keep1, keep2, keep3);
+    e = buf.ensure(n);
+  }
+  return e;
+}
+
 // Error helpers ---------------------------------------------------------------
 void set_global_error(const std::string& s);
 int fail(bemhip_ctx* c, int code, const std::string& msg);

@@ -790,13 +790,13 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);
   const bool fp64 = (c->precision == BEMHIP_FP64);
   const bool mfree = (c->precision == BEMHIP_MATRIX_FREE);
-  // the other storage's matrices go; the current storage's A stays allocated
-  // for the next system of the same size
-  if (fp64) { c->H32.release(); c->G32.release(); c->A32.release(); }
-  else if (mfree) {
+  // a matrix-free assembly stores no matrix: every stored one goes; the stored
+  // modes keep the other storage's matrices until memory runs short
+  // (ensure_matrix)
+  if (mfree) {
     c->H64.release(); c->G64.release(); c->A64.release();
     c->H32.release(); c->G32.release(); c->A32.release();
-  } else { c->H64.release(); c->G64.release(); c->A64.release(); }
+  }
   BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
   BEMHIP_HIP_TRY(c, c->diag64.ensure(4 * nt));
   BEMHIP_HIP_TRY(c, c->hpart.ensure(4 * (size_t)ncb * nt));
@@ -807,14 +807,14 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   if (mfree) {
     // no stored H, G: the far pass below computes the row sums only
   } else if (fp64) {
-    BEMHIP_HIP_TRY(c, c->H64.ensure(c->ld64 * ns));
-    BEMHIP_HIP_TRY(c, c->G64.ensure(c->ld64 * nb));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->H64, c->ld64 * ns));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->G64, c->ld64 * nb, &c->H64));
     if (c->ni > 0)  // columns of the internal nodes: zero but the diagonal blocks
       BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H64.ptr + nb * c->ld64, 0,
                                        sizeof(double) * c->ld64 * (ns - nb), c->stream));
   } else {
-    BEMHIP_HIP_TRY(c, c->H32.ensure(c->ld32 * ns));
-    BEMHIP_HIP_TRY(c, c->G32.ensure(c->ld32 * nb));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->H32, c->ld32 * ns));
+    BEMHIP_HIP_TRY(c, ensure_matrix(c, c->G32, c->ld32 * nb, &c->H32));
     if (c->ni > 0)
       BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + nb * c->ld32, 0,
                                        sizeof(float) * c->ld32 * (ns - nb), c->stream));
@@ -988,11 +988,11 @@ int kelvin_apply_bc_impl(bemhip_ctx* c) {
     ScopedTimer t(c, "apply_bc_swap");
     dim3 grid((unsigned)std::min((ns + 255) / 256, 64), (unsigned)std::min(ns, 32768));
     if (fp64) {
-      BEMHIP_HIP_TRY(c, c->A64.ensure(c->ld64 * (size_t)ns));
+      BEMHIP_HIP_TRY(c, ensure_matrix(c, c->A64, c->ld64 * (size_t)ns, &c->H64, &c->G64));
       hipLaunchKernelGGL((kelvin_swap_kernel<double>), grid, dim3(256), 0, c->stream, ns, nb,
                          c->ld64, c->d_bc.ptr, c->H64.ptr, c->G64.ptr, c->A64.ptr);
     } else {
-      BEMHIP_HIP_TRY(c, c->A32.ensure(c->ld32 * (size_t)ns));
+      BEMHIP_HIP_TRY(c, ensure_matrix(c, c->A32, c->ld32 * (size_t)ns, &c->H32, &c->G32));
       hipLaunchKernelGGL((kelvin_swap_kernel<float>), grid, dim3(256), 0, c->stream, ns, nb,
                          c->ld32, c->d_bc.ptr, c->H32.ptr, c->G32.ptr, c->A32.ptr);
     }

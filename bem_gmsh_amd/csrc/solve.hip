@@ -201,7 +201,7 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     if (!c->factored32) {
       if (c->precision == BEMHIP_FP64) {
         // FP32 copy of the FP64-assembled system (from H, G once A64 is factored)
-        BEMHIP_HIP_TRY(c, c->A32.ensure(c->ld32 * nt));
+        BEMHIP_HIP_TRY(c, ensure_matrix(c, c->A32, c->ld32 * nt, &c->H64, &c->G64, &c->A64));
         const dim3 grid((unsigned)std::min((nt + 255) / 256, 64), (unsigned)std::min(nt, 32768));
         if (c->factored64)
           hipLaunchKernelGGL(swap_round_kernel, grid, dim3(256), 0, c->stream, nt, c->nbnd(), c->ld64,
