@@ -479,6 +479,58 @@ def gmres_large(spec, npg, far="dense", rtol=1e-10, restart=100, itmax=4000):
     print(json.dumps(out))
 
 
+def getrf_yardstick(n: int):
+    """Same-box yardstick for the LU, outside the library: rocSOLVER's
+    rocsolver_sgetrf / rocsolver_dgetrf (blocked LU with partial pivoting) on a
+    random n x n matrix already on the GPU (torch allocates it), timed with
+    events on the stream the handle uses, beside this library's FP32 / FP64 LU
+    of the same matrix (device time of the factorization alone)."""
+    import torch
+    roc = C.CDLL("/opt/rocm/lib/librocsolver.so.0")
+    rb = C.CDLL("/opt/rocm/lib/librocblas.so.5")
+    handle = C.c_void_p()
+    assert rb.rocblas_create_handle(C.byref(handle)) == 0
+    stream = torch.cuda.current_stream()
+    rb.rocblas_set_stream(handle, C.c_void_p(stream.cuda_stream))
+    out = {"yardstick": "rocsolver getrf vs this library's LU, random normal matrix",
+           "n": n}
+    g = torch.Generator(device="cuda").manual_seed(0)
+    for dt, fn, key in ((torch.float32, roc.rocsolver_sgetrf, "s"),
+                        (torch.float64, roc.rocsolver_dgetrf, "d")):
+        A0 = torch.randn((n, n), dtype=dt, device="cuda", generator=g)
+        ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+        times = []
+        for rep in range(2):  # the first call includes rocSOLVER's set-up
+            A = A0.clone()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            st = fn(handle, n, n, C.c_void_p(A.data_ptr()), n, C.c_void_p(ipiv.data_ptr()),
+                    C.c_void_p(info.data_ptr()))
+            e1.record(stream)
+            torch.cuda.synchronize()
+            assert st == 0 and int(info.item()) == 0, (st, int(info.item()))
+            times.append(e0.elapsed_time(e1))
+            del A
+        out[f"rocsolver_{key}getrf_ms"] = times[-1]
+        # this library's LU of the same matrix (host round trip; device time read back)
+        Ah = np.asfortranarray(A0.cpu().numpy())
+        del A0
+        torch.cuda.empty_cache()
+        ctx = B.HipContext(0)
+        B.lu_factor(Ah, ctx=ctx)
+        ms = ctx.counters()["ms"]
+        ctx.close()
+        lk = "lu32" if key == "s" else "lu64"
+        out[f"bemhip_{lk}_ms"] = ms[lk]
+        flops = 2.0 / 3.0 * float(n) ** 3
+        out[f"tflops_rocsolver_{key}getrf"] = flops / (times[-1] * 1e-3) / 1e12
+        out[f"tflops_bemhip_{lk}"] = flops / (ms[lk] * 1e-3) / 1e12
+        log(f"[bench] {key}getrf {times[-1]:.1f} ms vs {lk} {ms[lk]:.1f} ms")
+    rb.rocblas_destroy_handle(handle)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -502,6 +554,8 @@ def main():
     ap.add_argument("--phase", choices=["all", "assemble"], default="all",
                     help="'assemble': only assembly + BC passes (for counter runs)")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--getrf-yardstick", action="store_true",
+                    help="only: rocSOLVER sgetrf / dgetrf vs this library's LU at n = N_eff")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-rows", type=int, default=16384, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-lu-n", type=int, default=10240, help=argparse.SUPPRESS)
@@ -516,6 +570,9 @@ def main():
         return
     if args.gmres_large:
         gmres_large(args.gmres_large, args.npg, far=args.far)
+        return
+    if args.getrf_yardstick:
+        print(json.dumps(getrf_yardstick(3 * args.ne)))
         return
     if args.elastic_only:
         print(json.dumps({"elastic": elastic_leg(npg=args.npg, cpu_rows=0)}))
@@ -589,6 +646,22 @@ def main():
             "fp64_relres": rel64,
             "fp64_entries_per_s": entries / (c64["assemble"] * 1e-3),
         })
+        st64 = ctx.counters()["stats"]
+        n64 = int(st64.get("lu64_big_gemm_launches", 0))
+        if n64 and c64.get("lu64_big_gemm"):
+            ach64 = st64["lu64_big_gemm_flops"] / (c64["lu64_big_gemm"] * 1e-3)
+            detail["roofline_lu64"] = {
+                "kernel": "gemm_tile_kernel<double, 256, 128, 4, 2, false> (FP64 LU trailing update)",
+                "bound": "mfma", "achieved": ach64 / 1e12, "peak": PEAK_MFMA_F64 / 1e12,
+                "unit": "TFLOP/s", "frac": ach64 / PEAK_MFMA_F64, "launches": n64,
+                "avg_launch_us": 1e3 * c64["lu64_big_gemm"] / n64,
+                "flops_per_launch": st64["lu64_big_gemm_flops"] / n64}
+        lu_flops = 2.0 / 3.0 * float(nt) ** 3
+        detail["lu_rates"] = {
+            "lu32_tflops": lu_flops / (float(np.mean(lu_ms)) * 1e-3) / 1e12,
+            "lu64_tflops": lu_flops / (c64["lu64"] * 1e-3) / 1e12,
+            "lu32_frac_of_fp32_peak": lu_flops / (float(np.mean(lu_ms)) * 1e-3) / PEAK_MFMA_F32,
+            "lu64_frac_of_fp64_peak": lu_flops / (c64["lu64"] * 1e-3) / PEAK_MFMA_F64}
         T64, q64 = B.split_sol(dad, x64)
         Tm, qm = B.split_sol(dad, x_mixed)
         eT = float(np.linalg.norm(Tm - T64) / np.linalg.norm(T64))
