@@ -1134,8 +1134,9 @@ __global__ __launch_bounds__(256, 2) void gemm_kernel(int M, int N, int K, const
 // identically for A and B, so each product term appears once.
 //
 // Tile order: workgroups are dealt to the 8 XCDs round-robin; block b is
-// mapped to tile xcd_chunk(b) so each XCD walks a contiguous range of
-// column-major tiles (shared B columns and neighbouring A rows in its L2).
+// mapped to tile xcd_chunk(b) so each XCD walks a contiguous range of the
+// linear tile order, which runs row by row through groups of 4 tile columns
+// (A and B blocks shared in the XCD's L2).
 // ---------------------------------------------------------------------------
 // SET = false: C -= A B. SET = true: C = A B, where B may be C itself when
 // one tile row covers all M rows (each workgroup reads its whole column strip
@@ -1164,7 +1165,17 @@ __global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
     const int q = ntiles >> 3, rem = ntiles & 7, x = bid & 7, j = bid >> 3;
     tile = (x < rem) ? x * (q + 1) + j : rem * (q + 1) + (x - rem) * q + j;
   }
-  const int bm = (tile % tiles_m) * BM, bn = (tile / tiles_m) * BN;
+  // Linear tile order in groups of kGroupN tile columns, row by row within a
+  // group: the ~32 tiles an XCD runs at once cover 8 tile rows x 4 tile
+  // columns, so its L2 serves each A block to 4 tiles and each B block to 8
+  // (a single tile column would re-read every A block from HBM).
+  constexpr int kGroupN = 4;
+  const int tiles_n = (ntiles + tiles_m - 1) / tiles_m;
+  const int group = tile / (tiles_m * kGroupN);
+  const int first_n = group * kGroupN;
+  const int gw = min(kGroupN, tiles_n - first_n);
+  const int gidx = tile - group * tiles_m * kGroupN;
+  const int bm = (gidx / gw) * BM, bn = (first_n + gidx % gw) * BN;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = (wave % NWM) * WTM, wn = (wave / NWM) * WTN;
   const int l16 = lane & 15, q4 = lane >> 4;
