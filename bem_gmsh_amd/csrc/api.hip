@@ -26,6 +26,27 @@ int fail(bemhip_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
+int check_matrix_free_scratch(bemhip_ctx* c, const char* who, size_t per_pair_block, size_t ncb,
+                              size_t rows) {
+  const double need = (double)per_pair_block * (double)ncb * (double)rows;
+  size_t free_b = 0, total_b = 0;
+  BEMHIP_HIP_TRY(c, hipMemGetInfo(&free_b, &total_b));
+  // buffers this context already holds count as available
+  const double held = (double)c->nearbits.count * sizeof(uint32_t) +
+                      (double)c->hpart.count * sizeof(double) +
+                      (double)c->part1.count * sizeof(double);
+  if (need > 0.9 * ((double)free_b + held)) {
+    char msg[512];
+    snprintf(msg, sizeof(msg),
+             "%s: the matrix-free assembly keeps %zu bytes per (row, 32-element block) of near "
+             "bits and FP64 partial sums, %.1f GB for %zu rows x %zu blocks, above the %.1f GB "
+             "the device has free; BEMHIP_FMM assembles in O(N) memory",
+             who, per_pair_block, need * 1e-9, rows, ncb, (0.9 * ((double)free_b + held)) * 1e-9);
+    return fail(c, BEMHIP_ERR_NOMEM, msg);
+  }
+  return BEMHIP_OK;
+}
+
 // Gauss-Legendre rule: the n roots of P_n by Newton on the three-term
 // recurrence, weights 2 / ((1 - x^2) P_n'(x)^2), ascending order (the rule
 // FastGaussQuadrature.gausslegendre returns; src/Core/Assembly_full.jl:719-723).

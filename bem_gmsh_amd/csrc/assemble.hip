@@ -707,6 +707,14 @@ int assemble_impl(bemhip_ctx* c) {
     const int st0 = fmm_build(c);
     if (st0) return st0;
   }
+  if (mfree && !fmm) {
+    // near bits (4 B), FP64 far row sums (8 B) and the products' partial sums
+    // (8 B) per (row, 32-element column block): refuse what does not fit
+    c->H64.release(); c->G64.release(); c->H32.release(); c->G32.release();
+    c->A64.release(); c->A32.release();
+    const int st0 = check_matrix_free_scratch(c, "assemble", 4 + 8 + 8, (size_t)ncb, nt);
+    if (st0) return st0;
+  }
   if (fmm)  // near pairs from the tree's candidates: no near bits
     c->nearbits.release();
   else

@@ -226,6 +226,12 @@ hipError_t ensure_matrix(bemhip_ctx* c, DevBuf<T>& buf, size_t n, const void* ke
   return e;
 }
 
+// Scratch check of the dense matrix-free modes, whose column-block partials
+// and near bits take `per_pair_block` bytes per (row, 32-element column block):
+// O(nt^2 / 32), unlike the O(nt) of the FMM mode.
+int check_matrix_free_scratch(bemhip_ctx* c, const char* who, size_t per_pair_block, size_t ncb,
+                              size_t rows);
+
 // Error helpers ---------------------------------------------------------------
 void set_global_error(const std::string& s);
 int fail(bemhip_ctx* c, int code, const std::string& msg);

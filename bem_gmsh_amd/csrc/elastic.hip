@@ -796,6 +796,11 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   if (mfree) {
     c->H64.release(); c->G64.release(); c->A64.release();
     c->H32.release(); c->G32.release(); c->A32.release();
+    // near bits (4 B), the four FP64 far row-sum components (32 B) and the
+    // products' two partial sums (16 B) per (row node, 32-element column block)
+    const int st0 = check_matrix_free_scratch(c, "assemble (elasticity)", 4 + 32 + 16, (size_t)ncb,
+                                              nt);
+    if (st0) return st0;
   }
   BEMHIP_HIP_TRY(c, c->nearbits.ensure((size_t)ncb * nt));
   BEMHIP_HIP_TRY(c, c->diag64.ensure(4 * nt));

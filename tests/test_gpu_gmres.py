@@ -94,3 +94,46 @@ def test_matrix_free_errors():
     assert not st["solved"] and st["niter"] == 3
     with pytest.raises(ValueError):
         B.solve_Hmat(dad, Pl=np.eye(2))
+
+
+def _circle_geometry(ne):
+    """Discontinuous linear elements on the unit circle, packed for
+    bemhip_set_geometry (numpy only, so a million elements build in seconds)."""
+    t = np.linspace(0.0, 2.0 * np.pi, ne + 1)
+    p0 = np.stack([np.cos(t[:-1]), np.sin(t[:-1])], axis=1)
+    p1 = np.stack([np.cos(t[1:]), np.sin(t[1:])], axis=1)
+    geo = np.ascontiguousarray(np.stack([p0, p1], axis=1))            # ne x 2 x 2
+    xi = np.array([-1.0, 1.0]) / np.sqrt(3.0)                          # 2-point GL nodes
+    coll = (0.5 * (1 - xi)[None, :, None] * p0[:, None, :] +
+            0.5 * (1 + xi)[None, :, None] * p1[:, None, :]).reshape(-1, 2)
+    L = np.linalg.norm(p1 - p0, axis=1)
+    tang = (p1 - p0) / L[:, None]
+    nrm = np.repeat(np.stack([tang[:, 1], -tang[:, 0]], axis=1), 2, axis=0)
+    J = np.repeat(0.5 * L, 2)
+    return (np.ascontiguousarray(coll), np.ascontiguousarray(nrm), J, np.array([1.0, 1.0]),
+            geo, L)
+
+
+def test_matrix_free_refuses_scratch_beyond_device_memory():
+    """The dense matrix-free mode keeps 20 bytes per (row, 32-element column
+    block): with 1.2e6 elements (2.4e6 rows) that is ~1.8 TB, far above one
+    GPU. The assembly refuses before allocating, with BEMHIP_ERR_NOMEM and a
+    message naming the O(N) FMM mode; the context stays usable."""
+    from bem_gmsh_amd._lib import lib, dptr
+    L = lib()
+    ctx = B.HipContext(0)
+    ne = 1_200_000
+    coll, nrm, J, w, geo, elen = _circle_geometry(ne)
+    n = 2 * ne
+    assert L.bemhip_set_geometry(ctx.h, 1, ne, n, 0, dptr(coll), dptr(nrm), dptr(J), dptr(w),
+                                 dptr(geo), dptr(elen), 1.0) == 0
+    st = L.bemhip_assemble(ctx.h, 4, 1.5, 2)  # BEMHIP_MATRIX_FREE
+    msg = L.bemhip_last_error(ctx.h).decode()
+    print("\n", st, msg)
+    assert st == 6 and "BEMHIP_FMM" in msg  # BEMHIP_ERR_NOMEM
+    # a small problem on the same context still assembles
+    coll, nrm, J, w, geo, elen = _circle_geometry(64)
+    assert L.bemhip_set_geometry(ctx.h, 1, 64, 128, 0, dptr(coll), dptr(nrm), dptr(J), dptr(w),
+                                 dptr(geo), dptr(elen), 1.0) == 0
+    assert L.bemhip_assemble(ctx.h, 4, 1.5, 2) == 0
+    ctx.close()
