@@ -8,7 +8,7 @@
 //   panel_kernel   cooperative launch, one workgroup per 64 panel rows, the
 //                  panel held in registers; one grid barrier per column for the
 //                  partial-pivot search (GEPP, ties -> lowest row).
-//   perm_build / perm_apply   the panel's kb interchanges turned into one gather
+//   build_gather_list / perm_apply   the panel's kb interchanges turned into one gather
 //                  list, applied to every column outside the panel (a wave per
 //                  column, coalesced over the contiguous destination rows).
 //   trsm_rec       U12 = L11^-1 A12, recursive halving: 32-row diagonal blocks
@@ -95,6 +95,9 @@ struct PanelArgs {
   unsigned long long* recs;   // [2][gstride][RW] candidate rows (tagged words)
   unsigned long long* rowc;   // [2][RW] row c before the interchange (tagged words)
   unsigned* abort_flag;       // raised by a workgroup whose wait ran out
+  int* list_dst;              // the panel's interchanges as a gather list, built at the
+  int* list_src;              // end by workgroup 0 (kb <= 128; list_len == nullptr: not built)
+  int* list_len;
 };
 
 __device__ __forceinline__ void st_ag(unsigned long long* p, unsigned long long v) {
@@ -184,6 +187,85 @@ __device__ bool read_record(const unsigned long long* w, unsigned tag, int kb, i
   }
 }
 
+// The gather list of perm_build for panels of at most 128 columns, with the
+// position -> content map in the wave's registers (lane l holds entries l, l + 64, l + 128,
+// l + 192: the kb panel positions, then the rows from outside the panel in the
+// order met) and the kb transpositions replayed with lane reads: no LDS round
+// trip per interchange.
+__device__ __forceinline__ int sel4(const int (&a)[4], int q) {
+  return q == 0 ? a[0] : (q == 1 ? a[1] : (q == 2 ? a[2] : a[3]));
+}
+// One full wave; lane l passes the pivots of columns l and l + 64 (piv0, piv1).
+__device__ void build_gather_list(int piv0, int piv1, int k0, int kb, int* __restrict__ list_dst,
+                                  int* __restrict__ list_src, int* __restrict__ list_len) {
+  const int lane = threadIdx.x & 63;
+  int pos[4], con[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int t = lane + 64 * q;
+    pos[q] = con[q] = (t < kb) ? k0 + t : -1;
+  }
+  int nout = 0;
+  for (int s = 0; s < kb; ++s) {
+    const int r = __builtin_amdgcn_readlane(s < 64 ? piv0 : piv1, s & 63);
+    int idx;
+    if (r < k0 + kb) {
+      idx = r - k0;
+    } else {
+      // the outside row's entry: the one already met, else a new one
+      int found = INT_MAX;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t = lane + 64 * q;
+        if (t >= kb && t < kb + nout && pos[q] == r) found = min(found, t);
+      }
+      const unsigned long long hit = __ballot(found != INT_MAX);
+      if (hit) {
+        idx = __builtin_amdgcn_readlane(found, (int)__ffsll((long long)hit) - 1);
+      } else {
+        idx = kb + nout;
+        ++nout;
+        if (lane == (idx & 63)) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (q == (idx >> 6)) { pos[q] = r; con[q] = r; }
+        }
+      }
+    }
+    if (idx != s) {
+      const int cs = __builtin_amdgcn_readlane(sel4(con, s >> 6), s & 63);
+      const int ci = __builtin_amdgcn_readlane(sel4(con, idx >> 6), idx & 63);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (lane == (s & 63) && q == (s >> 6)) con[q] = ci;
+        if (lane == (idx & 63) && q == (idx >> 6)) con[q] = cs;
+      }
+    }
+  }
+  // compact the entries that moved
+  const int total = kb + nout;
+  int base = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int t = lane + 64 * q;
+    const bool mv = (t < total) && pos[q] != con[q];
+    const unsigned long long m = __ballot(mv);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (mv) { list_dst[base + before] = pos[q]; list_src[base + before] = con[q]; }
+    base += __popcll(m);
+  }
+  if (lane == 0) *list_len = base;
+}
+
+__global__ __launch_bounds__(64) void perm_build_fast(const int* __restrict__ ipiv, int k0, int kb,
+                                                      int* __restrict__ list_dst,
+                                                      int* __restrict__ list_src,
+                                                      int* __restrict__ list_len) {
+  const int lane = threadIdx.x;
+  build_gather_list(lane < kb ? ipiv[k0 + lane] : 0, lane + 64 < kb ? ipiv[k0 + 64 + lane] : 0, k0,
+                    kb, list_dst, list_src, list_len);
+}
+
 template <typename T, int W>
 __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
   using TW = TagWords<T>;
@@ -196,6 +278,7 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
   __shared__ T s_wval[kPW];
   __shared__ int s_wrowi[kPW];
   __shared__ int s_p, s_ok, s_abort;
+  __shared__ int s_piv[W];  // workgroup 0: ipiv of the panel's columns
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int blk = blockIdx.x;
   const int rbase = blk * kPR;
@@ -378,6 +461,7 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
           s_abort = aborted ? 1 : 0;
           if (!aborted && blk == 0) {
             a.ipiv[a.k0 + c] = a.k0 + (nz ? p : c);
+            s_piv[c] = a.k0 + (nz ? p : c);
             if (!nz) atomicCAS(a.info, 0, a.k0 + c + 1);
           }
         }
@@ -432,6 +516,16 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
   for (int k = 0; k < W; ++k) {
     const int j = (rot + k) % W;
     if (valid && j < kb) Ap[(size_t)r + (size_t)j * lda] = v[k];
+  }
+  // workgroup 0: the panel's interchanges as a gather list (every s_piv entry
+  // was written before one of the loop's barriers); none after an abort
+  if (blk == 0 && w == 0 && a.list_len) {
+    if (stop) {
+      if (lane == 0) *a.list_len = 0;
+    } else {
+      build_gather_list(lane < kb ? s_piv[lane] : 0, lane + 64 < kb ? s_piv[lane + 64] : 0,
+                        a.k0, kb, a.list_dst, a.list_src, a.list_len);
+    }
   }
 }
 
@@ -881,68 +975,11 @@ __global__ __launch_bounds__(kApplyThreads) void tslu_apply(float* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Row interchanges of one panel as a gather list (dst[t] <- src[t]).
-// One wavefront; the kb transpositions are replayed on a position->content map.
+// Row interchanges of one panel as a gather list (dst[t] <- src[t]): built by
+// build_gather_list (above), at the end of the GEPP panel kernel or by
+// perm_build_fast after a tournament panel.
 // ---------------------------------------------------------------------------
 constexpr int kMaxPanel = 256;
-
-__global__ __launch_bounds__(64) void perm_build(const int* __restrict__ ipiv, int k0, int kb,
-                                                 int* __restrict__ list_dst,
-                                                 int* __restrict__ list_src,
-                                                 int* __restrict__ list_len) {
-  __shared__ int pos[2 * kMaxPanel];
-  __shared__ int content[2 * kMaxPanel];
-  __shared__ int piv[kMaxPanel];
-  __shared__ int s_nout;
-  const int lane = threadIdx.x;
-  for (int t = lane; t < kb; t += 64) {
-    pos[t] = k0 + t;
-    content[t] = k0 + t;
-    piv[t] = ipiv[k0 + t];
-  }
-  if (lane == 0) s_nout = 0;
-  __syncthreads();
-  for (int s = 0; s < kb; ++s) {
-    const int r = piv[s];
-    int idx;
-    if (r < k0 + kb) {
-      idx = r - k0;
-    } else {
-      const int nout = s_nout;
-      int found = -1;
-      for (int t0 = 0; t0 < nout; t0 += 64) {
-        const int t = t0 + lane;
-        const bool hit = (t < nout) && pos[kb + t] == r;
-        const uint64_t m = __ballot(hit);
-        if (m) { found = t0 + __ffsll((unsigned long long)m) - 1; break; }
-      }
-      if (found < 0) {
-        if (lane == 0) { pos[kb + nout] = r; content[kb + nout] = r; s_nout = nout + 1; }
-        found = nout;
-      }
-      idx = kb + found;
-    }
-    __syncthreads();
-    if (lane == 0 && idx != s) {
-      const int tmp = content[s];
-      content[s] = content[idx];
-      content[idx] = tmp;
-    }
-    __syncthreads();
-  }
-  const int total = kb + s_nout;
-  // compact the entries that moved
-  int base = 0;
-  for (int t0 = 0; t0 < total; t0 += 64) {
-    const int t = t0 + lane;
-    const bool mv = (t < total) && pos[t] != content[t];
-    const uint64_t m = __ballot(mv);
-    const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (mv) { list_dst[base + before] = pos[t]; list_src[base + before] = content[t]; }
-    base += __popcll(m);
-  }
-  if (lane == 0) *list_len = base;
-}
 
 // Apply the gather list to columns [c0, c1) and [c2, c3): one wave per column.
 template <typename T>
@@ -1434,6 +1471,73 @@ __global__ __launch_bounds__(kTriInvThreads) void tri_inv_kernel(int n, const T*
     if (i < nb) D[i + (size_t)(cg0 + cc) * NB] = (cg0 + cc < nb) ? Xt[cc * XP + i] : T(0);
   }
 }
+// X = L^-1 for the unit lower nb x nb FP32 block (nb <= 128) at F, written to
+// D (column-major NB x NB: zero above the diagonal and outside nb), by one
+// workgroup: the eight 16 x 16 diagonal blocks of X by forward substitution
+// (a thread per column, the column in registers), then block rows i = 1..7 of
+// X from X_ij = -X_ii sum_{j <= k < i} L_ik X_kj (two passes per block row,
+// the block row of L staged in LDS). Sums in FP64, entries rounded to FP32 as
+// they are stored, as tri_inv_kernel does; 15 barriers instead of a 128-row
+// chain per column.
+constexpr int kTiB = 16;
+__global__ __launch_bounds__(512) void tri_inv_lower_blocked(int nb, const float* __restrict__ F,
+                                                             int lda, float* __restrict__ D) {
+  constexpr int NB = kSolveNB;
+  constexpr int NBB = NB / kTiB;
+  __shared__ float Xs[NB][NB + 1];                  // X(i, c)
+  __shared__ float Lr[kTiB][NB + 1];                // block row of L (diagonal blocks first)
+  __shared__ double Ss[NBB - 1][kTiB][kTiB + 1];    // S_ij of the current block row
+  const int tid = threadIdx.x;
+  auto Lg = [&](int i, int k) -> float {  // L(i, k), identity outside nb
+    return (i < nb && k < nb) ? F[(size_t)i + (size_t)k * lda] : (i == k ? 1.f : 0.f);
+  };
+  // the eight diagonal blocks, block bb in Lr[r][bb * 16 + k]
+  for (int e = tid; e < NBB * kTiB * kTiB; e += 512) {
+    const int bb = e / (kTiB * kTiB), k = (e / kTiB) % kTiB, r = e % kTiB;  // coalesced over r
+    Lr[r][bb * kTiB + k] = Lg(bb * kTiB + r, bb * kTiB + k);
+  }
+  __syncthreads();
+  if (tid < NB) {
+    const int bb = tid / kTiB, cc = tid % kTiB, r0 = bb * kTiB;
+    double x[kTiB];
+#pragma unroll
+    for (int r = 0; r < kTiB; ++r) {
+      double acc = (r == cc) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < r; ++k) acc -= (double)Lr[r][r0 + k] * x[k];
+      x[r] = (double)(float)acc;  // x[k] = 0 above the diagonal, so the sum starts at cc
+      Xs[r0 + r][r0 + cc] = (float)acc;
+    }
+  }
+  __syncthreads();
+  for (int ib = 1; ib < NBB; ++ib) {
+    const int i0 = ib * kTiB;
+    for (int e = tid; e < kTiB * i0; e += 512) {
+      const int k = e / kTiB, r = e % kTiB;  // coalesced over the block row's 16 rows
+      Lr[r][k] = Lg(i0 + r, k);
+    }
+    __syncthreads();
+    for (int e = tid; e < ib * kTiB * kTiB; e += 512) {
+      const int j = e / (kTiB * kTiB), r = (e / kTiB) % kTiB, cc = e % kTiB;
+      double acc = 0.0;
+      for (int k = j * kTiB; k < i0; ++k) acc += (double)Lr[r][k] * (double)Xs[k][j * kTiB + cc];
+      Ss[j][r][cc] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < ib * kTiB * kTiB; e += 512) {
+      const int j = e / (kTiB * kTiB), r = (e / kTiB) % kTiB, cc = e % kTiB;
+      double acc = 0.0;
+      for (int t = 0; t <= r; ++t) acc += (double)Xs[i0 + r][i0 + t] * Ss[j][t][cc];
+      Xs[i0 + r][j * kTiB + cc] = (float)(-acc);
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < NB * NB; idx += 512) {
+    const int i = idx % NB, c = idx / NB;
+    D[idx] = (i < nb && c < nb && c <= i) ? Xs[i][c] : 0.f;
+  }
+}
+
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(int n, const T* __restrict__ F, int lda,
                                                          const T* __restrict__ Dinv,
@@ -1861,7 +1965,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
           ta.cand = t_cand; ta.cid = t_cid; ta.count = t_count; ta.top = t_top;
           ta.ipiv = dipiv; ta.info = info; ta.abort_flag = abort_flag;
           hipLaunchKernelGGL(tslu_kernel, dim3(G), dim3(kPR), 0, st, ta);
-          hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, st, dipiv, kc, kb, l_dst(q),
+          hipLaunchKernelGGL(perm_build_fast, dim3(1), dim3(64), 0, st, dipiv, kc, kb, l_dst(q),
                              l_src(q), l_len(q));
           hipLaunchKernelGGL((perm_apply<T>), dim3((ob + 3) / 4), dim3(256), 0, st, dA, lda, k0,
                              k0 + ob, 0, 0, (const int*)l_dst(q), (const int*)l_src(q),
@@ -1884,14 +1988,13 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
         pa.keys_next = keys + (size_t)(key_buf ^ 1) * 2 * Gmax;
         key_buf ^= 1;
         pa.abort_flag = abort_flag;
+        pa.list_dst = l_dst(q); pa.list_src = l_src(q); pa.list_len = l_len(q);
         void* args[] = {&pa};
         BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, KB>, dim3(G),
                                                      dim3(kPR), args, 0, st));
         seg_end(st, e0, kPanel);
         e0 = ev_now(st);
-        hipLaunchKernelGGL(perm_build, dim3(1), dim3(64), 0, st, dipiv, kc, kb, l_dst(q), l_src(q),
-                           l_len(q));
-        const int ncols_blk = ob - kb;
+        const int ncols_blk = ob - kb;  // (the panel kernel built the gather list)
         if (ncols_blk > 0)
           hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_blk + 3) / 4), dim3(256), 0, st, dA, lda,
                              k0, kc, kc + kb, k0 + ob, (const int*)l_dst(q), (const int*)l_src(q),
@@ -1904,10 +2007,15 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       if (aligned && ((s + kb) % kSolveNB == 0 || s + kb == ob)) {
         const int jb = (s + kb - 1) / kSolveNB;
         const int d0 = k0 + jb * kSolveNB;
-        hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(kTriInvGroups), dim3(kTriInvThreads),
-                           tri_inv_lds(sizeof(T)), st, std::min(kSolveNB, ob - jb * kSolveNB),
-                           (const T*)(dA + (size_t)d0 + (size_t)d0 * lda), lda,
-                           linv + (size_t)jb * kSolveNB * kSolveNB);
+        const int bw = std::min(kSolveNB, ob - jb * kSolveNB);
+        const T* Fb = dA + (size_t)d0 + (size_t)d0 * lda;
+        T* Db = linv + (size_t)jb * kSolveNB * kSolveNB;
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL(tri_inv_lower_blocked, dim3(1), dim3(512), 0, st, bw,
+                             (const float*)Fb, lda, (float*)Db);
+        else
+          hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(kTriInvGroups),
+                             dim3(kTriInvThreads), tri_inv_lds(sizeof(T)), st, bw, Fb, lda, Db);
         BEMHIP_HIP_TRY(c, hipGetLastError());
       }
       // the rest of this outer block
