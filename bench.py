@@ -59,6 +59,7 @@ PEAK_MFMA_F32 = 157.3e12   # FLOP/s dense
 PEAK_MFMA_F64 = 78.6e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_star_N32769.json")
 KELVIN_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_kelvin_star_N32766.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_flops_star_N32769.json")
 
 
 def log(msg):
@@ -629,7 +630,9 @@ def main():
     value = entries / (asm_avg * 1e-3)
     solve_mixed_s = float(np.mean(solve_ms)) * 1e-3    # FP32 LU + FP64 refinement
 
-    detail = {"assemble_ms": asm_avg, "far_kernel_ms": float(np.mean(far_ms)),
+    detail = {"valu": (json.load(open(VALU_FILE)) if os.path.exists(VALU_FILE) and nt == 32769
+                       else None),
+              "assemble_ms": asm_avg, "far_kernel_ms": float(np.mean(far_ms)),
               "lu32_ms": float(np.mean(lu_ms)), "refine_ms": float(np.mean(ref_ms)),
               "refine_iters": iters, "solve_mixed_s": solve_mixed_s,
               "counters": counters}
