@@ -226,10 +226,11 @@ int bemhip_create(bemhip_ctx** out, int device_id) {
   auto* c = new bemhip_ctx();
   c->device = device_id;
   if (hipSetDevice(device_id) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
-      hipStreamCreate(&c->stream2) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
-      hipEventCreate(&c->ev1) != hipSuccess) {
+      hipStreamCreate(&c->stream2) != hipSuccess || hipStreamCreate(&c->stream3) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->stream2) hipStreamDestroy(c->stream2);
+    if (c->stream3) hipStreamDestroy(c->stream3);
     delete c;
     return fail(nullptr, BEMHIP_ERR_HIP, "create: HIP stream/event creation failed");
   }
@@ -244,10 +245,12 @@ void bemhip_destroy(bemhip_ctx* c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   hipStreamSynchronize(c->stream2);
+  hipStreamSynchronize(c->stream3);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   hipStreamDestroy(c->stream);
   hipStreamDestroy(c->stream2);
+  hipStreamDestroy(c->stream3);
   if (c->h_count) hipHostFree(c->h_count);
   if (c->h_flag) hipHostFree(c->h_flag);
   delete c;
@@ -565,6 +568,7 @@ int bemhip_synchronize(bemhip_ctx* c) {
   hipSetDevice(c->device);
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream2));
+  BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream3));
   return BEMHIP_OK;
 }
 

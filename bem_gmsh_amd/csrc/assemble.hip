@@ -38,21 +38,55 @@ namespace {
 // Far-field tile kernel.
 // Block = 256 rows x 32 elements. Element data staged in LDS; each lane owns
 // one row i, so every store H[i + j*nt] of a wave is 64 consecutive entries.
-// Classification: the FP32 pre-test settles the clearly-far pairs; the rest
-// (near pairs and those within 0.1% of the cutoff) take the exact FP64 test.
+// Tiles whose row box lies farther than (1 + 1e-6) x the largest cutoff from
+// the column block's node box hold only far pairs (the box test of
+// near_classify, the same decision): they skip the pair tests. In the other
+// tiles the FP32 pre-test settles the clearly-far pairs and the rest (near
+// pairs and those within 0.1% of the cutoff) take the exact FP64 test.
 // FP64 storage: the entries with the FP64 operation order of the reference.
-// Mixed storage: FP32 entries (FP32 reciprocal and logarithm) and the FP64
-// row sum of the far H entries (Newton reciprocal) for the diagonal.
+// Mixed storage: the FP64 far H term (Newton reciprocal, about 1 ulp) goes
+// into the FP64 row sum for the diagonal and, rounded, into H; G from the FP32
+// logarithm of the rounded squared distance. The row sum is the diagonal of
+// the FP64 operator that the refinement iterates with: FP64 accuracy matters
+// there (an ill-conditioned system amplifies a relative 1e-13 into the
+// solution's digits).
 // H and G are written with non-temporal stores: nothing reads them back
 // before the kernel ends.
 // ---------------------------------------------------------------------------
+
+// Bounding box of each 256-row block of collocation rows: box[rb] =
+// {lox, hix, loy, hiy} (one wavefront per block, four rows per lane).
+__global__ __launch_bounds__(64) void row_block_boxes(Geo g, double* __restrict__ box) {
+  const int r0 = blockIdx.x * kFarThreads;
+  double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
+  for (int r = r0 + threadIdx.x; r < min(g.nt, r0 + kFarThreads); r += 64) {
+    const double x = g.rx[r], y = g.ry[r];
+    lox = fmin(lox, x); hix = fmax(hix, x);
+    loy = fmin(loy, y); hiy = fmax(hiy, y);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lox = fmin(lox, __shfl_xor(lox, off, 64));
+    hix = fmax(hix, __shfl_xor(hix, off, 64));
+    loy = fmin(loy, __shfl_xor(loy, off, 64));
+    hiy = fmax(hiy, __shfl_xor(hiy, off, 64));
+  }
+  if (threadIdx.x == 0) {
+    double* b = box + (size_t)blockIdx.x * 4;
+    b[0] = lox; b[1] = hix; b[2] = loy; b[3] = hiy;
+  }
+}
+
 template <typename TS, int NN>
-__global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* __restrict__ H,
-                                                          TS* __restrict__ G,
+__global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs,
+                                                          const double* __restrict__ cbox,
+                                                          const double* __restrict__ rbox,
+                                                          TS* __restrict__ H, TS* __restrict__ G,
                                                           double* __restrict__ hpart) {
+  constexpr bool kMixed = sizeof(TS) == 4;
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
       s_jw[kEB * NN], s_L[kEB];
-  __shared__ float s_nxf[kEB * NN], s_nyf[kEB * NN], s_jwf[kEB * NN], s_lim2f[kEB];
+  __shared__ float s_gwf[kEB * NN], s_lim2f[kEB];
   const int cb = blockIdx.y;
   const int e0 = cb * kEB;
   const int ecount = min(kEB, g.ne - e0);
@@ -60,9 +94,22 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
   for (int t = threadIdx.x; t < ecount * NN; t += blockDim.x) {
     const int j = e0 * NN + t;
     s_nx[t] = g.nx[j]; s_ny[t] = g.ny[j];
-    s_jw[t] = g.jw[j];
-    s_nxf[t] = (float)g.nx[j]; s_nyf[t] = (float)g.ny[j];
-    s_jwf[t] = (float)g.jw[j];
+    if constexpr (kMixed) {
+      s_jw[t] = cs.cT * g.jw[j];                                   // T weight
+      s_gwf[t] = (float)(-0.69314718055994531 * cs.cU * g.jw[j]);  // log2 -> G weight
+    } else {
+      s_jw[t] = g.jw[j];
+    }
+  }
+  // all-far tile test (uniform over the block)
+  bool all_far = false;
+  {
+    const double* bx = cbox + (size_t)cb * 5;
+    const double* rb = rbox + (size_t)blockIdx.x * 4;
+    const double dx = fmax(0.0, fmax(bx[0] - rb[1], rb[0] - bx[2]));
+    const double dy = fmax(0.0, fmax(bx[1] - rb[3], rb[2] - bx[3]));
+    const double ml = bx[4];
+    all_far = !(cs.nf > 1e20) && (dx * dx + dy * dy) > (ml * ml) * (1.0 + 1e-6);
   }
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
@@ -70,47 +117,55 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs, TS* 
   const double px = g.rx[i], py = g.ry[i];
   const size_t ld = g.ld;
   double rowsum = 0.0;
-  for (int el = 0; el < ecount; ++el) {
-    double rx[NN], ry[NN];
-    float rxf[NN], ryf[NN], d2f[NN];
-    const int col0 = (e0 + el) * NN;
-    const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
-    const bool nr = classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el],
-                                      s_lim2f[el], cs.nf, onel, rx, ry, rxf, ryf, d2f);
-    TS* Hc = H + (size_t)col0 * ld + i;
-    TS* Gc = G + (size_t)col0 * ld + i;
-    if (nr) {
-#pragma unroll
-      for (int k = 0; k < NN; ++k) {
-        __builtin_nontemporal_store(TS(0), Hc + k * ld);
-        __builtin_nontemporal_store(TS(0), Gc + k * ld);
-      }
-    } else if constexpr (sizeof(TS) == 8) {
-#pragma unroll
-      for (int k = 0; k < NN; ++k) {
-        const double d2 = rx[k] * rx[k] + ry[k] * ry[k];
-        const double U = -log(d2) * cs.cU;
-        const double T = (rx[k] * s_nx[el * NN + k] + ry[k] * s_ny[el * NN + k]) *
-                         (cs.cT / d2);
-        const double w = s_jw[el * NN + k];
-        const double h = T * w;
-        __builtin_nontemporal_store(h, Hc + k * ld);
-        __builtin_nontemporal_store(U * w, Gc + k * ld);
-        rowsum += h;
-      }
+  // one far entry pair (H, G) of node t from the FP64 offsets
+  auto far_entry = [&](double rx, double ry, int t, TS* Hp, TS* Gp) {
+    if constexpr (kMixed) {
+      const double d2 = fma(rx, rx, ry * ry);
+      const double T = fma(rx, s_nx[t], ry * s_ny[t]) * rcp_f64(d2);
+      rowsum = fma(T, s_jw[t], rowsum);
+      __builtin_nontemporal_store((float)(T * s_jw[t]), Hp);
+      __builtin_nontemporal_store(__builtin_amdgcn_logf((float)d2) * s_gwf[t], Gp);
     } else {
-      const float cUf = (float)cs.cU, cTf = (float)cs.cT;
+      const double d2 = rx * rx + ry * ry;
+      const double U = -log(d2) * cs.cU;
+      const double T = (rx * s_nx[t] + ry * s_ny[t]) * (cs.cT / d2);
+      const double w = s_jw[t];
+      const double h = T * w;
+      __builtin_nontemporal_store(h, Hp);
+      __builtin_nontemporal_store(U * w, Gp);
+      rowsum += h;
+    }
+  };
+  if (all_far) {
+    for (int el = 0; el < ecount; ++el) {
+      const int col0 = (e0 + el) * NN;
+      TS* Hc = H + (size_t)col0 * ld + i;
+      TS* Gc = G + (size_t)col0 * ld + i;
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
         const int t = el * NN + k;
-        const double d2 = fma(rx[k], rx[k], ry[k] * ry[k]);
-        const double T64 = fma(rx[k], s_nx[t], ry[k] * s_ny[t]) * (cs.cT * rcp_f64(d2));
-        rowsum = fma(T64, s_jw[t], rowsum);
-        const float rinv = __builtin_amdgcn_rcpf(d2f[k]);
-        const float U = -__logf(d2f[k]) * cUf;
-        const float T = (rxf[k] * s_nxf[t] + ryf[k] * s_nyf[t]) * (cTf * rinv);
-        __builtin_nontemporal_store(T * s_jwf[t], Hc + k * ld);
-        __builtin_nontemporal_store(U * s_jwf[t], Gc + k * ld);
+        far_entry(s_cx[t] - px, s_cy[t] - py, t, Hc + k * ld, Gc + k * ld);
+      }
+    }
+  } else {
+    for (int el = 0; el < ecount; ++el) {
+      double rx[NN], ry[NN];
+      float rxf[NN], ryf[NN], d2f[NN];
+      const int col0 = (e0 + el) * NN;
+      const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
+      const bool nr = classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el],
+                                        s_lim2f[el], cs.nf, onel, rx, ry, rxf, ryf, d2f);
+      TS* Hc = H + (size_t)col0 * ld + i;
+      TS* Gc = G + (size_t)col0 * ld + i;
+      if (nr) {
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          __builtin_nontemporal_store(TS(0), Hc + k * ld);
+          __builtin_nontemporal_store(TS(0), Gc + k * ld);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NN; ++k) far_entry(rx[k], ry[k], el * NN + k, Hc + k * ld, Gc + k * ld);
       }
     }
   }
@@ -314,6 +369,9 @@ __global__ __launch_bounds__(64) void cand_near_kernel(
 //                           nodes, 32 lanes per pair, two pairs per wavefront.
 // Lanes run over quadrature points; the values go to the pair list at the
 // pair's CSR index. list == nullptr: the k-th pair is CSR index k.
+// near_sing_kernel with prow == pelem == nullptr takes the k-th pair as row k
+// and the element holding its node, (k + row0) / NN, and writes slot k: the
+// on-element pairs need no classification, so the assembly starts them first.
 // ---------------------------------------------------------------------------
 
 template <int NN, int G>
@@ -371,14 +429,21 @@ __global__ __launch_bounds__(256) void near_reg_kernel(Geo g, const QuadConst* _
       const double nx = dxy / J, ny = -(dxx / J);
       if (nx * nrefx + ny * nrefy < 0.0) flip = -1.0;
     }
+    // at most two rule nodes per lane (nq <= 2 npg <= 2 G): placed first, so
+    // that the plan's registers are free during the integration
+    double xqs[2], wqs[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      xqs[r] = 0.0; wqs[r] = 0.0;
+      if (r * G + lane < nq) tanp3c_node(P, r * G + lane, npg, qc->gl_x, qc->gl_w, xqs[r], wqs[r]);
+    }
     double ha[NN], ga[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) { ha[k] = 0.0; ga[k] = 0.0; }
-    for (int q0 = 0; q0 < nq; q0 += G) {
-      const int q = q0 + lane;
-      if (q < nq) {
-        double xq, wq;
-        tanp3c_node(P, q, npg, qc->gl_x, qc->gl_w, xq, wq);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      if (r * G + lane < nq) {
+        const double xq = xqs[r], wq = wqs[r];
         double N[NN], dN[NN], Ng[NN], dNg[NN];
         shapefun<NN>(lx, lbw, lD, xq, N, dN);
         shapefun<NN>(ex, ebw, eD, xq, Ng, dNg);
@@ -434,8 +499,8 @@ __global__ __launch_bounds__(256) void near_sing_kernel(Geo g, const QuadConst* 
   const int64_t pair = list ? (int64_t)list[kk] : kk;
   const int lane = threadIdx.x & (G - 1);
   const int gshift = threadIdx.x & G;  // 0 or 32: bit offset of this group in a ballot
-  const int i = prow[pair];
-  const int e = pelem[pair];
+  const int i = prow ? prow[pair] : (int)pair;
+  const int e = pelem ? pelem[pair] : (int)((pair + g.row0) / NN);
   const double px = g.rx[i], py = g.ry[i];
   double cx[NN], cy[NN], gxx[NN], gyy[NN];
 #pragma unroll
@@ -595,6 +660,19 @@ __global__ __launch_bounds__(256) void near_sing_kernel(Geo g, const QuadConst* 
   }
 }
 
+// On-element values by row (near_sing_kernel's row mode) into the pair list:
+// slot r goes to CSR index sing[r].
+__global__ void sing_scatter(int nsing, int nn, const int* __restrict__ sing,
+                             const double* __restrict__ sh, const double* __restrict__ sg,
+                             double* __restrict__ ph, double* __restrict__ pg) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsing * nn) return;
+  const int r = t / nn, k = t - r * nn;
+  const size_t dst = (size_t)sing[r] * nn + k;
+  ph[dst] = sh[t];
+  pg[dst] = sg[t];
+}
+
 // Column swap A = H with Dirichlet columns -G (src/Core/Boundary_conditions.jl:54-64).
 template <typename TS>
 __global__ void bc_swap_kernel(int nt, int n, size_t ld, const int* __restrict__ bc,
@@ -609,11 +687,11 @@ __global__ void bc_swap_kernel(int nt, int n, size_t ld, const int* __restrict__
 }
 
 template <typename TS, int NN>
-int launch_far(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Consts& cs, TS* H, TS* G,
-               double* hpart) {
+int launch_far(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Consts& cs,
+               const double* cbox, const double* rbox, TS* H, TS* G, double* hpart) {
   dim3 grid((g.nt + kFarThreads - 1) / kFarThreads, (g.ne + kEB - 1) / kEB);
-  hipLaunchKernelGGL((far_kernel<TS, NN>), grid, dim3(kFarThreads), 0, stream, g, cs, H, G,
-                     hpart);
+  hipLaunchKernelGGL((far_kernel<TS, NN>), grid, dim3(kFarThreads), 0, stream, g, cs, cbox, rbox,
+                     H, G, hpart);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   return BEMHIP_OK;
 }
@@ -688,13 +766,16 @@ static Consts make_consts(const bemhip_ctx* c) {
   return cs;
 }
 
-// Full assembly on two streams:
-//   stream 1: near_classify -> near-pair CSR -> near_sing / near_reg --+-> finalize
-//   stream 2:               \-> far_kernel ----------------------------/
+// Full assembly on three streams:
+//   stream 3: near_sing (row mode) ----------------------------------------+
+//   stream 1: near_classify -> near-pair CSR -> near_reg -> sing_scatter -+-> finalize
+//   stream 2: far_kernel -------------------------------------------------/
 // The far field (HBM-write bound) and the near/singular integrals (FP64 VALU
-// bound) run concurrently. far_kernel writes 0 into the near entries, so the
-// near kernels keep their values in the pair list and the finalize pass,
-// after both, writes them into H and G together with the rigid-body diagonal.
+// bound) run concurrently; the on-element rule needs no classification (row
+// i's own element is (i + row0) / nn), so it starts first. far_kernel writes
+// 0 into the near entries, so the near kernels keep their values in the pair
+// list and the finalize pass, after all, writes them into H and G together
+// with the rigid-body diagonal.
 int assemble_impl(bemhip_ctx* c) {
   const Geo g = make_geo(c);
   const Consts cs = make_consts(c);
@@ -746,12 +827,14 @@ int assemble_impl(bemhip_ctx* c) {
                                        sizeof(float) * c->ld32 * c->ni, c->stream));
   }
   // events: 0 start, 1 classified, 2 far done, 3 pairs start, 4 pairs done,
-  // 5 near done, 6 diagonal done, 7 far start, 8 finalize start
-  hipEvent_t ev[9];
+  // 5 near done, 6 diagonal done, 7 far start, 8 finalize start, 9 on-element
+  // start, 10 on-element done, 11 block boxes done
+  constexpr int kEvents = 12;
+  hipEvent_t ev[kEvents];
   for (auto& e : ev) BEMHIP_HIP_TRY(c, hipEventCreate(&e));
   struct Destroy {
     hipEvent_t* e;
-    ~Destroy() { for (int k = 0; k < 9; ++k) hipEventDestroy(e[k]); }
+    ~Destroy() { for (int k = 0; k < kEvents; ++k) hipEventDestroy(e[k]); }
   } destroy{ev};
   const int nfb = (int)((nt + kFinRows - 1) / kFinRows);
   hipStream_t s2 = c->stream2;
@@ -761,7 +844,27 @@ int assemble_impl(bemhip_ctx* c) {
   // near-pair CSR and runs the near integrals.
   const int nrb = (int)((nt + kFarThreads - 1) / kFarThreads);  // row blocks
   BEMHIP_HIP_TRY(c, ensure_count_word(c));
+  const int64_t nsing = std::min<int64_t>(nt, c->n);  // one on-element pair per boundary row
+  BEMHIP_HIP_TRY(c, c->sing_h.ensure((size_t)std::max<int64_t>(nsing, 1) * c->nn));
+  BEMHIP_HIP_TRY(c, c->sing_g.ensure((size_t)std::max<int64_t>(nsing, 1) * c->nn));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
+  // stream 3: the on-element rule by row, first in line for the CUs
+  hipStream_t s3 = c->stream3;
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s3, ev[0], 0));
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[9], s3));
+  if (nsing > 0) {
+    const int64_t blocks = (nsing + 7) / 8;  // two rows per wavefront, four wavefronts
+    if (c->nn == 2)
+      hipLaunchKernelGGL((near_sing_kernel<2>), dim3((unsigned)blocks), dim3(256), 0, s3, g,
+                         c->qc.ptr, nsing, nullptr, nullptr, nullptr, c->sing_h.ptr,
+                         c->sing_g.ptr);
+    else
+      hipLaunchKernelGGL((near_sing_kernel<3>), dim3((unsigned)blocks), dim3(256), 0, s3, g,
+                         c->qc.ptr, nsing, nullptr, nullptr, nullptr, c->sing_h.ptr,
+                         c->sing_g.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[10], s3));
   const FmmPlan& FP = c->fmm;
   if (fmm) {
     if (c->nn == 2)
@@ -777,32 +880,42 @@ int assemble_impl(bemhip_ctx* c) {
     BEMHIP_HIP_TRY(c, hipGetLastError());
   } else {
     BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
+    BEMHIP_HIP_TRY(c, c->rowbox.ensure((size_t)nrb * 4));
     BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
     BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
   }
   if (!fmm) {
-    const int cpb = 32;  // column blocks per classify workgroup
-    dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)((ncb + cpb - 1) / cpb));
-    if (c->nn == 2) {
+    // node boxes of the column blocks and row blocks: the all-far tile test
+    // of near_classify and far_kernel
+    if (c->nn == 2)
       hipLaunchKernelGGL((elem_block_boxes<2>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
                          cs, ncb, c->blkbox.ptr);
-      hipLaunchKernelGGL((near_classify<2>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
-                         cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
-    } else {
+    else
       hipLaunchKernelGGL((elem_block_boxes<3>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
                          cs, ncb, c->blkbox.ptr);
+    hipLaunchKernelGGL(row_block_boxes, dim3((unsigned)nrb), dim3(64), 0, c->stream, g,
+                       c->rowbox.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[11], c->stream));
+  if (!fmm) {
+    const int cpb = 32;  // column blocks per classify workgroup
+    dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)((ncb + cpb - 1) / cpb));
+    if (c->nn == 2)
+      hipLaunchKernelGGL((near_classify<2>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
+                         cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
+    else
       hipLaunchKernelGGL((near_classify<3>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
                          cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
-    }
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[3], c->stream));
-  // stream 2: the far field (needs only the geometry; starts after the classify
-  // so that the phases can be timed apart)
+  // stream 2: the far field (needs the geometry, the block boxes and the unit
+  // weights): it starts with the classification, not after it
   int st = BEMHIP_OK;
   if (!fmm) {
-  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[1], 0));
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s2, ev[11], 0));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[7], s2));
   if (mfree) {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
@@ -815,16 +928,20 @@ int assemble_impl(bemhip_ctx* c) {
     BEMHIP_HIP_TRY(c, hipGetLastError());
     st = BEMHIP_OK;
   } else if (fp64)
-    st = (c->nn == 2) ? launch_far<double, 2>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr)
-                      : launch_far<double, 3>(c, s2, g, cs, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+    st = (c->nn == 2) ? launch_far<double, 2>(c, s2, g, cs, c->blkbox.ptr, c->rowbox.ptr,
+                                              c->H64.ptr, c->G64.ptr, c->hpart.ptr)
+                      : launch_far<double, 3>(c, s2, g, cs, c->blkbox.ptr, c->rowbox.ptr,
+                                              c->H64.ptr, c->G64.ptr, c->hpart.ptr);
   else
-    st = (c->nn == 2) ? launch_far<float, 2>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr)
-                      : launch_far<float, 3>(c, s2, g, cs, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
+    st = (c->nn == 2) ? launch_far<float, 2>(c, s2, g, cs, c->blkbox.ptr, c->rowbox.ptr,
+                                             c->H32.ptr, c->G32.ptr, c->hpart.ptr)
+                      : launch_far<float, 3>(c, s2, g, cs, c->blkbox.ptr, c->rowbox.ptr,
+                                             c->H32.ptr, c->G32.ptr, c->hpart.ptr);
   if (st) return st;
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], s2));
   }
-  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
-                     c->row_ptr.ptr, c->d_count);
+  hipLaunchKernelGGL(near_scan, dim3(1), dim3(kScanWaves * kWave), 0, c->stream, (int)nt,
+                     c->row_cnt.ptr, c->row_ptr.ptr, c->d_count);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(c->stream));
   const int total = *(volatile int*)c->h_count;
@@ -833,7 +950,6 @@ int assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, c->pair_elem.ensure((size_t)std::max(total, 1)));
   BEMHIP_HIP_TRY(c, c->pair_h.ensure((size_t)std::max(total, 1) * c->nn));
   BEMHIP_HIP_TRY(c, c->pair_g.ensure((size_t)std::max(total, 1) * c->nn));
-  const int64_t nsing = std::min<int64_t>(nt, c->n);  // one on-element pair per boundary row
   const int64_t nreg = (int64_t)total - nsing;
   BEMHIP_HIP_TRY(c, c->sing_idx.ensure((size_t)std::max<int64_t>(nsing, 1)));
   BEMHIP_HIP_TRY(c, c->reg_idx.ensure((size_t)std::max<int64_t>(nreg, 1)));
@@ -864,15 +980,23 @@ int assemble_impl(bemhip_ctx* c) {
   }
   // Near entries: far_kernel writes 0 into the near entries of H and G, so the
   // near values stay in the pair list and the finalize pass writes them into
-  // H and G after both.
+  // H and G after all. The on-element values (stream 3) join the list here.
   st = (c->nn == 2)
-               ? launch_near_pairs<2>(c, c->stream, g, c->qc.ptr, c->npg, nsing, c->sing_idx.ptr,
-                                      nreg, c->reg_idx.ptr, c->pair_row.ptr, c->pair_elem.ptr,
+               ? launch_near_pairs<2>(c, c->stream, g, c->qc.ptr, c->npg, 0, nullptr, nreg,
+                                      c->reg_idx.ptr, c->pair_row.ptr, c->pair_elem.ptr,
                                       c->pair_h.ptr, c->pair_g.ptr)
-               : launch_near_pairs<3>(c, c->stream, g, c->qc.ptr, c->npg, nsing, c->sing_idx.ptr,
-                                      nreg, c->reg_idx.ptr, c->pair_row.ptr, c->pair_elem.ptr,
+               : launch_near_pairs<3>(c, c->stream, g, c->qc.ptr, c->npg, 0, nullptr, nreg,
+                                      c->reg_idx.ptr, c->pair_row.ptr, c->pair_elem.ptr,
                                       c->pair_h.ptr, c->pair_g.ptr);
   if (st) return st;
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(c->stream, ev[10], 0));
+  if (nsing > 0) {
+    const int cnt = (int)nsing * c->nn;
+    hipLaunchKernelGGL(sing_scatter, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
+                       c->stream, (int)nsing, c->nn, c->sing_idx.ptr, c->sing_h.ptr,
+                       c->sing_g.ptr, c->pair_h.ptr, c->pair_g.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[5], c->stream));
   BEMHIP_HIP_TRY(c, hipStreamWaitEvent(c->stream, ev[2], 0));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[8], c->stream));
@@ -905,6 +1029,7 @@ int assemble_impl(bemhip_ctx* c) {
   c->ms["assemble_pairs"] = el(3, 4);
   c->ms["assemble_near"] = el(4, 5);
   c->ms["assemble_diag"] = el(8, 6);
+  c->ms["assemble_sing"] = el(9, 10);
   return BEMHIP_OK;
 }
 
@@ -1093,8 +1218,8 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
   hipLaunchKernelGGL((near_classify<NN>), cgrid, dim3(kFarThreads), 0, s1, g, cs, ncb, cpb,
                      c->ev_box.ptr, c->ev_bits.ptr, c->ev_cnt.ptr, c->ev_tiles.ptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
-  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, s1, npts, c->ev_cnt.ptr, c->ev_ptr.ptr,
-                     c->d_count);
+  hipLaunchKernelGGL(near_scan, dim3(1), dim3(kScanWaves * kWave), 0, s1, npts, c->ev_cnt.ptr,
+                     c->ev_ptr.ptr, c->d_count);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(s1));
   const int total = *(volatile int*)c->h_count;

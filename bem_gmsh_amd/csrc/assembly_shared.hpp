@@ -265,19 +265,24 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 // ---------------------------------------------------------------------------
 // Near-pair CSR from the bit matrix [word][row].
 // ---------------------------------------------------------------------------
-// Exclusive scan of cnt[0..nt) into ptr[0..nt] with one workgroup of 16
-// wavefronts: each wave owns a contiguous segment, reads it 64 rows at a time
-// (coalesced) for its total, then scans it with lane shuffles from the sum of
-// the segments before it. The total also goes to host_total (pinned host
-// memory, may be null), so the host reads it without a copy on the stream.
-__global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict__ cnt,
-                                                  int* __restrict__ ptr,
-                                                  int* __restrict__ host_total) {
-  __shared__ int s_tot[16];
+// Exclusive scan of cnt[0..nt) into ptr[0..nt] with one workgroup of
+// kScanWaves wavefronts (one per SIMD: it fits beside a kernel that holds most
+// of a CU): each wave owns a contiguous segment, reads it 64 rows at a time
+// (coalesced, four loads in flight) for its total, then scans it with lane
+// shuffles from the sum of the segments before it. The total also goes to
+// host_total (pinned host memory, may be null), so the host reads it without
+// a copy on the stream.
+constexpr int kScanWaves = 4;
+__global__ __launch_bounds__(kScanWaves * kWave) void near_scan(int nt,
+                                                                const int* __restrict__ cnt,
+                                                                int* __restrict__ ptr,
+                                                                int* __restrict__ host_total) {
+  __shared__ int s_tot[kScanWaves];
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  const int seg = (nt + 15) / 16;
+  const int seg = (nt + kScanWaves - 1) / kScanWaves;
   const int lo = min(nt, wv * seg), hi = min(nt, lo + seg);
   int tot = 0;
+#pragma unroll 4
   for (int r = lo + lane; r < hi; r += kWave) tot += cnt[r];
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) tot += __shfl_xor(tot, off, kWave);
@@ -285,21 +290,29 @@ __global__ __launch_bounds__(1024) void near_scan(int nt, const int* __restrict_
   __syncthreads();
   int run = 0;
   for (int w = 0; w < wv; ++w) run += s_tot[w];
-  for (int r0 = lo; r0 < hi; r0 += kWave) {
-    const int r = r0 + lane;
-    const int v = (r < hi) ? cnt[r] : 0;
-    int x = v;  // inclusive scan over the wave
+  for (int r0 = lo; r0 < hi; r0 += 4 * kWave) {
+    int v[4];
 #pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const int y = __shfl_up(x, off, kWave);
-      if (lane >= off) x += y;
+    for (int u = 0; u < 4; ++u) {
+      const int r = r0 + u * kWave + lane;
+      v[u] = (r < hi) ? cnt[r] : 0;
     }
-    if (r < hi) ptr[r] = run + x - v;
-    run += __shfl(x, kWave - 1, kWave);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = r0 + u * kWave + lane;
+      int x = v[u];  // inclusive scan over the wave
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+      }
+      if (r < hi) ptr[r] = run + x - v[u];
+      run += __shfl(x, kWave - 1, kWave);
+    }
   }
   if (threadIdx.x == 0) {
     int t = 0;
-    for (int w = 0; w < 16; ++w) t += s_tot[w];
+    for (int w = 0; w < kScanWaves; ++w) t += s_tot[w];
     ptr[nt] = t;
     if (host_total) __hip_atomic_store(host_total, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }

@@ -100,6 +100,7 @@ struct bemhip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // second stream (default priority) for concurrent phases
+  hipStream_t stream3 = nullptr;  // third stream (default priority): the assembly's on-element rule
   std::string err;
   int num_cu = 0;
 
@@ -130,9 +131,11 @@ struct bemhip_ctx {
   bemhip::DevBuf<int> row_cnt, row_ptr, pair_row, pair_elem;
   bemhip::DevBuf<double> pair_h, pair_g;  // [npairs][nn]
   bemhip::DevBuf<int> sing_idx, reg_idx;  // CSR indices of the on-element / other pairs
+  bemhip::DevBuf<double> sing_h, sing_g;  // [min(nt, n)][nn] on-element values by row
   int64_t npairs = 0;
   bemhip::DevBuf<double> hpart;   // [ncb][nt] far row sums of H
   bemhip::DevBuf<double> blkbox;  // [ncb][5] node bounding box and cutoff of each column block
+  bemhip::DevBuf<double> rowbox;  // [row block][4] node bounding box of each 256-row block
   bemhip::DevBuf<uint8_t> tiles;  // [row block][ncb] 1 where the pair tests ran
   int* h_count = nullptr;          // pinned host word: near-pair count
   int* d_count = nullptr;          // its device address

@@ -853,8 +853,8 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[1], c->stream));
   // the pair count (one workgroup) before the far field fills the GPU
   BEMHIP_HIP_TRY(c, ensure_count_word(c));
-  hipLaunchKernelGGL(near_scan, dim3(1), dim3(1024), 0, c->stream, (int)nt, c->row_cnt.ptr,
-                     c->row_ptr.ptr, c->d_count);
+  hipLaunchKernelGGL(near_scan, dim3(1), dim3(kScanWaves * kWave), 0, c->stream, (int)nt,
+                     c->row_cnt.ptr, c->row_ptr.ptr, c->d_count);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[4], c->stream));
   // pair CSR and the near rules
