@@ -54,12 +54,24 @@ namespace {
 // before the kernel ends.
 // ---------------------------------------------------------------------------
 
-// Bounding box of each 256-row block of collocation rows: box[rb] =
-// {lox, hix, loy, hiy} (one wavefront per block, four rows per lane).
-__global__ __launch_bounds__(64) void row_block_boxes(Geo g, double* __restrict__ box) {
-  const int r0 = blockIdx.x * kFarThreads;
+// Node bounding boxes of the column blocks (cbox, as elem_block_boxes) and of
+// the 256-row blocks (rbox[rb] = {lox, hix, loy, hiy}), one wavefront per
+// block in one launch: workgroups [0, ncb) take the column blocks, the rest
+// the row blocks, clearing the rows' near-pair counts on the way.
+template <int NN>
+__global__ __launch_bounds__(64) void block_boxes(Geo g, Consts cs, int ncb,
+                                                  double* __restrict__ cbox,
+                                                  double* __restrict__ rbox,
+                                                  int* __restrict__ cnt) {
+  if ((int)blockIdx.x < ncb) {
+    elem_box_wave<NN>(g, cs, blockIdx.x, cbox);
+    return;
+  }
+  const int rb = blockIdx.x - ncb;
+  const int r0 = rb * kFarThreads;
   double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
   for (int r = r0 + threadIdx.x; r < min(g.nt, r0 + kFarThreads); r += 64) {
+    cnt[r] = 0;
     const double x = g.rx[r], y = g.ry[r];
     lox = fmin(lox, x); hix = fmax(hix, x);
     loy = fmin(loy, y); hiy = fmax(hiy, y);
@@ -72,7 +84,7 @@ __global__ __launch_bounds__(64) void row_block_boxes(Geo g, double* __restrict_
     hiy = fmax(hiy, __shfl_xor(hiy, off, 64));
   }
   if (threadIdx.x == 0) {
-    double* b = box + (size_t)blockIdx.x * 4;
+    double* b = rbox + (size_t)rb * 4;
     b[0] = lox; b[1] = hix; b[2] = loy; b[3] = hiy;
   }
 }
@@ -686,11 +698,21 @@ __global__ void bc_swap_kernel(int nt, int n, size_t ld, const int* __restrict__
     dst[i] = dir ? -src[i] : src[i];
 }
 
+// Dynamic LDS that the FP32 far_kernel reserves and does not use: with its
+// 5.4 KB of element data a workgroup then holds 37.4 KB, so at most 4 of them
+// share a CU's 160 KB. The HBM-write-bound FP32 far field runs as fast at 4
+// workgroups per CU as at 8 (measured: 1.62-1.66 ms either way at N = 32769),
+// and the wave slots and registers left free let the near-pair chain, which
+// runs beside it, make progress (mixed assembly 1.96 -> 1.82 ms). The FP64
+// far field (a logarithm and a division per entry) needs its full occupancy.
+constexpr unsigned kFarLdsPad = 32768;
+
 template <typename TS, int NN>
 int launch_far(bemhip_ctx* c, hipStream_t stream, const Geo& g, const Consts& cs,
                const double* cbox, const double* rbox, TS* H, TS* G, double* hpart) {
   dim3 grid((g.nt + kFarThreads - 1) / kFarThreads, (g.ne + kEB - 1) / kEB);
-  hipLaunchKernelGGL((far_kernel<TS, NN>), grid, dim3(kFarThreads), 0, stream, g, cs, cbox, rbox,
+  const unsigned pad = sizeof(TS) == 4 ? kFarLdsPad : 0u;
+  hipLaunchKernelGGL((far_kernel<TS, NN>), grid, dim3(kFarThreads), pad, stream, g, cs, cbox, rbox,
                      H, G, hpart);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   return BEMHIP_OK;
@@ -848,23 +870,6 @@ int assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, c->sing_h.ensure((size_t)std::max<int64_t>(nsing, 1) * c->nn));
   BEMHIP_HIP_TRY(c, c->sing_g.ensure((size_t)std::max<int64_t>(nsing, 1) * c->nn));
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[0], c->stream));
-  // stream 3: the on-element rule by row, first in line for the CUs
-  hipStream_t s3 = c->stream3;
-  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s3, ev[0], 0));
-  BEMHIP_HIP_TRY(c, hipEventRecord(ev[9], s3));
-  if (nsing > 0) {
-    const int64_t blocks = (nsing + 7) / 8;  // two rows per wavefront, four wavefronts
-    if (c->nn == 2)
-      hipLaunchKernelGGL((near_sing_kernel<2>), dim3((unsigned)blocks), dim3(256), 0, s3, g,
-                         c->qc.ptr, nsing, nullptr, nullptr, nullptr, c->sing_h.ptr,
-                         c->sing_g.ptr);
-    else
-      hipLaunchKernelGGL((near_sing_kernel<3>), dim3((unsigned)blocks), dim3(256), 0, s3, g,
-                         c->qc.ptr, nsing, nullptr, nullptr, nullptr, c->sing_h.ptr,
-                         c->sing_g.ptr);
-    BEMHIP_HIP_TRY(c, hipGetLastError());
-  }
-  BEMHIP_HIP_TRY(c, hipEventRecord(ev[10], s3));
   const FmmPlan& FP = c->fmm;
   if (fmm) {
     if (c->nn == 2)
@@ -882,22 +887,38 @@ int assemble_impl(bemhip_ctx* c) {
     BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
     BEMHIP_HIP_TRY(c, c->rowbox.ensure((size_t)nrb * 4));
     BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
-    BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
   }
   if (!fmm) {
     // node boxes of the column blocks and row blocks: the all-far tile test
-    // of near_classify and far_kernel
+    // of near_classify and far_kernel (also clears row_cnt)
+    const unsigned nb = (unsigned)(ncb + nrb);
     if (c->nn == 2)
-      hipLaunchKernelGGL((elem_block_boxes<2>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
-                         cs, ncb, c->blkbox.ptr);
+      hipLaunchKernelGGL((block_boxes<2>), dim3(nb), dim3(64), 0, c->stream, g, cs, ncb,
+                         c->blkbox.ptr, c->rowbox.ptr, c->row_cnt.ptr);
     else
-      hipLaunchKernelGGL((elem_block_boxes<3>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
-                         cs, ncb, c->blkbox.ptr);
-    hipLaunchKernelGGL(row_block_boxes, dim3((unsigned)nrb), dim3(64), 0, c->stream, g,
-                       c->rowbox.ptr);
+      hipLaunchKernelGGL((block_boxes<3>), dim3(nb), dim3(64), 0, c->stream, g, cs, ncb,
+                         c->blkbox.ptr, c->rowbox.ptr, c->row_cnt.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[11], c->stream));
+  // stream 3: the on-element rule by row, in line for the CUs right after the
+  // block boxes (the far field waits for those)
+  hipStream_t s3 = c->stream3;
+  BEMHIP_HIP_TRY(c, hipStreamWaitEvent(s3, ev[0], 0));
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[9], s3));
+  if (nsing > 0) {
+    const int64_t blocks = (nsing + 7) / 8;  // two rows per wavefront, four wavefronts
+    if (c->nn == 2)
+      hipLaunchKernelGGL((near_sing_kernel<2>), dim3((unsigned)blocks), dim3(256), 0, s3, g,
+                         c->qc.ptr, nsing, nullptr, nullptr, nullptr, c->sing_h.ptr,
+                         c->sing_g.ptr);
+    else
+      hipLaunchKernelGGL((near_sing_kernel<3>), dim3((unsigned)blocks), dim3(256), 0, s3, g,
+                         c->qc.ptr, nsing, nullptr, nullptr, nullptr, c->sing_h.ptr,
+                         c->sing_g.ptr);
+    BEMHIP_HIP_TRY(c, hipGetLastError());
+  }
+  BEMHIP_HIP_TRY(c, hipEventRecord(ev[10], s3));
   if (!fmm) {
     const int cpb = 32;  // column blocks per classify workgroup
     dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)((ncb + cpb - 1) / cpb));
@@ -1272,10 +1293,10 @@ int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_p
   if (!all_near) BEMHIP_HIP_TRY(c, c->ev_part.ensure(2 * (size_t)ncb * chunk));
   BEMHIP_HIP_TRY(c, ensure_count_word(c));
   if (c->nn == 2)
-    hipLaunchKernelGGL((elem_block_boxes<2>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+    hipLaunchKernelGGL((elem_block_boxes<2>), dim3(ncb), dim3(64), 0, c->stream, g,
                        cs, ncb, c->ev_box.ptr);
   else
-    hipLaunchKernelGGL((elem_block_boxes<3>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+    hipLaunchKernelGGL((elem_block_boxes<3>), dim3(ncb), dim3(64), 0, c->stream, g,
                        cs, ncb, c->ev_box.ptr);
   BEMHIP_HIP_TRY(c, hipGetLastError());
   hipEvent_t ev[2];

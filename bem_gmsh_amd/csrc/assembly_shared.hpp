@@ -145,22 +145,37 @@ __device__ __forceinline__ void stage_class_block(const Geo& g, const Consts& cs
 // Bounding box of the collocation nodes of each 32-element column block and
 // its largest cutoff nf * L: box[cb] = {lox, loy, hix, hiy, maxlim}.
 // ---------------------------------------------------------------------------
+// One wavefront per column block: lanes over its nodes and elements, then lane
+// reductions (min / max are exact, so the order does not matter).
 template <int NN>
-__global__ void elem_block_boxes(Geo g, Consts cs, int ncb, double* __restrict__ box) {
-  const int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cb >= ncb) return;
+__device__ __forceinline__ void elem_box_wave(const Geo& g, const Consts& cs, int cb,
+                                              double* __restrict__ box) {
   const int e0 = cb * kEB, e1 = min(g.ne, e0 + kEB);
   double lox = INFINITY, loy = INFINITY, hix = -INFINITY, hiy = -INFINITY, ml = 0.0;
-  for (int e = e0; e < e1; ++e) {
-    for (int k = 0; k < NN; ++k) {
-      const double x = g.x[e * NN + k], y = g.y[e * NN + k];
-      lox = fmin(lox, x); hix = fmax(hix, x);
-      loy = fmin(loy, y); hiy = fmax(hiy, y);
-    }
-    ml = fmax(ml, cs.nf * g.elen[e]);
+  for (int t = e0 * NN + threadIdx.x; t < e1 * NN; t += 64) {
+    const double x = g.x[t], y = g.y[t];
+    lox = fmin(lox, x); hix = fmax(hix, x);
+    loy = fmin(loy, y); hiy = fmax(hiy, y);
   }
-  double* b = box + (size_t)cb * 5;
-  b[0] = lox; b[1] = loy; b[2] = hix; b[3] = hiy; b[4] = ml;
+  for (int e = e0 + threadIdx.x; e < e1; e += 64) ml = fmax(ml, cs.nf * g.elen[e]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lox = fmin(lox, __shfl_xor(lox, off, 64));
+    hix = fmax(hix, __shfl_xor(hix, off, 64));
+    loy = fmin(loy, __shfl_xor(loy, off, 64));
+    hiy = fmax(hiy, __shfl_xor(hiy, off, 64));
+    ml = fmax(ml, __shfl_xor(ml, off, 64));
+  }
+  if (threadIdx.x == 0) {
+    double* b = box + (size_t)cb * 5;
+    b[0] = lox; b[1] = loy; b[2] = hix; b[3] = hiy; b[4] = ml;
+  }
+}
+
+template <int NN>
+__global__ __launch_bounds__(64) void elem_block_boxes(Geo g, Consts cs, int ncb,
+                                                       double* __restrict__ box) {
+  if ((int)blockIdx.x < ncb) elem_box_wave<NN>(g, cs, blockIdx.x, box);
 }
 
 // ---------------------------------------------------------------------------

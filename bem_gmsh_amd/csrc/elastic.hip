@@ -838,12 +838,12 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
     const int cpb = 32;  // column blocks per classify workgroup
     dim3 grid((unsigned)nrb, (unsigned)((ncb + cpb - 1) / cpb));
     if (c->nn == 2) {
-      hipLaunchKernelGGL((elem_block_boxes<2>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+      hipLaunchKernelGGL((elem_block_boxes<2>), dim3(ncb), dim3(64), 0, c->stream, g,
                          cs, ncb, c->blkbox.ptr);
       hipLaunchKernelGGL((near_classify<2>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
                          cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
     } else {
-      hipLaunchKernelGGL((elem_block_boxes<3>), dim3((ncb + 63) / 64), dim3(64), 0, c->stream, g,
+      hipLaunchKernelGGL((elem_block_boxes<3>), dim3(ncb), dim3(64), 0, c->stream, g,
                          cs, ncb, c->blkbox.ptr);
       hipLaunchKernelGGL((near_classify<3>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
                          cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
