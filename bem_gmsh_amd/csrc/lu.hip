@@ -156,7 +156,7 @@ __device__ __forceinline__ bool better(T ov, int orr, T v, int r) {
 }
 
 // Reads the kb-value tagged record at `w` into s in the rotated order of column
-// step c (s[k] = column (c + k) mod W, zeros past kb); wave 0 only. Returns
+// step c (s[k] = column (c + k) mod W, zeros past kb); one whole wave. Returns
 // false if the wait ran out (abort raised).
 template <typename T, int W>
 __device__ bool read_record(const unsigned long long* w, unsigned tag, int kb, int c, T* s,
@@ -273,11 +273,14 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
   constexpr int RW = W * NWD;  // data words per record
   __shared__ __align__(16) T s_wrow[kPW][W];  // each wave's candidate row
   __shared__ __align__(16) T s_rowc[W];       // row at position c (workgroup 0)
-  __shared__ __align__(16) T s_pr[W];         // pivot row
-  __shared__ __align__(16) T s_rc[W];         // row c, for the owner of the pivot row
+  // s_rows[0]: the pivot row read from the winner's record; s_rows[1]: row c
+  // (for the owner of the pivot row, and the pivot row itself when p == c)
+  __shared__ __align__(16) T s_rows[2][W];
+  T* const s_pr = s_rows[0];
+  T* const s_rc = s_rows[1];
   __shared__ T s_wval[kPW];
   __shared__ int s_wrowi[kPW];
-  __shared__ int s_p, s_ok, s_abort;
+  __shared__ int s_p, s_ok, s_abort, s_use_rc;
   __shared__ int s_piv[W];  // workgroup 0: ipiv of the panel's columns
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int blk = blockIdx.x;
@@ -371,9 +374,17 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
           st_ag(&a.keys[(size_t)par * a.gstride + blk],
                 ((unsigned long long)k32 << 32) | ((unsigned long long)tag8 << 24) | row24);
         }
-        // poll the keys of all workgroups
+        // poll the keys of all workgroups and, in the same loads' round trip,
+        // row c (published by workgroup 0 with its key): the pivot row when
+        // p == c (the common case of a diagonally dominant matrix: no second
+        // dependent round trip), the interchange partner of the pivot row's
+        // owner otherwise
+        constexpr int RQ = (W + 63) / 64;
+        const unsigned long long* rcw = a.rowc + (size_t)par * RW;
+        T rcv[RQ];
         unsigned long long bk = 0;  // (key << 24) | (0xffffff - row): larger is better
         int bb = 0;
+        unsigned long long xk0 = 0, xk1 = 0;  // the key words of workgroups lane, lane + 64
         bool aborted = false;
         for (unsigned spin = 0;; ++spin) {
           bool ok = true;
@@ -381,11 +392,28 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
           int lb = 0;
           for (int q = lane; q < G; q += 64) {
             const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
+            if (q == lane) xk0 = x;
+            if (q == lane + 64) xk1 = x;
             if (((unsigned)(x >> 24) & 0xffu) != tag8) ok = false;
             const unsigned long long ck = ((x >> 32) << 24) | (0xffffffull - (x & 0xffffffull));
             if (ck > lk) { lk = ck; lb = q; }
           }
-          if (__all(ok)) { bk = lk; bb = lb; break; }
+#pragma unroll
+          for (int q = 0; q < RQ; ++q) {
+            const int j = lane + 64 * q;
+            rcv[q] = T(0);
+            if (j < kb) ok = TW::get(rcw + (size_t)j * NWD, tag32, rcv[q]) && ok;
+          }
+          if (__all(ok)) {
+            bk = lk;
+            bb = lb;
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+              const int j = lane + 64 * q;
+              if (j < W) s_rc[(j - c0 + W) % W] = rcv[q];
+            }
+            break;
+          }
           if (spin > kSpinLimit || ld_ag32(a.abort_flag)) {
             if (lane == 0) atomicExch(a.abort_flag, 1u);
             aborted = true;
@@ -393,25 +421,31 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
           }
         }
         int p = INT_MAX;
-        bool nz = false;
+        bool nz = false, use_rc = false;
         if (!aborted) {
-#pragma unroll
-          for (int off = 32; off >= 1; off >>= 1) {
-            const unsigned long long ok_ = __shfl_xor(bk, off, 64);
-            const int ob = __shfl_xor(bb, off, 64);
-            if (ok_ > bk) { bk = ok_; bb = ob; }
-          }
-          const unsigned kmax = (unsigned)(bk >> 24);
-          int pb = bb;
-          p = (int)(0xffffffull - (bk & 0xffffffull));
+          // the winner by two DPP wave maxima: the |a| key, then the lowest row
+          // (0xffffff - row largest) among the lanes holding that key
+          const unsigned lkey = (unsigned)(bk >> 24), linv = (unsigned)(bk & 0xffffffull);
+          const unsigned kmax = __ockl_wfred_max_u32(lkey);
+          const unsigned imax = __ockl_wfred_max_u32(lkey == kmax ? linv : 0u);
+          const unsigned long long hit = __ballot(lkey == kmax && linv == imax);
+          const int hl = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)hit) - 1);
+          int pb = __builtin_amdgcn_readlane(bb, hl);
+          p = (int)(0xffffffu - imax);
           if (sizeof(T) == 8) {
             // workgroups tied on the FP32-rounded key: decide on the exact values
+            auto tied = [&](unsigned long long x) {
+              return (unsigned)(x >> 32) == kmax && (x & 0xffffffull) != 0xffffffull;
+            };
             int ntie = 0;
-            for (int q = lane; q < G; q += 64) {
-              const unsigned long long x = ld_ag(&a.keys[(size_t)par * a.gstride + q]);
-              if ((unsigned)(x >> 32) == kmax && (x & 0xffffffull) != 0xffffffull) ++ntie;
+            if (G <= 128) {  // the key words of the last poll
+              ntie = __popcll(__ballot(lane < G && tied(xk0))) +
+                     __popcll(__ballot(lane + 64 < G && tied(xk1)));
+            } else {
+              for (int q = lane; q < G; q += 64)
+                if (tied(ld_ag(&a.keys[(size_t)par * a.gstride + q]))) ++ntie;
+              for (int off = 32; off >= 1; off >>= 1) ntie += __shfl_xor(ntie, off, 64);
             }
-            for (int off = 32; off >= 1; off >>= 1) ntie += __shfl_xor(ntie, off, 64);
             if (ntie > 1) {
               T ev = T(-1);
               int er = INT_MAX, eb = 0;
@@ -444,21 +478,19 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
               }
             }
           }
-          if (!aborted) {
+          if (!aborted && p == c) {
+            use_rc = true;  // the pivot row is row c, read with the keys
+          } else if (!aborted) {
             const unsigned long long* wrec = a.recs + ((size_t)par * a.gstride + pb) * RW;
             aborted = !read_record<T, W>(wrec, tag32, kb, c0, s_pr, a.abort_flag);
           }
-          if (!aborted) {
-            nz = s_pr[u] != T(0);
-            if (nz && p != c && p >= rbase && p < rbase + kPR)
-              aborted = !read_record<T, W>(a.rowc + (size_t)par * RW, tag32, kb, c0, s_rc,
-                                           a.abort_flag);
-          }
+          if (!aborted) nz = s_rows[use_rc ? 1 : 0][u] != T(0);
         }
         if (lane == 0) {
           s_p = p;
           s_ok = nz ? 1 : 0;
-          s_abort = aborted ? 1 : 0;
+          s_use_rc = use_rc ? 1 : 0;
+          if (aborted) s_abort = 1;
           if (!aborted && blk == 0) {
             a.ipiv[a.k0 + c] = a.k0 + (nz ? p : c);
             s_piv[c] = a.k0 + (nz ? p : c);
@@ -480,7 +512,8 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
             for (int k = 0; k < W; ++k) v[k] = s_pr[k];
           }
         }
-        const T piv = s_pr[u];
+        const T* const prow = s_rows[s_use_rc];  // the pivot row
+        const T piv = prow[u];
         const bool below = valid && r > c;
         // LAPACK's scaling: by the reciprocal, or by division below the safe minimum
         const bool recip = fabs(piv) >= std::numeric_limits<T>::min();
@@ -494,7 +527,7 @@ __global__ __launch_bounds__(kPR, 1) void panel_kernel(PanelArgs<T> a) {
           if (g0 < W - c0) {            // uniform: the group holds later columns
 #pragma unroll
             for (int k = g0; k < g0 + kPU; ++k)
-              if (k > u) v[k] -= l * s_pr[k];
+              if (k > u) v[k] -= l * prow[k];
           }
         }
       }
