@@ -187,13 +187,16 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs,
 // ---------------------------------------------------------------------------
 // FP64 operator pass: for every far pair recompute H_ij, G_ij exactly as
 // far_kernel<double> and accumulate acc1 += H wH1_j + G wG1_j (and acc2 with
-// wH2, wG2 when nvec == 2). Partial sums per 32-element column block.
+// wH2, wG2 when nvec == 2). Partial sums per 32-element column block. With
+// the block boxes of the assembly (cbox, rbox; null: none), all-far tiles skip
+// the pair tests as in far_kernel (the same decision, the same arithmetic).
 // ---------------------------------------------------------------------------
 template <int NN>
 __global__ __launch_bounds__(kFarThreads) void op64_kernel(
     Geo g, Consts cs, int nvec, const double* __restrict__ wH1, const double* __restrict__ wG1,
     const double* __restrict__ wH2, const double* __restrict__ wG2, double* __restrict__ part1,
-    double* __restrict__ part2) {
+    double* __restrict__ part2, const double* __restrict__ cbox,
+    const double* __restrict__ rbox) {
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
       s_jw[kEB * NN], s_L[kEB];
   __shared__ double s_w[4][kEB * NN];
@@ -216,6 +219,15 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
     const double lim = cs.nf * L;
     s_lim2f[t] = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
   }
+  bool all_far = false;  // uniform over the block
+  if (cbox && rbox) {
+    const double* bx = cbox + (size_t)cb * 5;
+    const double* rb = rbox + (size_t)blockIdx.x * 4;
+    const double dx = fmax(0.0, fmax(bx[0] - rb[1], rb[0] - bx[2]));
+    const double dy = fmax(0.0, fmax(bx[1] - rb[3], rb[2] - bx[3]));
+    const double ml = bx[4];
+    all_far = !(cs.nf > 1e20) && (dx * dx + dy * dy) > (ml * ml) * (1.0 + 1e-6);
+  }
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;
@@ -225,22 +237,27 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
     const double* cx = s_cx + el * NN;
     const double* cy = s_cy + el * NN;
     double rx[NN], ry[NN], d2[NN];
-    float rxf[NN], ryf[NN], d2f[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       rx[k] = cx[k] - px;
       ry[k] = cy[k] - py;
       d2[k] = rx[k] * rx[k] + ry[k] * ry[k];
-      rxf[k] = (float)rx[k];
-      ryf[k] = (float)ry[k];
-      d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
     }
-    const int col0 = (e0 + el) * NN;
-    const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
-    if (onel) continue;
-    if (!clearly_far<NN>(rxf, ryf, d2f, s_lim2f[el]) &&
-        is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf))
-      continue;
+    if (!all_far) {
+      float rxf[NN], ryf[NN], d2f[NN];
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        rxf[k] = (float)rx[k];
+        ryf[k] = (float)ry[k];
+        d2f[k] = rxf[k] * rxf[k] + ryf[k] * ryf[k];
+      }
+      const int col0 = (e0 + el) * NN;
+      const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
+      if (onel) continue;
+      if (!clearly_far<NN>(rxf, ryf, d2f, s_lim2f[el]) &&
+          is_near<NN>(px, py, cx, cy, d2, s_L[el], cs.nf))
+        continue;
+    }
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       const int t = el * NN + k;
@@ -942,10 +959,12 @@ int assemble_impl(bemhip_ctx* c) {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
     if (c->nn == 2)
       hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, s2, g, cs, 1, c->wH.ptr,
-                         c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
+                         c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr, c->blkbox.ptr,
+                         c->rowbox.ptr);
     else
       hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, s2, g, cs, 1, c->wH.ptr,
-                         c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr);
+                         c->wG.ptr, nullptr, nullptr, c->hpart.ptr, nullptr, c->blkbox.ptr,
+                         c->rowbox.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
     st = BEMHIP_OK;
   } else if (fp64)
@@ -1097,12 +1116,13 @@ static int operator_pass(bemhip_ctx* c, const double* wH, const double* wG, doub
   const int ncb = (c->ne + kEB - 1) / kEB;
   BEMHIP_HIP_TRY(c, c->part1.ensure((size_t)ncb * nt));
   dim3 grid((unsigned)((nt + kFarThreads - 1) / kFarThreads), (unsigned)ncb);
+  // the block boxes of the (non-FMM) assembly this operator belongs to
   if (c->nn == 2)
     hipLaunchKernelGGL(op64_kernel<2>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1, wH, wG,
-                       nullptr, nullptr, c->part1.ptr, nullptr);
+                       nullptr, nullptr, c->part1.ptr, nullptr, c->blkbox.ptr, c->rowbox.ptr);
   else
     hipLaunchKernelGGL(op64_kernel<3>, grid, dim3(kFarThreads), 0, c->stream, g, cs, 1, wH, wG,
-                       nullptr, nullptr, c->part1.ptr, nullptr);
+                       nullptr, nullptr, c->part1.ptr, nullptr, c->blkbox.ptr, c->rowbox.ptr);
   hipLaunchKernelGGL((op64_finalize<double>), dim3(nfb), dim3(256), 0, c->stream, (int)nt, c->n,
                      c->nn, ncb, 1, (size_t)0, c->part1.ptr, c->row_ptr.ptr, c->pair_elem.ptr,
                      c->pair_h.ptr, c->pair_g.ptr, wH, wG, c->diag64.ptr, d_y,
@@ -1228,7 +1248,7 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
     const int n = g.n;
     hipLaunchKernelGGL(op64_kernel<NN>, grid, dim3(kFarThreads), 0, s2, g, cs, 2, d_w, d_w + n,
                        d_w + 2 * (size_t)n, d_w + 3 * (size_t)n, part,
-                       part + (size_t)ncb * npts);
+                       part + (size_t)ncb * npts, nullptr, nullptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
     BEMHIP_HIP_TRY(c, hipEventRecord(far_done, s2));
   }
