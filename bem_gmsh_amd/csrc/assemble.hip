@@ -1239,6 +1239,11 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
   const int ncb = (g.ne + kEB - 1) / kEB;
   const int nrb = (npts + kFarThreads - 1) / kFarThreads;
   hipStream_t s1 = c->stream, s2 = c->stream2;
+  // node boxes of the point blocks (the all-far tile test of the far field)
+  // and the points' near-pair counts cleared
+  hipLaunchKernelGGL((block_boxes<NN>), dim3((unsigned)nrb), dim3(64), 0, s1, g, cs, 0,
+                     (double*)nullptr, c->ev_rowbox.ptr, c->ev_cnt.ptr);
+  BEMHIP_HIP_TRY(c, hipGetLastError());
   BEMHIP_HIP_TRY(c, hipEventRecord(start, s1));
   // far field: FP64 contraction of the far pairs with (-T, q) and (1, 0), stream 2
   if (!all_near) {
@@ -1248,12 +1253,11 @@ static int interior_chunk(bemhip_ctx* c, const Geo& g, const Consts& cs, const Q
     const int n = g.n;
     hipLaunchKernelGGL(op64_kernel<NN>, grid, dim3(kFarThreads), 0, s2, g, cs, 2, d_w, d_w + n,
                        d_w + 2 * (size_t)n, d_w + 3 * (size_t)n, part,
-                       part + (size_t)ncb * npts, nullptr, nullptr);
+                       part + (size_t)ncb * npts, c->ev_box.ptr, c->ev_rowbox.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
     BEMHIP_HIP_TRY(c, hipEventRecord(far_done, s2));
   }
   // near pairs: classification, CSR, near rule (stream 1)
-  BEMHIP_HIP_TRY(c, hipMemsetAsync(c->ev_cnt.ptr, 0, sizeof(int) * npts, s1));
   const int cpb = 32;
   dim3 cgrid((unsigned)nrb, (unsigned)((ncb + cpb - 1) / cpb));
   hipLaunchKernelGGL((near_classify<NN>), cgrid, dim3(kFarThreads), 0, s1, g, cs, ncb, cpb,
@@ -1310,6 +1314,7 @@ int interior_impl(bemhip_ctx* c, int npts, const double* d_px, const double* d_p
   BEMHIP_HIP_TRY(c, c->ev_cnt.ensure((size_t)chunk));
   BEMHIP_HIP_TRY(c, c->ev_ptr.ensure((size_t)chunk + 1));
   BEMHIP_HIP_TRY(c, c->ev_tiles.ensure((size_t)nrb * ncb));
+  BEMHIP_HIP_TRY(c, c->ev_rowbox.ensure((size_t)nrb * 4));
   if (!all_near) BEMHIP_HIP_TRY(c, c->ev_part.ensure(2 * (size_t)ncb * chunk));
   BEMHIP_HIP_TRY(c, ensure_count_word(c));
   if (c->nn == 2)

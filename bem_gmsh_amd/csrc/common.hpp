@@ -173,6 +173,7 @@ struct bemhip_ctx {
   bemhip::DevBuf<double> ev_w;      // [4][n] -T, q, 1, 0
   bemhip::DevBuf<double> ev_out;    // [npts]
   bemhip::DevBuf<double> ev_box;    // [ncb][5]
+  bemhip::DevBuf<double> ev_rowbox; // [point block][4]
   bemhip::DevBuf<uint32_t> ev_bits; // [ncb][chunk]
   bemhip::DevBuf<int> ev_cnt, ev_ptr, ev_prow, ev_pelem;
   bemhip::DevBuf<uint8_t> ev_tiles;
