@@ -236,8 +236,8 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
     if ((st = lu_solve_device(c, nt, 1, c->A32.ptr, (int)c->ld32, c->perm32.ptr, dx))) return st;
     bool converged = false;
     double dn_prev = INFINITY;
-    for (const char* key : {"refine_stagnated", "refine_gmres_ir", "refine_gmres_outer",
-                            "refine_gmres_products"})
+    for (const char* key : {"refine_stagnated", "refine_predicted", "refine_gmres_ir",
+                            "refine_gmres_outer", "refine_gmres_products"})
       c->stats.erase(key);
     for (auto it2 = c->stats.begin(); it2 != c->stats.end();)
       it2 = (it2->first.rfind("refine_dx_", 0) == 0 || it2->first.rfind("refine_gmres_", 0) == 0)
@@ -259,6 +259,17 @@ extern "C" int bemhip_solve(bemhip_ctx* c, int mode, double tol, int maxit, doub
       if (!std::isfinite(dn)) break;
       hipLaunchKernelGGL(axpy_kernel, dim3(nb), dim3(tb), 0, c->stream, nt, dr, dx);
       if (dn <= tol * xn) { converged = true; ++it; break; }
+      // The error left after this correction is about rho * dn, rho = dn / dn_prev
+      // the contraction of the step (the refinement converges linearly with the
+      // rate ~ kappa u32): stop when that estimate is ten times below tol and the
+      // step contracted by at least ten, instead of spending one more operator
+      // pass and solve to observe a correction below tol.
+      if (it >= 1 && dn < 0.1 * dn_prev && (dn / dn_prev) * dn <= 0.1 * tol * xn) {
+        converged = true;
+        c->stats["refine_predicted"] = 1.0;
+        ++it;
+        break;
+      }
       // Stagnation at the FP64 noise floor (kappa * eps64): the correction no
       // longer halves and is already below 1e-10 relative.
       if (it >= 1 && dn > 0.5 * dn_prev && dn <= 1e-10 * xn) {
