@@ -1,13 +1,17 @@
 // Influence-matrix assembly for the dense 2-D Laplace BEM path (gfx950).
 //
-// Kernels (far_kernel on a second stream, concurrent with the near chain):
+// Kernels (three streams: the far field, the near-pair chain and the
+// on-element rule run concurrently):
+//   block_boxes     node bounding boxes of the 32-element column blocks and of
+//                   the 256-row blocks (the all-far tile test)
 //   near_classify   near/far classification of every (row, element) pair
 //                   (src/Core/Assembly_full.jl:736-754) into a bit per pair and
 //                   per-row pair counts; column blocks far from the rows'
 //                   bounding box are settled without pair tests.
 //   far_kernel      rows x 32-element column blocks: the same classification
-//                   and the far nodal lumping (:801-810) written column-major
-//                   with lanes over rows (coalesced), FP64 row sums.
+//                   (skipped on all-far tiles) and the far nodal lumping
+//                   (:801-810) written column-major with lanes over rows
+//                   (coalesced), FP64 row sums.
 //   near_scan / near_fill   counts and near bits -> row-sorted CSR of pairs.
 //   near_reg_kernel / near_sing_kernel   one lane group per near pair:
 //                   tan-p3c rule for off-element pairs (:649-695,
@@ -15,7 +19,8 @@
 //                   Guiggiani for on-element pairs (:397-424,
 //                   src/Core/Integration.jl:1597-1845); lanes over quadrature
 //                   points, group reductions; the two kinds in separate
-//                   launches.
+//                   launches (the on-element rule by row, before the pair
+//                   list exists; sing_scatter moves its values into the list).
 //   op64_finalize   rigid-body diagonal H_ii = -sum_{j!=i} H_ij (:384-390) and
 //                   the near / on-element entries into H and G.
 //   op64_kernel / op64_finalize   FP64 operator passes that recompute the far
