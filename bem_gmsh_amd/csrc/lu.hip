@@ -1801,6 +1801,44 @@ bool gemm_aligned(const void* A, int lda, const void* B, int ldb, int vec) {
 
 // X = Linv X in place (M <= 128 rows of X, Linv M x M with leading dimension
 // kSolveNB), one 128-row tile row. Requires 16-byte aligned operands.
+// X := L^-1 X for a unit lower 128 x 128 block L (column-major, ld) and the
+// ncols columns of X (column-major, ldx): forward substitution, 32 columns per
+// workgroup of 256 threads (thread = row i of the block, 16 of the columns),
+// row i of L in registers (128 coalesced column loads), the finished row k
+// broadcast through LDS once per step (double-buffered: one barrier per step).
+// The in-block U12 of the FP32 LU, so that the panel chain does not wait for
+// the block inverse.
+constexpr int kTrsmCols = 32;
+__global__ __launch_bounds__(256) void trsm_unit_lower128(const float* __restrict__ L, int ld,
+                                                          float* __restrict__ X, int ldx,
+                                                          int ncols) {
+  __shared__ __align__(16) float s_row[2][2][16];
+  const int i = threadIdx.x & 127, h = threadIdx.x >> 7;
+  const int j0 = blockIdx.x * kTrsmCols + h * 16;
+  float l[128];
+#pragma unroll
+  for (int k = 0; k < 128; ++k) l[k] = (k < i) ? L[(size_t)i + (size_t)k * ld] : 0.f;
+  float x[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) x[q] = (j0 + q < ncols) ? X[(size_t)i + (size_t)(j0 + q) * ldx] : 0.f;
+#pragma unroll
+  for (int k = 0; k < 128; ++k) {
+    float* row = s_row[k & 1][h];
+    if (i == k) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) row[q] = x[q];
+    }
+    __syncthreads();
+    if (i > k) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) x[q] -= l[k] * row[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if (j0 + q < ncols) X[(size_t)i + (size_t)(j0 + q) * ldx] = x[q];
+}
+
 template <typename T>
 void gemm_set_inplace(bemhip_ctx* c, int M, int N, const T* Linv, T* X, int ldx) {
   if (M <= 0 || N <= 0) return;
@@ -1924,7 +1962,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   auto l_len = [&](int q) { return lists + (size_t)q * LSLOT + 4 * kMaxPanel; };
   if ((size_t)(wp - c->lu_work.ptr) > c->lu_work.count)
     return fail(c, BEMHIP_ERR_STATE, "lu: work buffer carve overflow");
-  hipStream_t sA = c->stream, sB = c->stream2;
+  hipStream_t sA = c->stream, sB = c->stream2, sC = c->stream3;
   BEMHIP_HIP_TRY(c, hipMemsetAsync(flags, 0, sizeof(int) * 16, sA));
   BEMHIP_HIP_TRY(c, hipMemsetAsync(keys, 0, 8 * 4 * (size_t)Gmax, sA));
   if (tslu) BEMHIP_HIP_TRY(c, hipMemsetAsync(t_count, 0, 4 * tnodes, sA));
@@ -2043,12 +2081,22 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
         const int bw = std::min(kSolveNB, ob - jb * kSolveNB);
         const T* Fb = dA + (size_t)d0 + (size_t)d0 * lda;
         T* Db = linv + (size_t)jb * kSolveNB * kSolveNB;
+        // on stream C when the block is factored on stream B: only stream A's
+        // TRSM needs the inverses (the in-block U12 is solved by substitution
+        // in FP32, by trsm_rec in FP64), and it waits for them
+        hipStream_t si = st;
+        if (st == sB) {
+          hipEvent_t e = ev_now(st);
+          joins.push_back(e);
+          BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sC, e, 0));
+          si = sC;
+        }
         if constexpr (sizeof(T) == 4)
-          hipLaunchKernelGGL(tri_inv_lower_blocked, dim3(1), dim3(512), 0, st, bw,
+          hipLaunchKernelGGL(tri_inv_lower_blocked, dim3(1), dim3(512), 0, si, bw,
                              (const float*)Fb, lda, (float*)Db);
         else
           hipLaunchKernelGGL((tri_inv_kernel<T, true>), dim3(kTriInvGroups),
-                             dim3(kTriInvThreads), tri_inv_lds(sizeof(T)), st, bw, Fb, lda, Db);
+                             dim3(kTriInvThreads), tri_inv_lds(sizeof(T)), si, bw, Fb, lda, Db);
         BEMHIP_HIP_TRY(c, hipGetLastError());
       }
       // the rest of this outer block
@@ -2056,6 +2104,21 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       if (rest > 0) {
         T* L11 = dA + (size_t)kc + (size_t)kc * lda;
         T* A12 = dA + (size_t)kc + (size_t)(kc + kb) * lda;
+        if constexpr (sizeof(T) == 4) {
+          if (kb == 128) {  // U12 = L11^-1 A12 by substitution: no wait for the inverse
+            hipLaunchKernelGGL(trsm_unit_lower128, dim3((rest + kTrsmCols - 1) / kTrsmCols),
+                               dim3(256), 0, st, (const float*)L11, lda, (float*)A12, lda, rest);
+            BEMHIP_HIP_TRY(c, hipGetLastError());
+            seg_end(st, e0, kTrsm);
+            e0 = ev_now(st);
+            T* L21 = dA + (size_t)(kc + kb) + (size_t)kc * lda;
+            T* A22 = dA + (size_t)(kc + kb) + (size_t)(kc + kb) * lda;
+            gemm_sub(c, n - kc - kb, rest, kb, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
+            BEMHIP_HIP_TRY(c, hipGetLastError());
+            seg_end(st, e0, kGemm);
+            continue;
+          }
+        }
         if (aligned && KB == kSolveNB) {  // the panel is one 128-block: U12 = L11^-1 A12
           gemm_set_inplace(c, kb, rest,
                            (const T*)(linv + (size_t)(s / kSolveNB) * kSolveNB * kSolveNB), A12,
@@ -2152,10 +2215,13 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     if ((st = factor_block(sB, k1, ob1))) return st;
     hipEvent_t factored = ev_now(sB);
     joins.push_back(factored);
+    hipEvent_t inverted = ev_now(sC);  // block k1's L11 inverses (stream C)
+    joins.push_back(inverted);
     // meanwhile the remaining trailing columns
     if ((st = outer_trsm(k0, ob, k2, n))) return st;
     if ((st = outer_gemm(k0, ob, k2, n))) return st;
     BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sA, factored, 0));
+    BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sA, inverted, 0));
     if ((st = outer_perms(k1, ob1))) return st;
   }
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(sA));
