@@ -1802,41 +1802,56 @@ bool gemm_aligned(const void* A, int lda, const void* B, int ldb, int vec) {
 // X = Linv X in place (M <= 128 rows of X, Linv M x M with leading dimension
 // kSolveNB), one 128-row tile row. Requires 16-byte aligned operands.
 // X := L^-1 X for a unit lower 128 x 128 block L (column-major, ld) and the
-// ncols columns of X (column-major, ldx): forward substitution, 32 columns per
-// workgroup of 256 threads (thread = row i of the block, 16 of the columns),
-// row i of L in registers (128 coalesced column loads), the finished row k
-// broadcast through LDS once per step (double-buffered: one barrier per step).
-// The in-block U12 of the FP32 LU, so that the panel chain does not wait for
-// the block inverse.
+// ncols columns of X (column-major, ldx): forward substitution with L in LDS
+// and every wave on 8 columns of its own (lane l holds rows l and l + 64), the
+// finished row k taken from its lane with v_readlane: no barrier per step.
+// 32 columns per workgroup of 4 waves. The in-block U12 of the FP32 LU, so
+// that the panel chain does not wait for the block inverse.
 constexpr int kTrsmCols = 32;
 __global__ __launch_bounds__(256) void trsm_unit_lower128(const float* __restrict__ L, int ld,
                                                           float* __restrict__ X, int ldx,
                                                           int ncols) {
-  __shared__ __align__(16) float s_row[2][2][16];
-  const int i = threadIdx.x & 127, h = threadIdx.x >> 7;
-  const int j0 = blockIdx.x * kTrsmCols + h * 16;
-  float l[128];
+  __shared__ __align__(16) float sL[128 * 128];  // column-major, 64 KB
+  for (int t = threadIdx.x; t < 128 * 32; t += 256) {  // float4 per (column, row quad)
+    const int k = t >> 5, r = (t & 31) * 4;
+    const float* src = L + (size_t)k * ld + r;
+    *(float4*)(sL + k * 128 + r) = make_float4(src[0], src[1], src[2], src[3]);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j0 = blockIdx.x * kTrsmCols + w * 8;
+  float xa[8], xb[8];
 #pragma unroll
-  for (int k = 0; k < 128; ++k) l[k] = (k < i) ? L[(size_t)i + (size_t)k * ld] : 0.f;
-  float x[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) x[q] = (j0 + q < ncols) ? X[(size_t)i + (size_t)(j0 + q) * ldx] : 0.f;
-#pragma unroll
-  for (int k = 0; k < 128; ++k) {
-    float* row = s_row[k & 1][h];
-    if (i == k) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) row[q] = x[q];
-    }
-    __syncthreads();
-    if (i > k) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) x[q] -= l[k] * row[q];
-    }
+  for (int q = 0; q < 8; ++q) {
+    const bool ok = j0 + q < ncols;
+    xa[q] = ok ? X[(size_t)lane + (size_t)(j0 + q) * ldx] : 0.f;
+    xb[q] = ok ? X[(size_t)(lane + 64) + (size_t)(j0 + q) * ldx] : 0.f;
   }
 #pragma unroll
-  for (int q = 0; q < 16; ++q)
-    if (j0 + q < ncols) X[(size_t)i + (size_t)(j0 + q) * ldx] = x[q];
+  for (int k = 0; k < 127; ++k) {
+    float xk[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      xk[q] = __int_as_float(
+          __builtin_amdgcn_readlane(__float_as_int(k < 64 ? xa[q] : xb[q]), k & 63));
+    if (k < 63) {
+      const float la = sL[k * 128 + lane];
+      const bool below = lane > k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) xa[q] = below ? xa[q] - la * xk[q] : xa[q];
+    }
+    const float lb = sL[k * 128 + 64 + lane];
+    const bool below_b = lane + 64 > k;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) xb[q] = below_b ? xb[q] - lb * xk[q] : xb[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if (j0 + q < ncols) {
+      X[(size_t)lane + (size_t)(j0 + q) * ldx] = xa[q];
+      X[(size_t)(lane + 64) + (size_t)(j0 + q) * ldx] = xb[q];
+    }
+  }
 }
 
 template <typename T>
