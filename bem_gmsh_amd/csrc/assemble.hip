@@ -59,41 +59,6 @@ namespace {
 // before the kernel ends.
 // ---------------------------------------------------------------------------
 
-// Node bounding boxes of the column blocks (cbox, as elem_block_boxes) and of
-// the 256-row blocks (rbox[rb] = {lox, hix, loy, hiy}), one wavefront per
-// block in one launch: workgroups [0, ncb) take the column blocks, the rest
-// the row blocks, clearing the rows' near-pair counts on the way.
-template <int NN>
-__global__ __launch_bounds__(64) void block_boxes(Geo g, Consts cs, int ncb,
-                                                  double* __restrict__ cbox,
-                                                  double* __restrict__ rbox,
-                                                  int* __restrict__ cnt) {
-  if ((int)blockIdx.x < ncb) {
-    elem_box_wave<NN>(g, cs, blockIdx.x, cbox);
-    return;
-  }
-  const int rb = blockIdx.x - ncb;
-  const int r0 = rb * kFarThreads;
-  double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
-  for (int r = r0 + threadIdx.x; r < min(g.nt, r0 + kFarThreads); r += 64) {
-    cnt[r] = 0;
-    const double x = g.rx[r], y = g.ry[r];
-    lox = fmin(lox, x); hix = fmax(hix, x);
-    loy = fmin(loy, y); hiy = fmax(hiy, y);
-  }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    lox = fmin(lox, __shfl_xor(lox, off, 64));
-    hix = fmax(hix, __shfl_xor(hix, off, 64));
-    loy = fmin(loy, __shfl_xor(loy, off, 64));
-    hiy = fmax(hiy, __shfl_xor(hiy, off, 64));
-  }
-  if (threadIdx.x == 0) {
-    double* b = rbox + (size_t)rb * 4;
-    b[0] = lox; b[1] = hix; b[2] = loy; b[3] = hiy;
-  }
-}
-
 template <typename TS, int NN>
 __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs,
                                                           const double* __restrict__ cbox,
@@ -118,16 +83,7 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs,
       s_jw[t] = g.jw[j];
     }
   }
-  // all-far tile test (uniform over the block)
-  bool all_far = false;
-  {
-    const double* bx = cbox + (size_t)cb * 5;
-    const double* rb = rbox + (size_t)blockIdx.x * 4;
-    const double dx = fmax(0.0, fmax(bx[0] - rb[1], rb[0] - bx[2]));
-    const double dy = fmax(0.0, fmax(bx[1] - rb[3], rb[2] - bx[3]));
-    const double ml = bx[4];
-    all_far = !(cs.nf > 1e20) && (dx * dx + dy * dy) > (ml * ml) * (1.0 + 1e-6);
-  }
+  const bool all_far = tile_all_far(cbox, rbox, cb, blockIdx.x, cs.nf);
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;
@@ -224,15 +180,7 @@ __global__ __launch_bounds__(kFarThreads) void op64_kernel(
     const double lim = cs.nf * L;
     s_lim2f[t] = (cs.nf > 1e20) ? INFINITY : (float)((lim * lim) * (1.0 + 1e-3));
   }
-  bool all_far = false;  // uniform over the block
-  if (cbox && rbox) {
-    const double* bx = cbox + (size_t)cb * 5;
-    const double* rb = rbox + (size_t)blockIdx.x * 4;
-    const double dx = fmax(0.0, fmax(bx[0] - rb[1], rb[0] - bx[2]));
-    const double dy = fmax(0.0, fmax(bx[1] - rb[3], rb[2] - bx[3]));
-    const double ml = bx[4];
-    all_far = !(cs.nf > 1e20) && (dx * dx + dy * dy) > (ml * ml) * (1.0 + 1e-6);
-  }
+  const bool all_far = tile_all_far(cbox, rbox, cb, blockIdx.x, cs.nf);
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;

@@ -178,6 +178,56 @@ __global__ __launch_bounds__(64) void elem_block_boxes(Geo g, Consts cs, int ncb
   if ((int)blockIdx.x < ncb) elem_box_wave<NN>(g, cs, blockIdx.x, box);
 }
 
+// Node bounding boxes of the column blocks (cbox, as elem_block_boxes) and of
+// the 256-row blocks (rbox[rb] = {lox, hix, loy, hiy}), one wavefront per
+// block in one launch: workgroups [0, ncb) take the column blocks, the rest
+// the row blocks, clearing the rows' near-pair counts on the way.
+template <int NN>
+__global__ __launch_bounds__(64) void block_boxes(Geo g, Consts cs, int ncb,
+                                                  double* __restrict__ cbox,
+                                                  double* __restrict__ rbox,
+                                                  int* __restrict__ cnt) {
+  if ((int)blockIdx.x < ncb) {
+    elem_box_wave<NN>(g, cs, blockIdx.x, cbox);
+    return;
+  }
+  const int rb = blockIdx.x - ncb;
+  const int r0 = rb * kFarThreads;
+  double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
+  for (int r = r0 + threadIdx.x; r < min(g.nt, r0 + kFarThreads); r += 64) {
+    cnt[r] = 0;
+    const double x = g.rx[r], y = g.ry[r];
+    lox = fmin(lox, x); hix = fmax(hix, x);
+    loy = fmin(loy, y); hiy = fmax(hiy, y);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lox = fmin(lox, __shfl_xor(lox, off, 64));
+    hix = fmax(hix, __shfl_xor(hix, off, 64));
+    loy = fmin(loy, __shfl_xor(loy, off, 64));
+    hiy = fmax(hiy, __shfl_xor(hiy, off, 64));
+  }
+  if (threadIdx.x == 0) {
+    double* b = rbox + (size_t)rb * 4;
+    b[0] = lox; b[1] = hix; b[2] = loy; b[3] = hiy;
+  }
+}
+
+// True when every pair of the tile (row block rb, column block cb) is far:
+// the row block's node box lies farther than (1 + 1e-6) x the column block's
+// largest cutoff from its node box (uniform over a workgroup).
+__device__ __forceinline__ bool tile_all_far(const double* __restrict__ cbox,
+                                             const double* __restrict__ rbox, int cb, int rb,
+                                             double nf) {
+  if (!cbox || !rbox || nf > 1e20) return false;
+  const double* bx = cbox + (size_t)cb * 5;
+  const double* r = rbox + (size_t)rb * 4;
+  const double dx = fmax(0.0, fmax(bx[0] - r[1], r[0] - bx[2]));
+  const double dy = fmax(0.0, fmax(bx[1] - r[3], r[2] - bx[3]));
+  const double ml = bx[4];
+  return (dx * dx + dy * dy) > (ml * ml) * (1.0 + 1e-6);
+}
+
 // ---------------------------------------------------------------------------
 // Near bits of every (row, element) pair (src/Core/Assembly_full.jl:736-754)
 // and the per-row near-pair counts: the classification of far_kernel without

@@ -131,6 +131,8 @@ __device__ __forceinline__ void kelvin_UT_op(const Kelvin& K, double rx, double 
 // STORE = false: the row sums only (the matrix-free assembly's diagonal).
 template <typename TS, int NN, bool STORE = true>
 __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts cs, Kelvin K,
+                                                                 const double* __restrict__ cbox,
+                                                                 const double* __restrict__ rbox,
                                                                  TS* __restrict__ H,
                                                                  TS* __restrict__ G,
                                                                  double* __restrict__ hpart) {
@@ -147,6 +149,8 @@ __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts c
     s_nx[t] = g.nx[j]; s_ny[t] = g.ny[j];
     s_jw[t] = g.jw[j];
   }
+  // all-far tiles (the box test of near_classify) skip the pair tests
+  const bool all_far = tile_all_far(cbox, rbox, cb, blockIdx.x, cs.nf);
   __syncthreads();
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   if (i >= g.nt) return;
@@ -155,11 +159,20 @@ __global__ __launch_bounds__(kFarThreads) void kelvin_far_kernel(Geo g, Consts c
   double rs[4] = {0.0, 0.0, 0.0, 0.0};
   for (int el = 0; el < ecount; ++el) {
     double rx[NN], ry[NN];
-    float rxf[NN], ryf[NN], d2f[NN];
     const int col0 = (e0 + el) * NN;
-    const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
-    const bool nr = classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el],
-                                      s_lim2f[el], cs.nf, onel, rx, ry, rxf, ryf, d2f);
+    bool nr = false;
+    if (all_far) {
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        rx[k] = s_cx[el * NN + k] - px;
+        ry[k] = s_cy[el * NN + k] - py;
+      }
+    } else {
+      float rxf[NN], ryf[NN], d2f[NN];
+      const bool onel = (i + g.row0 >= col0) && (i + g.row0 < col0 + NN);
+      nr = classify_pair<NN>(px, py, s_cx + el * NN, s_cy + el * NN, s_L[el], s_lim2f[el], cs.nf,
+                             onel, rx, ry, rxf, ryf, d2f);
+    }
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
       const size_t c0 = (size_t)(2 * (col0 + k)) * ld + 2 * (size_t)i;  // column 2j, row 2i
@@ -808,6 +821,7 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   BEMHIP_HIP_TRY(c, c->row_cnt.ensure(nt));
   BEMHIP_HIP_TRY(c, c->row_ptr.ensure(nt + 1));
   BEMHIP_HIP_TRY(c, c->blkbox.ensure((size_t)ncb * 5));
+  BEMHIP_HIP_TRY(c, c->rowbox.ensure((size_t)nrb * 4));
   BEMHIP_HIP_TRY(c, c->tiles.ensure((size_t)nrb * ncb));
   if (mfree) {
     // no stored H, G: the far pass below computes the row sums only
@@ -824,7 +838,6 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
       BEMHIP_HIP_TRY(c, hipMemsetAsync(c->H32.ptr + nb * c->ld32, 0,
                                        sizeof(float) * c->ld32 * (ns - nb), c->stream));
   }
-  BEMHIP_HIP_TRY(c, hipMemsetAsync(c->row_cnt.ptr, 0, sizeof(int) * nt, c->stream));
   // events: 0 start, 1 classified, 2 far done, 3 far start, 4 pairs counted,
   // 5 near done, 6 finalized
   hipEvent_t ev[7];
@@ -837,14 +850,17 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
   {
     const int cpb = 32;  // column blocks per classify workgroup
     dim3 grid((unsigned)nrb, (unsigned)((ncb + cpb - 1) / cpb));
+    // column- and row-block boxes (the all-far tile test of near_classify and
+    // kelvin_far_kernel); row_cnt cleared on the way
+    const unsigned nbx = (unsigned)(ncb + nrb);
     if (c->nn == 2) {
-      hipLaunchKernelGGL((elem_block_boxes<2>), dim3(ncb), dim3(64), 0, c->stream, g,
-                         cs, ncb, c->blkbox.ptr);
+      hipLaunchKernelGGL((block_boxes<2>), dim3(nbx), dim3(64), 0, c->stream, g, cs, ncb,
+                         c->blkbox.ptr, c->rowbox.ptr, c->row_cnt.ptr);
       hipLaunchKernelGGL((near_classify<2>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
                          cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
     } else {
-      hipLaunchKernelGGL((elem_block_boxes<3>), dim3(ncb), dim3(64), 0, c->stream, g,
-                         cs, ncb, c->blkbox.ptr);
+      hipLaunchKernelGGL((block_boxes<3>), dim3(nbx), dim3(64), 0, c->stream, g, cs, ncb,
+                         c->blkbox.ptr, c->rowbox.ptr, c->row_cnt.ptr);
       hipLaunchKernelGGL((near_classify<3>), grid, dim3(kFarThreads), 0, c->stream, g, cs, ncb,
                          cpb, c->blkbox.ptr, c->nearbits.ptr, c->row_cnt.ptr, c->tiles.ptr);
     }
@@ -888,22 +904,28 @@ int kelvin_assemble_impl(bemhip_ctx* c) {
     dim3 grid((unsigned)nrb, (unsigned)ncb);
     if (mfree && c->nn == 2)
       hipLaunchKernelGGL((kelvin_far_kernel<double, 2, false>), grid, dim3(kFarThreads), 0,
-                         c->stream, g, cs, K, nullptr, nullptr, c->hpart.ptr);
+                         c->stream, g, cs, K, c->blkbox.ptr, c->rowbox.ptr,
+                         nullptr, nullptr, c->hpart.ptr);
     else if (mfree)
       hipLaunchKernelGGL((kelvin_far_kernel<double, 3, false>), grid, dim3(kFarThreads), 0,
-                         c->stream, g, cs, K, nullptr, nullptr, c->hpart.ptr);
+                         c->stream, g, cs, K, c->blkbox.ptr, c->rowbox.ptr,
+                         nullptr, nullptr, c->hpart.ptr);
     else if (fp64 && c->nn == 2)
       hipLaunchKernelGGL((kelvin_far_kernel<double, 2>), grid, dim3(kFarThreads), 0, c->stream,
-                         g, cs, K, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+                         g, cs, K, c->blkbox.ptr, c->rowbox.ptr,
+                         c->H64.ptr, c->G64.ptr, c->hpart.ptr);
     else if (fp64)
       hipLaunchKernelGGL((kelvin_far_kernel<double, 3>), grid, dim3(kFarThreads), 0, c->stream,
-                         g, cs, K, c->H64.ptr, c->G64.ptr, c->hpart.ptr);
+                         g, cs, K, c->blkbox.ptr, c->rowbox.ptr,
+                         c->H64.ptr, c->G64.ptr, c->hpart.ptr);
     else if (c->nn == 2)
       hipLaunchKernelGGL((kelvin_far_kernel<float, 2>), grid, dim3(kFarThreads), 0, c->stream,
-                         g, cs, K, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
+                         g, cs, K, c->blkbox.ptr, c->rowbox.ptr,
+                         c->H32.ptr, c->G32.ptr, c->hpart.ptr);
     else
       hipLaunchKernelGGL((kelvin_far_kernel<float, 3>), grid, dim3(kFarThreads), 0, c->stream,
-                         g, cs, K, c->H32.ptr, c->G32.ptr, c->hpart.ptr);
+                         g, cs, K, c->blkbox.ptr, c->rowbox.ptr,
+                         c->H32.ptr, c->G32.ptr, c->hpart.ptr);
     BEMHIP_HIP_TRY(c, hipGetLastError());
   }
   BEMHIP_HIP_TRY(c, hipEventRecord(ev[2], c->stream));
