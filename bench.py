@@ -12,7 +12,8 @@ T = e^x cos y, Dirichlet/Neumann arcs of 64 elements.
 A step = one pass of the hot path with the geometry resident in HBM:
 assemble H, G (far field FP32, near/singular/diagonal FP64) -> mixed-BC
 column swap and FP64 right-hand side -> FP32 LU (MFMA trailing updates) ->
-FP64 iterative refinement until ||dx|| <= 1e-10 ||x||. The all-FP64 pipeline
+FP64 iterative refinement until ||dx|| <= 1e-10 ||x|| (or until the error left,
+estimated from the observed contraction, is ten times below that). The all-FP64 pipeline
 (FP64 assembly + FP64 LU) runs once after the timed steps for the speed-up and
 the accuracy check (mixed vs FP64, per field).
 
