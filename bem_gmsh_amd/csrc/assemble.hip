@@ -66,6 +66,10 @@ __global__ __launch_bounds__(kFarThreads) void far_kernel(Geo g, Consts cs,
                                                           TS* __restrict__ H, TS* __restrict__ G,
                                                           double* __restrict__ hpart) {
   constexpr bool kMixed = sizeof(TS) == 4;
+  // wave priority 2: the HBM-bound far field issues ahead of the FP64 near
+  // rules running beside it (priority 0), which take the cycles it leaves;
+  // the short latency-bound kernels of the pair-list chain run at 3
+  __builtin_amdgcn_s_setprio(2);
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_nx[kEB * NN], s_ny[kEB * NN],
       s_jw[kEB * NN], s_L[kEB];
   __shared__ float s_gwf[kEB * NN], s_lim2f[kEB];

@@ -249,6 +249,7 @@ __global__ __launch_bounds__(kFarThreads) void near_classify(Geo g, Consts cs, i
                                                              uint32_t* __restrict__ nearbits,
                                                              int* __restrict__ row_cnt,
                                                              uint8_t* __restrict__ tiles) {
+  __builtin_amdgcn_s_setprio(3);  // pair-list chain: ahead of the far field
   __shared__ double s_cx[kEB * NN], s_cy[kEB * NN], s_L[kEB];
   __shared__ float s_lim2f[kEB];
   __shared__ double s_red[4][kFarThreads / 64];
@@ -342,6 +343,7 @@ __global__ __launch_bounds__(kScanWaves * kWave) void near_scan(int nt,
                                                                 const int* __restrict__ cnt,
                                                                 int* __restrict__ ptr,
                                                                 int* __restrict__ host_total) {
+  __builtin_amdgcn_s_setprio(3);  // pair-list chain: ahead of the far field
   __shared__ int s_tot[kScanWaves];
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   const int seg = (nt + kScanWaves - 1) / kScanWaves;
@@ -397,6 +399,7 @@ __global__ __launch_bounds__(kFarThreads) void near_fill(int nt, int nwords, int
                                                          int* __restrict__ pelem,
                                                          int* __restrict__ sing,
                                                          int* __restrict__ reg) {
+  __builtin_amdgcn_s_setprio(3);  // pair-list chain: ahead of the far field
   const int i = blockIdx.x * kFarThreads + threadIdx.x;
   const uint8_t* tl = tiles + (size_t)blockIdx.x * nwords;
   const int nsing = max(0, min(nt, n - row0));
