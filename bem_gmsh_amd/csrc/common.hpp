@@ -100,7 +100,8 @@ struct bemhip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // second stream (default priority) for concurrent phases
-  hipStream_t stream3 = nullptr;  // third stream (default priority): the assembly's on-element rule
+  hipStream_t stream3 = nullptr;  // third stream (default priority): the assembly's on-element
+                                  // rule, the LU's block inverses
   std::string err;
   int num_cu = 0;
 

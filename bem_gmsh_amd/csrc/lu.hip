@@ -1799,8 +1799,6 @@ bool gemm_aligned(const void* A, int lda, const void* B, int ldb, int vec) {
          (ldb % vec == 0);
 }
 
-// X = Linv X in place (M <= 128 rows of X, Linv M x M with leading dimension
-// kSolveNB), one 128-row tile row. Requires 16-byte aligned operands.
 // X := L^-1 X for a unit lower 128 x 128 block L (column-major, ld) and the
 // ncols columns of X (column-major, ldx): forward substitution with L in LDS
 // and every wave on 8 columns of its own (lane l holds rows l and l + 64), the
@@ -1854,6 +1852,8 @@ __global__ __launch_bounds__(256) void trsm_unit_lower128(const float* __restric
   }
 }
 
+// X = Linv X in place (M <= 128 rows of X, Linv M x M with leading dimension
+// kSolveNB), one 128-row tile row. Requires 16-byte aligned operands.
 template <typename T>
 void gemm_set_inplace(bemhip_ctx* c, int M, int N, const T* Linv, T* X, int ldx) {
   if (M <= 0 || N <= 0) return;
