@@ -15,7 +15,7 @@ from .analytical import (AnalyticalSolution, ana_elasticity_patch, ana_exp_cos, 
                          ana_quadratic_mixed, analytical, analytical_flux, apply_analytical_bc,
                          apply_bc_from_physical, attach_analytical, rel_error, rel_error_flux)
 from .api import (ArgumentError, HipContext, adjoint, apply_bc, apply_operator, assemble,
-                  get_HG, lu_factor, lu_solve, near_pairs, set_lu_pivoting, solve, solve_Hmat,
+                  get_HG, lu_factor, lu_solve, near_pairs, set_lu_gemm, set_lu_pivoting, solve, solve_Hmat,
                   split_sol, unique_node_gradient)
 from .autodiff import FluxMisfit, objective, rrule_flux_misfit
 from .interior import (gera_p_in, interior_laplace, internal_grid, internal_grid_, internal_layer,
@@ -30,7 +30,7 @@ __all__ = [
     "analytical", "analytical_flux", "apply_analytical_bc", "apply_bc_from_physical",
     "attach_analytical", "rel_error", "rel_error_flux", "ArgumentError", "HipContext",
     "adjoint", "apply_bc", "apply_operator", "assemble", "get_HG", "lu_factor", "lu_solve",
-    "near_pairs", "set_lu_pivoting", "solve", "solve_Hmat", "split_sol", "unique_node_gradient", "gera_p_in",
+    "near_pairs", "set_lu_gemm", "set_lu_pivoting", "solve", "solve_Hmat", "split_sol", "unique_node_gradient", "gera_p_in",
     "interior_laplace", "internal_grid", "internal_grid_", "internal_layer", "internal_layer_",
     "point_in_domain", "set_internal_nodes", "FluxMisfit", "objective", "rrule_flux_misfit",
 ]

@@ -45,6 +45,7 @@ SIGNATURES = [
      [C.c_void_p, C.c_int, C.c_double, C.c_int, c_double_p, c_int_p, c_double_p]),
     ("bemhip_apply_operator", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     ("bemhip_set_lu_pivoting", C.c_int, [C.c_void_p, C.c_int]),
+    ("bemhip_set_lu_gemm", C.c_int, [C.c_void_p, C.c_int]),
     ("bemhip_lu_factor", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p]),
     ("bemhip_lu_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p,
                                   c_double_p, C.c_int]),

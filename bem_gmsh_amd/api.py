@@ -315,6 +315,19 @@ def set_lu_pivoting(ctx: HipContext, pivoting: str) -> None:
     ctx.check(lib().bemhip_set_lu_pivoting(ctx.h, PIVOTING[pivoting]))
 
 
+LU_GEMM = {"fp32": 0, "split_bf16": 1}
+
+
+def set_lu_gemm(ctx: HipContext, mode: str) -> None:
+    """Arithmetic of the FP32 LU's large trailing updates: 'fp32' (FP32 matrix
+    cores) or 'split_bf16' (the default: each FP32 operand split exactly into
+    three bf16 parts, the six partial products down to 2^-16 |a b| on the bf16
+    matrix cores with FP32 sums)."""
+    if mode not in LU_GEMM:
+        raise ArgumentError(f"mode must be 'fp32' or 'split_bf16'; got {mode!r}")
+    ctx.check(lib().bemhip_set_lu_gemm(ctx.h, LU_GEMM[mode]))
+
+
 def lu_factor(A: np.ndarray, ctx: Optional[HipContext] = None, pivoting: Optional[str] = None):
     """Dense LU on the GPU of a host matrix (float64/float32): partial pivoting in
     FP64; FP32 pivots by ``pivoting`` ('tournament' or 'partial'; default: the

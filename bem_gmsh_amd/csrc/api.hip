@@ -345,6 +345,16 @@ int bemhip_set_lu_pivoting(bemhip_ctx* c, int mode) {
   return BEMHIP_OK;
 }
 
+int bemhip_set_lu_gemm(bemhip_ctx* c, int mode) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "set_lu_gemm: NULL context");
+  if (mode != BEMHIP_LU_GEMM_FP32 && mode != BEMHIP_LU_GEMM_SPLIT_BF16)
+    return fail(c, BEMHIP_ERR_ARG,
+                "set_lu_gemm: mode must be BEMHIP_LU_GEMM_FP32 or BEMHIP_LU_GEMM_SPLIT_BF16");
+  c->lu32_gemm = mode;
+  c->factored32 = false;
+  return BEMHIP_OK;
+}
+
 int bemhip_assemble(bemhip_ctx* c, int npg, double near_factor, int precision) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "assemble: NULL context");
   if (!c->has_geo) return fail(c, BEMHIP_ERR_STATE, "assemble: set_geometry first");

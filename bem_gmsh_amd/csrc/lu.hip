@@ -1405,6 +1405,264 @@ __global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// C -= A B for FP32 operands on the bf16 matrix cores. Each operand value x is
+// split exactly into three bf16 parts, x = h + m + l: h = bf16(x) (round to
+// nearest), m = bf16(x - h), l = x - h - m (both differences are exact in FP32,
+// and l fits a bf16), so |m| <= 2^-8 |x| and |l| <= 2^-16 |x|. Of the nine
+// partial products the six of order down to 2^-16 |a b| are kept,
+//   a b ~ hh + hm + mh + hl + lh + mm,
+// and the dropped ml + lm + ll stay below 2^-23 |a b| with no bias in sign:
+// FP32's unit roundoff is 2^-24 and the matrix cores sum in FP32, so the result
+// meets the error bound of an FP32 GEMM. Per 16 x 16 x 16 block that is 3
+// v_mfma_f32_16x16x32_bf16 (16 times the flop rate of v_mfma_f32_16x16x4f32)
+// in place of 4 FP32 MFMAs: 2.67x less matrix-core time.
+//
+// The MFMA takes 8 bf16 per lane and operand: lane (l16, g) supplies row l16 at
+// k-slots 8g..8g+7, the same slots in both operands. A K-step covers 16 source
+// k: lanes g = 0, 1 carry one pair of parts and lanes g = 2, 3 another, each
+// over the k-group g & 1 (source k = 8 (g & 1) + e for element e), so each
+// instruction sums two partial products (a runs over the columns of C, b over
+// its rows, as in gemm_tile_kernel):
+//   I1: a = [B_h | B_m], b = [A_h | A_h]  ->  hh + (B_m A_h)
+//   I2: a = [B_h | B_l], b = [A_m | A_h]  ->  (B_h A_m) + (B_l A_h)
+//   I3: a = [B_h | B_m], b = [A_l | A_m]  ->  (B_h A_l) + mm
+//
+// LDS per stage and operand: [part 3][k-group 2][row 256] chunks of 8 bf16 (16
+// bytes), each (part, k-group) region padded by 128 bytes; a fragment is one
+// 16-byte read with 16 lanes on 16 consecutive rows. Global loads: A (column-
+// major, rows contiguous) as dwords, one row and one k-group per thread (a wave
+// reads 256 contiguous bytes per k); B (k contiguous) as two 16-byte loads, one
+// column and one k-group per thread. The parts are formed in registers once per
+// element and stored with 16-byte writes, double-buffered (99 KB: one
+// workgroup of 8 waves per CU). Tile order, wave tiles (64 x 128) and the
+// epilogue are those of gemm_tile_kernel<float, 256, 256, 4, 2>.
+constexpr int kSplitRows = 256;
+constexpr int kSplitRegion = kSplitRows + 8;  // 16-byte chunks per (part, k-group)
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// x0, x1 -> the packed bf16 pairs of the three parts (x0 in the low half)
+__device__ __forceinline__ void split3_pair(float x0, float x1, unsigned& h, unsigned& m,
+                                            unsigned& l) {
+  const bf16x2_t h2 = {(__bf16)x0, (__bf16)x1};
+  const float r0 = x0 - (float)h2[0], r1 = x1 - (float)h2[1];
+  const bf16x2_t m2 = {(__bf16)r0, (__bf16)r1};
+  const float s0 = r0 - (float)m2[0], s1 = r1 - (float)m2[1];
+  const bf16x2_t l2 = {(__bf16)s0, (__bf16)s1};
+  h = __builtin_bit_cast(unsigned, h2);
+  m = __builtin_bit_cast(unsigned, m2);
+  l = __builtin_bit_cast(unsigned, l2);
+}
+
+__global__ __launch_bounds__(512) void gemm_split_kernel(int M, int N, int K,
+                                                         const float* __restrict__ A, int lda,
+                                                         const float* __restrict__ B, int ldb,
+                                                         float* __restrict__ C, int ldc,
+                                                         int tiles_m, int ntiles) {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  constexpr int BM = kSplitRows, BN = kSplitRows, BK = 16, NWM = 4;
+  constexpr int WTM = 64, WTN = 128, TI = WTM / 16, TJ = WTN / 16;
+  constexpr int R = kSplitRegion;
+  // [buffer][part][k-group][row]
+  __shared__ uint4 As[2][3][2][R];
+  __shared__ uint4 Bs[2][3][2][R];
+
+  // XCD-chunked tile index, groups of 4 tile columns (as gemm_tile_kernel)
+  const int bid = blockIdx.x;
+  int tile = bid;
+  if (ntiles >= 8) {
+    const int q = ntiles >> 3, rem = ntiles & 7, x = bid & 7, j = bid >> 3;
+    tile = (x < rem) ? x * (q + 1) + j : rem * (q + 1) + (x - rem) * q + j;
+  }
+  constexpr int kGroupN = 4;
+  const int tiles_n = (ntiles + tiles_m - 1) / tiles_m;
+  const int group = tile / (tiles_m * kGroupN);
+  const int first_n = group * kGroupN;
+  const int gw = min(kGroupN, tiles_n - first_n);
+  const int gidx = tile - group * tiles_m * kGroupN;
+  const int bm = (gidx / gw) * BM, bn = (first_n + gidx % gw) * BN;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave % NWM) * WTM, wn = (wave / NWM) * WTN;
+  const int l16 = lane & 15, q4 = lane >> 4;
+  const int kg = q4 & 1, half = q4 >> 1;
+  // parts read by this lane: a-operand (B) F1 = h|m, F2 = h|l; b-operand (A)
+  // E1 = h|h, E2 = m|h, E3 = l|m (left of the bar: lanes g = 0, 1)
+  const int pF1 = half ? 1 : 0, pF2 = half ? 2 : 0;
+  const int pE2 = half ? 0 : 1, pE3 = half ? 1 : 2;
+
+  const bool interior = (bm + BM <= M) && (bn + BN <= N);
+  const int lane_col = q4 * 4;
+  float* Cl = C + (size_t)(bm + wm + l16) + (size_t)(bn + wn + lane_col) * ldc;
+  acc_t acc[TI][TJ];
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj) acc[ti][tj] = acc_t{0, 0, 0, 0};
+
+  // loader roles: A row am, k-group akg; B column bnn, k-group bkg
+  const int am = tid & (BM - 1), akg = tid >> 8;
+  const int bnn = tid >> 1, bkg = tid & 1;
+  const float* Ap = A + (size_t)(bm + am) + (size_t)(8 * akg) * lda;
+  const float* Bp = B + (size_t)(8 * bkg) + (size_t)(bn + bnn) * ldb;
+
+  auto mainloop = [&](auto chk) {
+    constexpr bool CHK = decltype(chk)::value;
+    float ra[8], rb[8];
+    auto gload = [&](int kt) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int kk = kt + 8 * akg + e;
+        if (!CHK || (kk < K && bm + am < M))
+          ra[e] = Ap[(size_t)(kt + e) * lda];
+        else
+          ra[e] = 0.f;
+      }
+      const int k0 = kt + 8 * bkg;
+      if (!CHK || (bn + bnn < N && k0 + 8 <= K)) {
+        const float4 u = *reinterpret_cast<const float4*>(Bp + kt);
+        const float4 v = *reinterpret_cast<const float4*>(Bp + kt + 4);
+        rb[0] = u.x; rb[1] = u.y; rb[2] = u.z; rb[3] = u.w;
+        rb[4] = v.x; rb[5] = v.y; rb[6] = v.z; rb[7] = v.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) rb[e] = (bn + bnn < N && k0 + e < K) ? Bp[kt + e] : 0.f;
+      }
+    };
+    // parts of 8 values (element e in bf16 slot e) into the chunk of row r
+    auto put = [&](uint4 (*X)[2][kSplitRegion], int g, int r, const unsigned (&hw)[4],
+                   const unsigned (&mw)[4], const unsigned (&lw)[4]) {
+      X[0][g][r] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+      X[1][g][r] = make_uint4(mw[0], mw[1], mw[2], mw[3]);
+      X[2][g][r] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    };
+    auto sstore = [&](int buf) {
+      unsigned hw[4], mw[4], lw[4];
+#pragma unroll
+      for (int p = 0; p < 4; ++p) split3_pair(ra[2 * p], ra[2 * p + 1], hw[p], mw[p], lw[p]);
+      put(As[buf], akg, am, hw, mw, lw);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) split3_pair(rb[2 * p], rb[2 * p + 1], hw[p], mw[p], lw[p]);
+      put(Bs[buf], bkg, bnn, hw, mw, lw);
+    };
+
+    // One K-step on buffer buf: sweep 1 (I1 and I3, sharing the a-operand F1),
+    // then sweep 2 (I2). With STORE, the parts of the next K-step (loaded into
+    // ra, rb before the step) are formed and written to buffer nbuf during
+    // sweep 2. The step is branch-free, so the scheduling groups at its end
+    // order all of it: each a-fragment read one 16-column block ahead of its
+    // MFMAs, the split's VALU work spread between sweep 2's MFMAs.
+    auto step = [&](int buf, int nbuf, auto store) {
+      constexpr bool STORE = decltype(store)::value;
+      bf16x8_t e1[TI], e3[TI], e2[TI], f[2];
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti) {
+        const int r = wm + ti * 16 + l16;
+        e1[ti] = __builtin_bit_cast(bf16x8_t, As[buf][0][kg][r]);
+        e3[ti] = __builtin_bit_cast(bf16x8_t, As[buf][pE3][kg][r]);
+      }
+      f[0] = __builtin_bit_cast(bf16x8_t, Bs[buf][pF1][kg][wn + l16]);
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj) {
+        if (tj + 1 < TJ)
+          f[(tj + 1) & 1] =
+              __builtin_bit_cast(bf16x8_t, Bs[buf][pF1][kg][wn + (tj + 1) * 16 + l16]);
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+          acc[ti][tj] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[tj & 1], e1[ti], acc[ti][tj], 0, 0, 0);
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+          acc[ti][tj] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[tj & 1], e3[ti], acc[ti][tj], 0, 0, 0);
+      }
+#pragma unroll
+      for (int ti = 0; ti < TI; ++ti)
+        e2[ti] = __builtin_bit_cast(bf16x8_t, As[buf][pE2][kg][wm + ti * 16 + l16]);
+      f[0] = __builtin_bit_cast(bf16x8_t, Bs[buf][pF2][kg][wn + l16]);
+      // sweep 2, column block tj: its 4 MFMAs, then (STORE) a quarter of one
+      // operand's split (tj < 4: A, else B) and, after the last quarter, that
+      // operand's 3 LDS writes, fenced so that each slice of VALU work issues
+      // while the block's MFMAs run
+      unsigned hw[4], mw[4], lw[4];
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj) {
+        if (tj + 1 < TJ)
+          f[(tj + 1) & 1] =
+              __builtin_bit_cast(bf16x8_t, Bs[buf][pF2][kg][wn + (tj + 1) * 16 + l16]);
+#pragma unroll
+        for (int ti = 0; ti < TI; ++ti)
+          acc[ti][tj] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(f[tj & 1], e2[ti], acc[ti][tj], 0, 0, 0);
+        if constexpr (STORE) {
+          const int p = tj & 3;
+          if (tj < 4)
+            split3_pair(ra[2 * p], ra[2 * p + 1], hw[p], mw[p], lw[p]);
+          else
+            split3_pair(rb[2 * p], rb[2 * p + 1], hw[p], mw[p], lw[p]);
+          if (tj == 3) put(As[nbuf], akg, am, hw, mw, lw);
+          if (tj == 7) put(Bs[nbuf], bkg, bnn, hw, mw, lw);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    const int nsteps = (K + BK - 1) / BK;
+    for (int st = 0; st + 1 < nsteps; ++st) {
+      gload((st + 1) * BK);
+      step(buf, buf ^ 1, std::true_type{});
+      __syncthreads();
+      buf ^= 1;
+    }
+    step(buf, 0, std::false_type{});
+  };
+  if (interior && (K % BK) == 0)
+    mainloop(std::false_type{});
+  else
+    mainloop(std::true_type{});
+
+  // C -= acc (as gemm_tile_kernel): one subtraction per entry, the next MFMA
+  // row block of C loaded while the current one is written
+  if (interior) {
+    float cv[2][TJ][4];
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) cv[0][tj][r] = Cl[(size_t)(tj * 16 + r) * ldc];
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+      if (ti + 1 < TI) {
+#pragma unroll
+        for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            cv[(ti + 1) & 1][tj][r] = Cl[(ti + 1) * 16 + (size_t)(tj * 16 + r) * ldc];
+      }
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Cl[ti * 16 + (size_t)(tj * 16 + r) * ldc] = cv[ti & 1][tj][r] - acc[ti][tj][r];
+    }
+  } else {
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+      const int i = bm + wm + ti * 16 + l16;
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = bn + wn + lane_col + tj * 16 + r;
+          if (i < M && j < N) Cl[ti * 16 + (size_t)(tj * 16 + r) * ldc] -= acc[ti][tj][r];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Triangular solves with FP64 accumulation (factors in T), blocked by kSolveNB
 // with the inverses of the diagonal blocks precomputed once per factorization.
 // ---------------------------------------------------------------------------
@@ -1880,9 +2138,15 @@ int gemm_sub(bemhip_ctx* c, int M, int N, int K, const T* A, int lda, const T* B
         hipEventCreate(&e1);
         hipEventRecord(e0, c->stream);
       }
-      if constexpr (sizeof(T) == 4)
-        launch_tile_gemm<T, 256, 256, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
-      else
+      if constexpr (sizeof(T) == 4) {
+        if (c->lu32_gemm == BEMHIP_LU_GEMM_SPLIT_BF16) {
+          const int tm = (M + kSplitRows - 1) / kSplitRows, tn = (N + kSplitRows - 1) / kSplitRows;
+          hipLaunchKernelGGL(gemm_split_kernel, dim3(tm * tn), dim3(512), 0, c->stream, M, N, K,
+                             A, lda, B, ldb, C, ldc, tm, tm * tn);
+        } else {
+          launch_tile_gemm<T, 256, 256, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
+        }
+      } else
         launch_tile_gemm<T, 256, 128, 4, 2>(c, M, N, K, A, lda, B, ldb, C, ldc);
       if (c->gemm_log) {
         hipEventRecord(e1, c->stream);

@@ -230,6 +230,20 @@ int bemhip_apply_operator(bemhip_ctx* ctx, const double* x, double* y);
 #define BEMHIP_PIVOT_TOURNAMENT 1
 int bemhip_set_lu_pivoting(bemhip_ctx* ctx, int mode);
 
+/* Arithmetic of the FP32 LU's large trailing updates (C -= A B with at least
+ * one 256 x 256 tile per CU). No counterpart in the reference, whose LU is
+ * LAPACK's FP64 getrf (src/Core/LinearSolveUtils.jl:44-49).
+ *   BEMHIP_LU_GEMM_FP32: FP32 matrix cores (v_mfma_f32_16x16x4f32)
+ *   BEMHIP_LU_GEMM_SPLIT_BF16 (default): every FP32 operand split exactly into three
+ *     bf16 parts (x = h + m + l), the six partial products down to 2^-16 |a b|
+ *     on the bf16 matrix cores with FP32 sums; the dropped terms stay below
+ *     2^-23 |a b| per product (FP32's unit roundoff is 2^-24), at 2.67x less
+ *     matrix-core time
+ * Discards FP32 factors held by the context. */
+#define BEMHIP_LU_GEMM_FP32 0
+#define BEMHIP_LU_GEMM_SPLIT_BF16 1
+int bemhip_set_lu_gemm(bemhip_ctx* ctx, int mode);
+
 /* Standalone dense LU of a host matrix (column-major n x n), the LU seam of
  * bem_linsolve (src/Core/LinearSolveUtils.jl:44-49): partial pivoting in FP64,
  * FP32 pivoting as set by bemhip_set_lu_pivoting.

@@ -69,6 +69,8 @@ def test_argument_errors_before_device_work():
     assert L.bemhip_solve(None, 0, 1e-12, 30, None, None, None) == 1
     assert L.bemhip_adjoint(None, None, None, None, None) == 1
     assert L.bemhip_lu_factor(None, 4, 0, None, None) == 1
+    assert L.bemhip_set_lu_gemm(None, 0) == 1
+    assert L.bemhip_set_lu_pivoting(None, 0) == 1
     assert L.bemhip_create(None, 0) == 1
     del dp
 

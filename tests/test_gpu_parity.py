@@ -259,3 +259,23 @@ def test_tournament_tree(n):
     winners = sets[0]
     perm = _perm_of(piv)
     assert list(perm[:128]) == winners
+
+
+@pytest.mark.parametrize("n", [6149])
+def test_lu_split_gemm_matches_fp32_gemm(n):
+    """The FP32 LU with its large trailing updates on the bf16 matrix cores
+    (each operand split exactly into three bf16 parts, six partial products)
+    against the same LU with FP32 matrix-core updates, on one matrix: the split
+    drops terms below 2^-23 |a b| per product, so the backward error stays at
+    the FP32 level. A lost or doubled partial product (error 2^-16 |a b| per
+    product or more) would raise it many times over."""
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    ctx = B.HipContext(0)
+    rec = {}
+    for mode in ["fp32", "split_bf16"]:
+        B.set_lu_gemm(ctx, mode)
+        LU, piv = B.lu_factor(A, ctx=ctx)
+        rec[mode], _ = _check_lu(A, LU, piv, np.float32, 1.0 + 1e-6)
+    print(f"\n[lu n={n}] |PA-LU|/|A| fp32 GEMM {rec['fp32']:.3e}, split-bf16 {rec['split_bf16']:.3e}")
+    assert rec["split_bf16"] < 1.25 * rec["fp32"], rec
