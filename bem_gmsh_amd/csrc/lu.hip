@@ -1411,7 +1411,8 @@ __global__ __launch_bounds__(NWM * NWN * 64) void gemm_tile_kernel(
 // and l fits a bf16), so |m| <= 2^-8 |x| and |l| <= 2^-16 |x|. Of the nine
 // partial products the six of order down to 2^-16 |a b| are kept,
 //   a b ~ hh + hm + mh + hl + lh + mm,
-// and the dropped ml + lm + ll stay below 2^-23 |a b| with no bias in sign:
+// and the dropped ml + lm + ll stay within 2^-23 (1 + 2^-9) |a b| (each term is
+// at most 2^-8 |m_a m_b| <= 2^-24 |a b|) with no bias in sign:
 // FP32's unit roundoff is 2^-24 and the matrix cores sum in FP32, so the result
 // meets the error bound of an FP32 GEMM. Per 16 x 16 x 16 block that is 3
 // v_mfma_f32_16x16x32_bf16 (16 times the flop rate of v_mfma_f32_16x16x4f32)

@@ -236,8 +236,8 @@ int bemhip_set_lu_pivoting(bemhip_ctx* ctx, int mode);
  *   BEMHIP_LU_GEMM_FP32: FP32 matrix cores (v_mfma_f32_16x16x4f32)
  *   BEMHIP_LU_GEMM_SPLIT_BF16 (default): every FP32 operand split exactly into three
  *     bf16 parts (x = h + m + l), the six partial products down to 2^-16 |a b|
- *     on the bf16 matrix cores with FP32 sums; the dropped terms stay below
- *     2^-23 |a b| per product (FP32's unit roundoff is 2^-24), at 2.67x less
+ *     on the bf16 matrix cores with FP32 sums; the dropped terms stay within
+ *     2^-23 (1 + 2^-9) |a b| per product (FP32's unit roundoff is 2^-24), at 2.67x less
  *     matrix-core time
  * Discards FP32 factors held by the context. */
 #define BEMHIP_LU_GEMM_FP32 0

@@ -266,7 +266,7 @@ def test_lu_split_gemm_matches_fp32_gemm(n):
     """The FP32 LU with its large trailing updates on the bf16 matrix cores
     (each operand split exactly into three bf16 parts, six partial products)
     against the same LU with FP32 matrix-core updates, on one matrix: the split
-    drops terms below 2^-23 |a b| per product, so the backward error stays at
+    drops terms within 2^-23 (1 + 2^-9) |a b| per product, so the backward error stays at
     the FP32 level. A lost or doubled partial product (error 2^-16 |a b| per
     product or more) would raise it many times over."""
     rng = np.random.default_rng(11)
