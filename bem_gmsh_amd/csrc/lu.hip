@@ -2111,6 +2111,68 @@ __global__ __launch_bounds__(256) void trsm_unit_lower128(const float* __restric
   }
 }
 
+// X := L11^-1 X for the columns of X (ob <= 512 rows, column-major, ldx), L11
+// the unit lower ob x ob diagonal block of the factor (leading dimension lda)
+// whose 128-blocks' inverses are linv (kSolveNB x kSolveNB each, leading
+// dimension kSolveNB, zero above the diagonal): by 128-blocks b,
+// Y = X_b - L[b, <b] X_<b (the product summed, then subtracted once), then
+// X_b = Linv_b Y, the solved rows kept in LDS. One workgroup per 8 columns;
+// thread (row i of the 128-block, column group cg) holds 4 columns. The
+// look-ahead TRSM of the next block's columns in one launch.
+constexpr int kTrsmInvCols = 8;
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_inv_strip(int ob, int nc, const T* __restrict__ L,
+                                                      int lda, const T* __restrict__ linv,
+                                                      T* __restrict__ X, int ldx) {
+  constexpr int CPT = kTrsmInvCols / 2;
+  __shared__ __align__(16) T Xs[kOuterBlocks * kSolveNB][kTrsmInvCols];
+  __shared__ __align__(16) T Ys[kSolveNB][kTrsmInvCols];
+  const int tid = threadIdx.x, i = tid & (kSolveNB - 1), cg = tid >> 7;
+  const int j0 = blockIdx.x * kTrsmInvCols;
+  for (int jj = 0; jj < kTrsmInvCols; ++jj) {
+    const bool in = j0 + jj < nc;
+    for (int r = tid; r < ob; r += 256) Xs[r][jj] = in ? X[(size_t)r + (size_t)(j0 + jj) * ldx] : T(0);
+  }
+  __syncthreads();
+  for (int b0 = 0; b0 < ob; b0 += kSolveNB) {
+    const int bw = min(kSolveNB, ob - b0);
+    const bool row = i < bw;
+    T acc[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) acc[q] = T(0);
+    const T* Lr = L + (size_t)(b0 + (row ? i : 0));
+#pragma unroll 8
+    for (int k = 0; k < b0; ++k) {
+      const T a = row ? Lr[(size_t)k * lda] : T(0);
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) acc[q] += a * Xs[k][cg * CPT + q];
+    }
+    if (row) {
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) Ys[i][cg * CPT + q] = Xs[b0 + i][cg * CPT + q] - acc[q];
+    }
+    __syncthreads();
+    const T* Lb = linv + (size_t)(b0 / kSolveNB) * kSolveNB * kSolveNB + (row ? i : 0);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) acc[q] = T(0);
+#pragma unroll 8
+    for (int k = 0; k < bw; ++k) {
+      const T a = row ? Lb[(size_t)k * kSolveNB] : T(0);
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) acc[q] += a * Ys[k][cg * CPT + q];
+    }
+    if (row) {
+#pragma unroll
+      for (int q = 0; q < CPT; ++q) Xs[b0 + i][cg * CPT + q] = acc[q];
+    }
+    __syncthreads();
+  }
+  for (int jj = 0; jj < kTrsmInvCols; ++jj) {
+    if (j0 + jj >= nc) break;
+    for (int r = tid; r < ob; r += 256) X[(size_t)r + (size_t)(j0 + jj) * ldx] = Xs[r][jj];
+  }
+}
+
 // X = Linv X in place (M <= 128 rows of X, Linv M x M with leading dimension
 // kSolveNB), one 128-row tile row. Requires 16-byte aligned operands.
 template <typename T>
@@ -2440,7 +2502,11 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     hipEvent_t e0 = ev_now(sA);
     T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
     T* X = dA + (size_t)k0 + (size_t)c0 * lda;
-    if (aligned) {
+    if (aligned && nc <= kOuterBlocks * kSolveNB) {
+      // the next block's columns (on the panel chain): one launch
+      hipLaunchKernelGGL((trsm_inv_strip<T>), dim3((nc + kTrsmInvCols - 1) / kTrsmInvCols),
+                         dim3(256), 0, sA, ob, nc, (const T*)L11, lda, linv, X, lda);
+    } else if (aligned) {
       // by 128-blocks b: X_b -= L[b, <b] X_<b, then X_b = Linv_b X_b
       for (int b0 = 0; b0 < ob; b0 += kSolveNB) {
         const int bw = std::min(kSolveNB, ob - b0);
@@ -2500,9 +2566,11 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     // meanwhile the remaining trailing columns
     if ((st = outer_trsm(k0, ob, k2, n))) return st;
     if ((st = outer_gemm(k0, ob, k2, n))) return st;
+    // block k1's interchanges need only its factorization; its TRSM (the
+    // next iteration) needs the inverses too
     BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sA, factored, 0));
-    BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sA, inverted, 0));
     if ((st = outer_perms(k1, ob1))) return st;
+    BEMHIP_HIP_TRY(c, hipStreamWaitEvent(sA, inverted, 0));
   }
   BEMHIP_HIP_TRY(c, hipStreamSynchronize(sA));
   double tcat[4] = {0, 0, 0, 0};
