@@ -46,6 +46,7 @@ SIGNATURES = [
     ("bemhip_apply_operator", C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     ("bemhip_set_lu_pivoting", C.c_int, [C.c_void_p, C.c_int]),
     ("bemhip_set_lu_gemm", C.c_int, [C.c_void_p, C.c_int]),
+    ("bemhip_set_lu_timers", C.c_int, [C.c_void_p, C.c_int]),
     ("bemhip_lu_factor", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p]),
     ("bemhip_lu_solve", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, c_int_p,
                                   c_double_p, C.c_int]),

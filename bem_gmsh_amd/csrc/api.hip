@@ -345,6 +345,12 @@ int bemhip_set_lu_pivoting(bemhip_ctx* c, int mode) {
   return BEMHIP_OK;
 }
 
+int bemhip_set_lu_timers(bemhip_ctx* c, int on) {
+  if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "set_lu_timers: NULL context");
+  c->lu_timers = on != 0;
+  return BEMHIP_OK;
+}
+
 int bemhip_set_lu_gemm(bemhip_ctx* c, int mode) {
   if (!c) return fail(nullptr, BEMHIP_ERR_ARG, "set_lu_gemm: NULL context");
   if (mode != BEMHIP_LU_GEMM_FP32 && mode != BEMHIP_LU_GEMM_SPLIT_BF16)

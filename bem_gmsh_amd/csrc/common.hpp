@@ -186,6 +186,7 @@ struct bemhip_ctx {
   unsigned panel_epoch = 0;           // tag base of the LU panel exchange words
   int lu32_pivot = BEMHIP_PIVOT_PARTIAL;  // FP32 panels: partial or tournament pivoting
   int lu32_gemm = BEMHIP_LU_GEMM_SPLIT_BF16;  // FP32 trailing updates: split-bf16 or FP32 MFMA
+  bool lu_timers = false;  // HIP events around the LU's phases (the lu32_*/lu64_* phase counters)
   std::vector<bemhip::GemmTiming>* gemm_log = nullptr;  // set during an LU: large-tile GEMM timings
   bemhip::DevBuf<char> dinv;          // inverses of the diagonal blocks of L and U
   const void* dinv_of = nullptr;      // factor the inverses belong to

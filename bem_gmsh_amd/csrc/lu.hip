@@ -2342,8 +2342,12 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     hipEventRecord(e, st);
     return e;
   };
+  // phase timing events (off by default: each event record costs the panel
+  // chain time, ~10 ms over an LU at n = 32769)
+  const bool timers = c->lu_timers;
+  auto tev_now = [&](hipStream_t st) -> hipEvent_t { return timers ? ev_now(st) : nullptr; };
   auto seg_end = [&](hipStream_t st, hipEvent_t a0, int category) {
-    segs.push_back({a0, ev_now(st), category});
+    if (a0) segs.push_back({a0, ev_now(st), category});
   };
   // the helpers (gemm_sub, trsm_rec, ...) enqueue on c->stream
   struct StreamScope {
@@ -2366,7 +2370,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
       const int kb = std::min(KB, ob - s);
       const int m = n - kc;
       const int G = (m + kPR - 1) / kPR;
-      hipEvent_t e0 = ev_now(st);
+      hipEvent_t e0 = tev_now(st);
       bool tournament = false;
       if constexpr (sizeof(T) == 4) {
         if (tslu) {
@@ -2389,7 +2393,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
                              (const float*)t_top);
           BEMHIP_HIP_TRY(c, hipGetLastError());
           seg_end(st, e0, kPanel);
-          e0 = ev_now(st);
+          e0 = tev_now(st);
         }
       }
       if (!tournament) {
@@ -2406,7 +2410,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
         BEMHIP_HIP_TRY(c, hipLaunchCooperativeKernel((const void*)panel_kernel<T, KB>, dim3(G),
                                                      dim3(kPR), args, 0, st));
         seg_end(st, e0, kPanel);
-        e0 = ev_now(st);
+        e0 = tev_now(st);
         const int ncols_blk = ob - kb;  // (the panel kernel built the gather list)
         if (ncols_blk > 0)
           hipLaunchKernelGGL((perm_apply<T>), dim3((ncols_blk + 3) / 4), dim3(256), 0, st, dA, lda,
@@ -2414,7 +2418,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
                              (const int*)l_len(q));
         BEMHIP_HIP_TRY(c, hipGetLastError());
         seg_end(st, e0, kSwap);
-        e0 = ev_now(st);
+        e0 = tev_now(st);
       }
       // inverse of each completed 128-block of this outer block's L11
       if (aligned && ((s + kb) % kSolveNB == 0 || s + kb == ob)) {
@@ -2452,7 +2456,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
                                dim3(256), 0, st, (const float*)L11, lda, (float*)A12, lda, rest);
             BEMHIP_HIP_TRY(c, hipGetLastError());
             seg_end(st, e0, kTrsm);
-            e0 = ev_now(st);
+            e0 = tev_now(st);
             T* L21 = dA + (size_t)(kc + kb) + (size_t)kc * lda;
             T* A22 = dA + (size_t)(kc + kb) + (size_t)(kc + kb) * lda;
             gemm_sub(c, n - kc - kb, rest, kb, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
@@ -2470,7 +2474,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
           if (stt) return stt;
         }
         seg_end(st, e0, kTrsm);
-        e0 = ev_now(st);
+        e0 = tev_now(st);
         T* L21 = dA + (size_t)(kc + kb) + (size_t)kc * lda;
         T* A22 = dA + (size_t)(kc + kb) + (size_t)(kc + kb) * lda;
         gemm_sub(c, n - kc - kb, rest, kb, (const T*)L21, lda, (const T*)A12, lda, A22, lda);
@@ -2484,7 +2488,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   };
   // The block's interchanges on the columns outside it (stream A).
   auto outer_perms = [&](int k0, int ob) -> int {
-    hipEvent_t e0 = ev_now(sA);
+    hipEvent_t e0 = tev_now(sA);
     const int ncols = n - ob;
     for (int s = 0, q = 0; s < ob && ncols > 0; s += KB, ++q)
       hipLaunchKernelGGL((perm_apply<T>), dim3((ncols + 3) / 4), dim3(256), 0, sA, dA, lda, 0, k0,
@@ -2499,7 +2503,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
     const int nc = c1 - c0;
     if (nc <= 0) return BEMHIP_OK;
     const T* linv = linv_of(k0);
-    hipEvent_t e0 = ev_now(sA);
+    hipEvent_t e0 = tev_now(sA);
     T* L11 = dA + (size_t)k0 + (size_t)k0 * lda;
     T* X = dA + (size_t)k0 + (size_t)c0 * lda;
     if (aligned && nc <= kOuterBlocks * kSolveNB) {
@@ -2527,7 +2531,7 @@ int lu_factor_t(bemhip_ctx* c, int n, T* dA, int lda, int* dipiv, int* info_host
   // A[k0+ob.., c0..c1) -= L21 U12[:, c0..c1) (stream A).
   auto outer_gemm = [&](int k0, int ob, int c0, int c1) -> int {
     if (c1 <= c0 || n - k0 - ob <= 0) return BEMHIP_OK;
-    hipEvent_t e0 = ev_now(sA);
+    hipEvent_t e0 = tev_now(sA);
     gemm_sub(c, n - k0 - ob, c1 - c0, ob, (const T*)(dA + (size_t)(k0 + ob) + (size_t)k0 * lda),
              lda, (const T*)(dA + (size_t)k0 + (size_t)c0 * lda), lda,
              dA + (size_t)(k0 + ob) + (size_t)c0 * lda, lda);

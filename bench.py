@@ -627,6 +627,15 @@ def main():
             f"relres {rel:.2e}")
     t_total = time.perf_counter() - t_start
     counters = ctx.counters()
+    # the LU's phase breakdown from one more, untimed, pipeline with the phase
+    # timers on (their event records lengthen the LU, so the timed steps run
+    # without them)
+    ctx.chk(ctx.L.bemhip_set_lu_timers(ctx.h, 1))
+    ctx.pipeline(dad, args.npg, 1, 1)
+    ph = ctx.counters()["ms"]
+    ctx.chk(ctx.L.bemhip_set_lu_timers(ctx.h, 0))
+    lu_phases = {k: ph[k] for k in ("lu32", "lu32_panel", "lu32_swap", "lu32_trsm", "lu32_gemm",
+                                    "lu32_big_gemm") if k in ph}
     ms_step = 1e3 * t_total / args.steps
     asm_avg = float(np.mean(asm_ms))
     entries = 2.0 * nt * n
@@ -638,7 +647,7 @@ def main():
               "assemble_ms": asm_avg, "far_kernel_ms": float(np.mean(far_ms)),
               "lu32_ms": float(np.mean(lu_ms)), "refine_ms": float(np.mean(ref_ms)),
               "refine_iters": iters, "solve_mixed_s": solve_mixed_s,
-              "counters": counters}
+              "counters": counters, "lu32_phases_ms_timers_on": lu_phases}
 
     # all-FP64 pipeline once: speed-up and accuracy reference
     if not args.no_fp64:

@@ -244,6 +244,14 @@ int bemhip_set_lu_pivoting(bemhip_ctx* ctx, int mode);
 #define BEMHIP_LU_GEMM_SPLIT_BF16 1
 int bemhip_set_lu_gemm(bemhip_ctx* ctx, int mode);
 
+/* Phase timers of the LU (no counterpart in the reference): on != 0 records
+ * HIP events around its panels, interchanges, triangular solves and updates,
+ * reported by bemhip_counters as lu32_panel / lu32_swap / lu32_trsm /
+ * lu32_gemm (lu64_* for FP64). Off by default: the event records lengthen
+ * the panel chain (about 10 ms of a 280 ms FP32 LU at n = 32769). The
+ * large-update timings (lu32_big_gemm) are always recorded. */
+int bemhip_set_lu_timers(bemhip_ctx* ctx, int on);
+
 /* Standalone dense LU of a host matrix (column-major n x n), the LU seam of
  * bem_linsolve (src/Core/LinearSolveUtils.jl:44-49): partial pivoting in FP64,
  * FP32 pivoting as set by bemhip_set_lu_pivoting.

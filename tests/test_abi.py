@@ -70,6 +70,7 @@ def test_argument_errors_before_device_work():
     assert L.bemhip_adjoint(None, None, None, None, None) == 1
     assert L.bemhip_lu_factor(None, 4, 0, None, None) == 1
     assert L.bemhip_set_lu_gemm(None, 0) == 1
+    assert L.bemhip_set_lu_timers(None, 1) == 1
     assert L.bemhip_set_lu_pivoting(None, 0) == 1
     assert L.bemhip_create(None, 0) == 1
     del dp
